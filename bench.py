@@ -1,0 +1,172 @@
+#!/usr/bin/env python3
+"""Benchmark of the pbf simulation step on one MI355X.
+
+A step is one SPH::Run (src/SPH.cpp:246-334) over the whole scene: predict,
+sort, cell table, I x (lambda, position update), update, XSPH/vorticity.
+Default workload: 1,048,576 particles as a 128x64x128 lattice block at 0.5h
+(seeded +-0.01h jitter), grid 256x128x256, 4 solver iterations, dt 0.008,
+vorticity confinement and XSPH on; inputs are resident in HBM before timing.
+
+Prints one JSON line on stdout (everything else goes to stderr):
+  value        particle-steps/s over the timed steps (whole job, 1 GPU)
+  roofline     the dominant kernel: algorithmic bytes per launch / its mean
+               launch time (HIP events on the library's stream), against 8 TB/s
+  cpu_baseline the oracle (C restatement, OpenMP) timed on this host for a
+               bounded sample of the same workload
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "particle-steps/s @1M (4 iters), % HBM roofline; batched & adjoint step time"
+
+# Algorithmic HBM bytes per particle of each kernel launch: every stream read and
+# written once, neighbour gathers counted as cache reuse (FP32 float4 records,
+# u32 keys and indices). lambda: record 16 in + lambda 4 out; updatepos: record
+# 16 + lambda 4 in, record 16 out; vorticity passes: 56 each (112 for both).
+ALG_BYTES = {
+    "lambda": 20.0,
+    "updatepos": 36.0,
+    "vorticity_a": 56.0,
+    "vorticity_b": 56.0,
+    "predict": 40.0,
+    "update": 48.0,
+    "neighbourcells": 12.0,
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload(name):
+    from pbf_amd import scene as S
+    return {"block_1m": S.cfg_block_1m, "dam_32k": S.cfg_dam_32k, "stress_4m": S.cfg_stress_4m}[name]()
+
+
+def step_alg_bytes(scene) -> float:
+    """B_alg = 296 + 20 P8 + 56 I + 4 C/N bytes per particle-step."""
+    from pbf_amd.scene import key_bits
+    p8 = (key_bits(scene.grid) + 7) // 8
+    cells = scene.grid[0] * scene.grid[1] * scene.grid[2]
+    return 296.0 + 20.0 * p8 + 56.0 * scene.params.iterations + 4.0 * cells / scene.n
+
+
+def cpu_baseline(scene, steps: int):
+    import oracle as O
+    cfg = O.cfg_for(scene)
+    pos, vel = scene.pos, scene.vel
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        pos, vel, _, _ = O.step(cfg, pos, vel, None, None, O.ORDER_JACOBI)
+    dt = time.perf_counter() - t0
+    return {"value": scene.n * steps / dt, "unit": "particle-steps/s", "cores": O.num_threads(), "kind": "port",
+            "sample": f"{steps} step(s) of the {scene.name} workload ({scene.n} particles, "
+                      f"{scene.params.iterations} iterations), oracle/pbf_oracle.c FP32 race-free order, "
+                      f"{O.num_threads()} OpenMP threads, {dt:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="block_1m", choices=["block_1m", "dam_32k", "stress_4m"])
+    ap.add_argument("--kernel-steps", type=int, default=10, help="steps timed per kernel after the main run")
+    ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps of the CPU baseline sample (0: skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        log("bench.py measures one GPU only: this build has no multi-GPU path (scenes would shard "
+            "by slabs or by scene; see DESIGN.md). Run with --gpus 1.")
+        sys.exit(2)
+
+    import torch
+    from pbf_amd import sph as P
+
+    scene = workload(args.workload)
+    n = scene.n
+    log(f"[bench] workload {scene.name}: n={n} grid={scene.grid} iterations={scene.params.iterations} "
+        f"dt={scene.params.timestep}")
+    sph = P.SPH(n, scene.grid)
+    sph.set_params(P.scene_params(scene))
+    sph.upload(scene.pos, scene.vel)
+    torch.cuda.init()
+    for _ in range(args.warmup):
+        sph.Run()
+    P._lib.check(P._lib.load().pbf_synchronize(sph.stream()))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sph.Run()
+    P._lib.check(P._lib.load().pbf_synchronize(sph.stream()))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    value = n * args.steps / elapsed
+    ms_per_step = 1e3 * elapsed / args.steps
+    log(f"[bench] {args.steps} steps in {elapsed:.4f} s: {ms_per_step:.4f} ms/step, {value:.4e} particle-steps/s")
+    phases = sph.timing()
+    log("[bench] last step phases (ms): " + ", ".join(f"{k} {v:.4f}" for k, v in phases.items()))
+
+    # per-kernel durations, HIP events on the library's stream
+    sph.set_kernel_timing(True)
+    for _ in range(args.kernel_steps):
+        sph.Run()
+    kt = sph.kernel_times()
+    sph.set_kernel_timing(False)
+    for name, (ms, cnt) in sorted(kt.items(), key=lambda kv: -kv[1][0]):
+        if cnt:
+            log(f"[bench] kernel {name:16s} {ms / cnt * 1e3:10.2f} us/launch x {cnt // args.kernel_steps} per step")
+    dom = max((k for k in kt if k in ALG_BYTES and kt[k][1]), key=lambda k: kt[k][0])
+    dom_ms = kt[dom][0] / kt[dom][1]
+    alg = ALG_BYTES[dom] * n
+    achieved = alg / (dom_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dom,
+                "launch_us": round(dom_ms * 1e3, 2), "alg_bytes_per_launch": int(alg)}
+    b_alg = step_alg_bytes(scene)
+    step_frac = b_alg * value / (HBM_PEAK_GBS * 1e9)
+
+    cpu = None
+    if args.cpu_steps > 0:
+        log(f"[bench] CPU baseline: {args.cpu_steps} oracle step(s) ...")
+        cpu = cpu_baseline(scene, args.cpu_steps)
+        log(f"[bench] CPU baseline {cpu['value']:.4e} particle-steps/s on {cpu['cores']} threads")
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{scene.name}: {n} particles, 128x64x128 lattice block at 0.5h, grid "
+                               f"{scene.grid[0]}x{scene.grid[1]}x{scene.grid[2]}, {scene.params.iterations} "
+                               f"iterations, dt {scene.params.timestep}, XSPH + vorticity on"
+                   if scene.name == "block_1m" else scene.name,
+                   "particles": n, "grid": list(scene.grid), "iterations": scene.params.iterations,
+                   "parallelism": "single GPU"},
+        "step_roofline": {"alg_bytes_per_particle_step": b_alg, "frac": round(step_frac, 5)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    sph.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,213 @@
+/* pbf_amd: C ABI of the MI355X (gfx950) simulation step of ekpyron/pbf.
+ *
+ * Drop-in boundary for the reference's three step classes. Every entry point
+ * names the reference interface it replaces (paths relative to the reference
+ * checkout). Conventions:
+ *   - every function returns PBF_OK (0) or a negative pbf_status; none throws;
+ *     pbf_last_error() returns the thread's last message;
+ *   - handles own their device memory; pointers they return are borrowed and
+ *     stay valid until the handle's next run/step (as RadixSort::GetBuffer,
+ *     src/RadixSort.h:45-53) or its destruction;
+ *   - "dev" pointers are HIP device pointers, "host" pointers are host memory;
+ *   - work is enqueued on the handle's stream (pbf_*_set_stream; default: a
+ *     stream the handle creates), in order, asynchronously to the host unless
+ *     a function says it synchronizes;
+ *   - one host thread per handle at a time; distinct handles are independent.
+ *
+ * Arithmetic: FP32 with the reference shaders' operation order and IEEE
+ * rounding, no fused multiply-add (the library is built with
+ * -ffp-contract=off); keys, sort permutation and neighbour tables are integer
+ * exact. Solver passes are race-free (Jacobi): the reference updates positions
+ * in place while neighbours read them (updatepos.glsl:53-54, :104).
+ */
+#ifndef PBF_H
+#define PBF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum pbf_status {
+    PBF_OK = 0,
+    PBF_ERR_INVALID = -1,   /* bad argument (size, grid, null pointer) */
+    PBF_ERR_HIP = -2,       /* a HIP runtime call failed */
+    PBF_ERR_NOMEM = -3,     /* device allocation failed */
+    PBF_ERR_LOGIC = -4,     /* e.g. an odd sort block size, src/RadixSort.cpp:41-42 */
+    PBF_ERR_NODEVICE = -5   /* no HIP device */
+} pbf_status;
+
+/* Solver parameters: the eight floats of the reference's uniform block
+ * (sphparams_t, src/SPH.h:252-285, defaults src/SPH.cpp:137-144), the host
+ * fields of SPH (iterations src/SPH.h:197, vorticity flag :220, external force
+ * src/SPH.cpp:242-244), the header constant h (src/SPH.cpp:58), and the wall
+ * and gravity direction that the reference hard-codes (updatepos.glsl:98-100,
+ * predictpos.glsl:32). pbf_params_default() returns the reference's values. */
+typedef struct pbf_params {
+    float one_over_rho_0;
+    float epsilon;
+    float gravity;
+    float timestep;
+    float tensile_instability_k;
+    float tensile_instability_scale;
+    float xsph_viscosity_c;
+    float vorticity_epsilon;
+    uint32_t iterations;
+    int32_t vorticity;
+    int32_t extforce;
+    float h;
+    float wall_lo[3];    /* clamp lower bound; reference (16, 0, 16) */
+    float wall_hi[3];    /* clamp upper bound; reference GRID_SIZE - (16, 0, 16) */
+    float gravity_dir[3];/* reference (0, -1, 0) */
+} pbf_params;
+
+/* Five phases of SPH::OutputTiming (src/SPH.cpp:218-240), milliseconds. */
+typedef struct pbf_timing {
+    double predict_ms;
+    double sort_ms;
+    double neighbour_cells_ms;
+    double solver_ms;
+    double vorticity_ms; /* update + XSPH/vorticity, as the reference's last query */
+} pbf_timing;
+
+typedef struct pbf_sph pbf_sph;
+typedef struct pbf_radixsort pbf_radixsort;
+typedef struct pbf_ncf pbf_ncf;
+
+/* ---------------------------------------------------------------- library */
+const char *pbf_last_error(void);
+const char *pbf_version(void);
+int pbf_device_count(void);
+/* Copy between host and device memory on a stream (kind: 0 host->device,
+ * 1 device->host, 2 device->device) and wait for it; for callers that hold
+ * the library's device pointers but no HIP binding of their own. */
+int pbf_memcpy(void *dst, const void *src, size_t bytes, int kind, void *stream);
+int pbf_synchronize(void *stream);
+/* Reference defaults for a grid: src/SPH.cpp:137-144 plus h = 2 (:58), wall
+ * (16,0,16) (updatepos.glsl:98), gravity (0,-1,0) (predictpos.glsl:32). */
+void pbf_params_default(pbf_params *out, int32_t gx, int32_t gy, int32_t gz);
+/* SPH::Wpoly6, src/SPH.cpp:159-164. */
+float pbf_wpoly6(float r, float h);
+/* Bits the reference's sort orders by for a grid: 2*ceil(count_sortbits(C-1)/2),
+ * src/RadixSort.cpp:24-30, 44, 127. */
+int pbf_key_bits(int32_t gx, int32_t gy, int32_t gz);
+
+/* ---------------------------------------------------------------- SPH */
+/* SPH::SPH (src/SPH.h:40, src/SPH.cpp:24-151). numparticles must be a positive
+ * multiple of 512, as the reference's dispatch sizes require (src/Simulation.cpp:202). */
+int pbf_sph_create(pbf_sph **out, uint32_t numparticles, int32_t gx, int32_t gy, int32_t gz);
+/* SPH::~SPH (src/SPH.cpp:153-157). */
+int pbf_sph_destroy(pbf_sph *s);
+/* SPH::Set* / UploadSPHParams (src/SPH.cpp:166-216), SetNumSolverIterations
+ * (src/SPH.h:197), SetVorticityConfinementEnabled (:220), SetExternalForce
+ * (src/SPH.cpp:242-244), in one call. */
+int pbf_sph_set_params(pbf_sph *s, const pbf_params *p);
+/* SPH::Get* (src/SPH.h:57-222). */
+int pbf_sph_get_params(const pbf_sph *s, pbf_params *out);
+/* Replaces Simulation::ResetParticleBuffer's glCopyBufferSubData into the
+ * position and velocity buffers (src/Simulation.cpp:248-268): float4 per
+ * particle in id order (w ignored, stored 0). host != 0: host pointers. */
+int pbf_sph_upload_state(pbf_sph *s, const float *pos4, const float *vel4, int host);
+/* Reads the float4 id-order state back (synchronizes the handle's stream when host != 0). */
+int pbf_sph_download_state(pbf_sph *s, float *pos4, float *vel4, int host);
+/* SPH::Run (src/SPH.cpp:246-334): one step. */
+int pbf_sph_run(pbf_sph *s);
+/* nsteps consecutive SPH::Run calls. */
+int pbf_sph_step(pbf_sph *s, int32_t nsteps);
+/* SPH::GetPositionBuffer (src/SPH.h:49-51): float4 positions in id order,
+ * stride 16 bytes, as PointSprite::SetPositionBuffer consumes them
+ * (src/PointSprite.cpp:64-73). */
+int pbf_sph_positions_device(pbf_sph *s, void **dev, size_t *stride);
+/* SPH::GetVelocityBuffer (src/SPH.h:181-183). */
+int pbf_sph_velocities_device(pbf_sph *s, void **dev, size_t *stride);
+/* SPH::OutputTiming (src/SPH.cpp:218-240): phase times of the last step
+ * (synchronizes the handle's stream). */
+int pbf_sph_timing(pbf_sph *s, pbf_timing *out);
+/* Per-kernel HIP-event timing of the handle's launches (no reference
+ * counterpart; finer than OutputTiming's five phases). While enabled, every
+ * kernel launch of pbf_sph_run is bracketed by two events on the handle's
+ * stream; pbf_sph_kernel_times synchronizes, and returns and resets per-kernel
+ * totals. Kernel ids: see pbf_kernel_name. */
+#define PBF_NUM_KERNEL_IDS 11
+typedef struct pbf_kernel_time {
+    double total_ms;
+    uint32_t launches;
+} pbf_kernel_time;
+int pbf_sph_set_kernel_timing(pbf_sph *s, int enable);
+int pbf_sph_kernel_times(pbf_sph *s, pbf_kernel_time out[PBF_NUM_KERNEL_IDS]);
+const char *pbf_kernel_name(int id);
+/* Stream of the handle (hipStream_t); NULL restores the handle's own stream. */
+int pbf_sph_set_stream(pbf_sph *s, void *stream);
+void *pbf_sph_get_stream(const pbf_sph *s);
+/* Persistent end table of the step's NeighbourCellFinder (never cleared,
+ * src/NeighbourCellFinder.cpp:113-126): read into or write from int32[gx*gy*gz]. */
+int pbf_sph_get_end_table(pbf_sph *s, int32_t *dst, int host);
+int pbf_sph_set_end_table(pbf_sph *s, const int32_t *src, int host);
+/* Neighbour table of the last step in the reference's layout, 3 ivec4 per
+ * particle in sorted order (NeighbourCellFinder::GetResult, src/NeighbourCellFinder.h:58). */
+int pbf_sph_neighbour_table(pbf_sph *s, int32_t *dst12, int host);
+/* Sorted particle records of the last step, {x, y, z, id bits} per particle
+ * (RadixSort::GetBuffer after SPH::Run, src/RadixSort.cpp:118-122). */
+int pbf_sph_sorted_records(pbf_sph *s, float *dst4, int host);
+
+/* ---------------------------------------------------------------- RadixSort */
+/* RadixSort::RadixSort (src/RadixSort.h:41, src/RadixSort.cpp:32-109): sorts
+ * blocksize*numblocks float4 records {x, y, z, id bits} by the grid-cell key of
+ * their position. An odd blocksize fails with PBF_ERR_LOGIC like the reference. */
+int pbf_radixsort_create(pbf_radixsort **out, uint32_t blocksize, uint32_t numblocks, int32_t gx, int32_t gy,
+                         int32_t gz);
+int pbf_radixsort_destroy(pbf_radixsort *r);
+/* RadixSort::GetBuffer (src/RadixSort.cpp:118-122): the record buffer to fill
+ * before run and holding the sorted records after it. */
+int pbf_radixsort_buffer(pbf_radixsort *r, void **dev);
+/* RadixSort::Run (src/RadixSort.cpp:124-133). */
+int pbf_radixsort_run(pbf_radixsort *r);
+/* Sorted-slot -> input-index permutation of the last run (int32[n], device). */
+int pbf_radixsort_permutation(pbf_radixsort *r, void **dev);
+int pbf_radixsort_set_stream(pbf_radixsort *r, void *stream);
+void *pbf_radixsort_get_stream(const pbf_radixsort *r);
+
+/* ---------------------------------------------------------------- NeighbourCellFinder */
+/* NeighbourCellFinder::NeighbourCellFinder (src/NeighbourCellFinder.h:42,
+ * src/NeighbourCellFinder.cpp:24-100). */
+int pbf_ncf_create(pbf_ncf **out, uint32_t numparticles, int32_t gx, int32_t gy, int32_t gz);
+int pbf_ncf_destroy(pbf_ncf *f);
+/* NeighbourCellFinder::FindNeighbourCells (src/NeighbourCellFinder.cpp:113-148):
+ * from sorted float4 records (device). */
+int pbf_ncf_find(pbf_ncf *f, const void *dev_sorted_records);
+/* NeighbourCellFinder::GetResult (src/NeighbourCellFinder.cpp:108-111): 3
+ * ivec4 per particle, each component start + (count << 24), w = 0 (device). */
+int pbf_ncf_result(pbf_ncf *f, void **dev12);
+/* The start table (cleared to -1 every call) and the persistent end table as
+ * int32[gx*gy*gz]. */
+int pbf_ncf_get_tables(pbf_ncf *f, int32_t *start, int32_t *end, int host);
+int pbf_ncf_set_end_table(pbf_ncf *f, const int32_t *end, int host);
+int pbf_ncf_set_stream(pbf_ncf *f, void *stream);
+void *pbf_ncf_get_stream(const pbf_ncf *f);
+
+/* ---------------------------------------------------------------- stages
+ * Single shader dispatches of SPH::Run on caller memory (device pointers,
+ * reference layouts), so each stage can be checked alone. n is a multiple of
+ * 256 (the reference's BLOCKSIZE dispatch); nbr12 is the GetResult layout. */
+/* predictpos.glsl:18-38: rec4_out[i] = {x + dt v', i}. */
+int pbf_stage_predict(uint32_t n, const int32_t grid[3], const pbf_params *p, const void *pos4, const void *vel4,
+                      void *rec4_out, void *stream);
+/* calclambda.glsl:66-103. */
+int pbf_stage_lambda(uint32_t n, const int32_t grid[3], const pbf_params *p, const void *rec4, const void *nbr12,
+                     void *lambda_out, void *stream);
+/* updatepos.glsl:42-105, race-free: reads rec4_in, writes rec4_out. */
+int pbf_stage_updatepos(uint32_t n, const int32_t grid[3], const pbf_params *p, const void *rec4_in,
+                        const void *lambda, const void *nbr12, void *rec4_out, void *stream);
+/* update.glsl:16-28: pos4/vel4 (id order) updated in place. */
+int pbf_stage_update(uint32_t n, const pbf_params *p, const void *rec4, void *pos4, void *vel4, void *stream);
+/* vorticity.glsl:34-86 as two race-free passes: vel4 (id order) updated in
+ * place, |omega| per sorted slot to vort_out. */
+int pbf_stage_vorticity(uint32_t n, const int32_t grid[3], const pbf_params *p, const void *rec4,
+                        const void *nbr12, void *vel4, void *vort_out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

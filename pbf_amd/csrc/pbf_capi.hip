@@ -1,0 +1,889 @@
+// C ABI of libpbf_amd (declared in include/pbf.h): handles for SPH, RadixSort
+// and NeighbourCellFinder, plus single-stage entry points. Host-side
+// orchestration only; the kernels live in pbf_kernels.hip and pbf_sort.hip.
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "pbf_internal.h"
+
+namespace pbf {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+const char *last_error() { return g_last_error.c_str(); }
+
+int hip_fail(hipError_t e, const char *what) {
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? PBF_ERR_NOMEM : PBF_ERR_HIP;
+}
+
+static uint32_t count_sortbits(uint64_t v) {  // src/RadixSort.cpp:24-30
+    uint32_t r = 1;
+    while (v >>= 1) r++;
+    return r;
+}
+
+int key_bits_of(const int32_t g[3]) {
+    const uint32_t nb = count_sortbits((uint64_t)g[0] * (uint64_t)g[1] * (uint64_t)g[2] - 1);
+    return (int)(2 * ((nb + 1) >> 1));
+}
+
+SolverConsts make_consts(uint32_t n, const int32_t grid[3], const pbf_params &p, int key_bits) {
+    SolverConsts c;
+    std::memset(&c, 0, sizeof c);
+    c.n = (int32_t)n;
+    c.gx = grid[0];
+    c.gy = grid[1];
+    c.gz = grid[2];
+    const float h = p.h;
+    c.h = h;
+    c.hh = h * h;
+    c.h9 = h * h * h * h * h * h * h * h * h;
+    c.h_is_one = h == 1.0f;
+    // Any float sq above h2cut has fl(sqrt(sq)) > h: h^2 (1 + 1e-5), rounded up.
+    c.h2cut = std::nextafter((float)((double)h * (double)h * (1.0 + 1e-5)), INFINITY);
+    c.inv_rho0 = p.one_over_rho_0;
+    c.eps = p.epsilon;
+    c.gravity = p.gravity;
+    c.dt = p.timestep;
+    c.k = p.tensile_instability_k;
+    c.scale = p.tensile_instability_scale;
+    c.xsph_c = p.xsph_viscosity_c;
+    c.vort_eps = p.vorticity_epsilon;
+    for (int a = 0; a < 3; a++) {
+        c.lo[a] = p.wall_lo[a];
+        c.hi[a] = p.wall_hi[a];
+        c.gdir[a] = p.gravity_dir[a];
+    }
+    c.extforce = p.extforce;
+    c.key_mask = key_bits >= 32 ? 0xFFFFFFFFu : ((1u << key_bits) - 1u);
+    c.epoch = 0;
+    return c;
+}
+
+}  // namespace pbf
+
+using namespace pbf;
+
+namespace {
+
+int invalid(const std::string &msg) {
+    set_error(msg);
+    return PBF_ERR_INVALID;
+}
+
+bool valid_grid(int32_t gx, int32_t gy, int32_t gz) {
+    return gx > 0 && gy > 0 && gz > 0 && (uint64_t)gx * gy * gz < (1ull << 31);
+}
+
+template <class T>
+hipError_t dalloc(T **p, size_t count) {
+    return hipMalloc((void **)p, (count ? count : 1) * sizeof(T));
+}
+
+template <class T>
+void dfree(T *&p) {
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+hipMemcpyKind to_dev(int host) { return host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice; }
+hipMemcpyKind from_dev(int host) { return host ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice; }
+
+// Packed-range neighbour table and cell tables of one NeighbourCellFinder.
+struct CellTables {
+    size_t cells = 0;
+    int32_t *start = nullptr;
+    unsigned long long *end64 = nullptr;
+    int32_t *nbr9 = nullptr;
+    uint32_t epoch = 0;
+
+    hipError_t alloc(uint32_t n, size_t c) {
+        cells = c;
+        hipError_t e;
+        if ((e = dalloc(&start, cells)) != hipSuccess) return e;
+        if ((e = dalloc(&end64, cells)) != hipSuccess) return e;
+        if ((e = dalloc(&nbr9, (size_t)9 * n)) != hipSuccess) return e;
+        // The reference's end texture starts as zeros (glTexImage3D with NULL
+        // data reads 0 on llvmpipe) and is never cleared afterwards.
+        return hipMemset(end64, 0, cells * sizeof(unsigned long long));
+    }
+    void release() {
+        dfree(start);
+        dfree(end64);
+        dfree(nbr9);
+    }
+    // FindNeighbourCells (src/NeighbourCellFinder.cpp:113-148).
+    void find(SolverConsts c, const float4 *sorted, hipStream_t st) {
+        c.epoch = ++epoch;
+        launch_fill_i32(start, -1, cells, st);
+        launch_findcells(c, sorted, start, end64, st);
+        launch_neighbourcells(c, sorted, start, end64, nbr9, st);
+    }
+};
+
+enum KernelId {
+    KID_PREDICT = 0,
+    KID_SORT,
+    KID_GATHER,
+    KID_CLEAR_START,
+    KID_FINDCELLS,
+    KID_NEIGHBOURCELLS,
+    KID_LAMBDA,
+    KID_UPDATEPOS,
+    KID_UPDATE,
+    KID_VORTICITY_A,
+    KID_VORTICITY_B,
+    KID_COUNT
+};
+static_assert(KID_COUNT == PBF_NUM_KERNEL_IDS, "kernel id table");
+
+const char *const kKernelNames[KID_COUNT] = {
+    "predict",  "radix_sort", "gather_records", "clear_start", "findcells",  "neighbourcells",
+    "lambda",   "updatepos",  "update",         "vorticity_a", "vorticity_b"};
+
+// Event brackets around kernel launches, folded into per-kernel totals.
+struct KernelTimer {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    std::vector<int> ids;
+    double total_ms[KID_COUNT] = {};
+    uint32_t launches[KID_COUNT] = {};
+
+    hipError_t begin(int id, hipStream_t st) {
+        if (!on) return hipSuccess;
+        hipError_t e;
+        if (used + 2 > pool.size()) {
+            if ((e = fold()) != hipSuccess) return e;
+            while (pool.size() < used + 2) {
+                hipEvent_t ev;
+                if ((e = hipEventCreate(&ev)) != hipSuccess) return e;
+                pool.push_back(ev);
+            }
+        }
+        ids.push_back(id);
+        return hipEventRecord(pool[used++], st);
+    }
+    hipError_t end(hipStream_t st) {
+        if (!on) return hipSuccess;
+        return hipEventRecord(pool[used++], st);
+    }
+    hipError_t fold() {
+        hipError_t e;
+        for (size_t k = 0; k < ids.size(); k++) {
+            if ((e = hipEventSynchronize(pool[2 * k + 1])) != hipSuccess) return e;
+            float ms = 0.0f;
+            if ((e = hipEventElapsedTime(&ms, pool[2 * k], pool[2 * k + 1])) != hipSuccess) return e;
+            total_ms[ids[k]] += ms;
+            launches[ids[k]]++;
+        }
+        ids.clear();
+        used = 0;
+        return hipSuccess;
+    }
+    void reset() {
+        for (int k = 0; k < KID_COUNT; k++) {
+            total_ms[k] = 0.0;
+            launches[k] = 0;
+        }
+    }
+    void release() {
+        for (auto ev : pool) (void)hipEventDestroy(ev);
+        pool.clear();
+        ids.clear();
+        used = 0;
+    }
+};
+
+}  // namespace
+
+struct pbf_sph {
+    uint32_t n = 0;
+    int32_t grid[3] = {0, 0, 0};
+    size_t cells = 0;
+    int key_bits = 0;
+    pbf_params params;
+    SolverConsts consts;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    float4 *pos = nullptr, *vel = nullptr;       // id-order state
+    float4 *xs = nullptr;                        // predicted records, id order
+    float4 *rec = nullptr, *rec_alt = nullptr;   // sorted records, Jacobi double buffer
+    float4 *vel_sorted = nullptr, *vxsph = nullptr, *omega = nullptr;
+    float *lambda = nullptr;
+    uint32_t *keys = nullptr, *perm = nullptr;
+    CellTables cells_t;
+    RadixScratch rs;
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    bool stepped = false;
+    KernelTimer kt;
+
+    void release() {
+        kt.release();
+        dfree(pos);
+        dfree(vel);
+        dfree(xs);
+        dfree(rec);
+        dfree(rec_alt);
+        dfree(vel_sorted);
+        dfree(vxsph);
+        dfree(omega);
+        dfree(lambda);
+        dfree(keys);
+        dfree(perm);
+        cells_t.release();
+        radix_free(rs);
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (own_stream) (void)hipStreamDestroy(own_stream);
+    }
+};
+
+struct pbf_radixsort {
+    uint32_t n = 0;
+    int32_t grid[3] = {0, 0, 0};
+    int key_bits = 0;
+    hipStream_t own_stream = nullptr, stream = nullptr;
+    float4 *buf = nullptr, *buf_alt = nullptr;
+    uint32_t *keys = nullptr, *perm = nullptr;
+    RadixScratch rs;
+};
+
+struct pbf_ncf {
+    uint32_t n = 0;
+    int32_t grid[3] = {0, 0, 0};
+    hipStream_t own_stream = nullptr, stream = nullptr;
+    CellTables t;
+    int32_t *nbr12 = nullptr;
+};
+
+extern "C" {
+
+const char *pbf_last_error(void) { return pbf::last_error(); }
+const char *pbf_version(void) { return "pbf_amd 0.1 (gfx950)"; }
+
+int pbf_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int pbf_memcpy(void *dst, const void *src, size_t bytes, int kind, void *stream) {
+    if (!dst || !src) return invalid("null pointer");
+    static const hipMemcpyKind kinds[3] = {hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice};
+    if (kind < 0 || kind > 2) return invalid("memcpy kind must be 0, 1 or 2");
+    PBF_HIP(hipMemcpyAsync(dst, src, bytes, kinds[kind], (hipStream_t)stream));
+    PBF_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return PBF_OK;
+}
+
+int pbf_synchronize(void *stream) {
+    PBF_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return PBF_OK;
+}
+
+void pbf_params_default(pbf_params *out, int32_t gx, int32_t gy, int32_t gz) {
+    if (!out) return;
+    std::memset(out, 0, sizeof *out);
+    out->one_over_rho_0 = 1.0f;  // src/SPH.cpp:137-144
+    out->epsilon = 5.0f;
+    out->gravity = 10.0f;
+    out->timestep = 0.016f;
+    out->tensile_instability_k = 0.1f;
+    out->tensile_instability_scale = 1.0f / pbf_wpoly6(0.2f, 2.0f);
+    out->xsph_viscosity_c = 0.01f;
+    out->vorticity_epsilon = 5.0f;
+    out->iterations = 5;     // src/SPH.cpp:26
+    out->vorticity = 0;      // src/SPH.cpp:25
+    out->extforce = 0;
+    out->h = 2.0f;           // src/SPH.cpp:58
+    const float wall[3] = {16.0f, 0.0f, 16.0f};  // updatepos.glsl:98
+    const int32_t g[3] = {gx, gy, gz};
+    for (int a = 0; a < 3; a++) {
+        out->wall_lo[a] = wall[a];
+        out->wall_hi[a] = (float)g[a] - wall[a];
+    }
+    out->gravity_dir[0] = 0.0f;
+    out->gravity_dir[1] = -1.0f;
+    out->gravity_dir[2] = 0.0f;
+}
+
+float pbf_wpoly6(float r, float h) {  // src/SPH.cpp:159-164
+    if (r > h) return 0.0f;
+    float tmp = h * h - r * r;
+    return 1.56668147106f * tmp * tmp * tmp / (h * h * h * h * h * h * h * h * h);
+}
+
+int pbf_key_bits(int32_t gx, int32_t gy, int32_t gz) {
+    if (!valid_grid(gx, gy, gz)) return invalid("grid dimensions must be positive and below 2^31 cells");
+    const int32_t g[3] = {gx, gy, gz};
+    return key_bits_of(g);
+}
+
+// ---------------------------------------------------------------- SPH
+int pbf_sph_create(pbf_sph **out, uint32_t n, int32_t gx, int32_t gy, int32_t gz) {
+    if (!out) return invalid("pbf_sph_create: null output handle");
+    *out = nullptr;
+    if (n == 0 || n % 512 != 0) return invalid("numparticles must be a positive multiple of 512");
+    if (n >= (1u << 24)) return invalid("numparticles must be below 2^24 (24-bit start of the packed ranges)");
+    if (!valid_grid(gx, gy, gz)) return invalid("grid dimensions must be positive and below 2^31 cells");
+    if (pbf_device_count() < 1) {
+        set_error("no HIP device");
+        return PBF_ERR_NODEVICE;
+    }
+    pbf_sph *s = new (std::nothrow) pbf_sph();
+    if (!s) return invalid("host allocation failed");
+    s->n = n;
+    s->grid[0] = gx;
+    s->grid[1] = gy;
+    s->grid[2] = gz;
+    s->cells = (size_t)gx * gy * gz;
+    s->key_bits = key_bits_of(s->grid);
+    pbf_params_default(&s->params, gx, gy, gz);
+    s->consts = make_consts(n, s->grid, s->params, s->key_bits);
+    hipError_t e = hipSuccess;
+#define TRY(x)                                \
+    if ((e = (x)) != hipSuccess) {            \
+        int rc = hip_fail(e, #x);             \
+        s->release();                         \
+        delete s;                             \
+        return rc;                            \
+    }
+    TRY(hipStreamCreateWithFlags(&s->own_stream, hipStreamNonBlocking));
+    s->stream = s->own_stream;
+    TRY(dalloc(&s->pos, n));
+    TRY(dalloc(&s->vel, n));
+    TRY(dalloc(&s->xs, n));
+    TRY(dalloc(&s->rec, n));
+    TRY(dalloc(&s->rec_alt, n));
+    TRY(dalloc(&s->vel_sorted, n));
+    TRY(dalloc(&s->vxsph, n));
+    TRY(dalloc(&s->omega, n));
+    TRY(dalloc(&s->lambda, n));
+    TRY(dalloc(&s->keys, n));
+    TRY(dalloc(&s->perm, n));
+    TRY(s->cells_t.alloc(n, s->cells));
+    if (!radix_alloc(s->rs, (int32_t)n)) TRY(hipErrorOutOfMemory);
+    for (auto &ev : s->ev) TRY(hipEventCreate(&ev));
+    TRY(hipMemset(s->pos, 0, (size_t)n * sizeof(float4)));
+    TRY(hipMemset(s->vel, 0, (size_t)n * sizeof(float4)));
+    TRY(hipMemset(s->omega, 0, (size_t)n * sizeof(float4)));
+    TRY(hipDeviceSynchronize());
+#undef TRY
+    *out = s;
+    return PBF_OK;
+}
+
+int pbf_sph_destroy(pbf_sph *s) {
+    if (!s) return PBF_OK;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    s->release();
+    delete s;
+    return PBF_OK;
+}
+
+static int check_params(const pbf_params *p) {
+    if (!p) return invalid("null parameters");
+    if (!(p->h > 0.0f) || !std::isfinite(p->h)) return invalid("h must be positive and finite");
+    if (!(p->timestep != 0.0f) || !std::isfinite(p->timestep)) return invalid("timestep must be nonzero and finite");
+    if (p->iterations > 1000) return invalid("iterations above 1000");
+    return PBF_OK;
+}
+
+int pbf_sph_set_params(pbf_sph *s, const pbf_params *p) {
+    if (!s) return invalid("null SPH handle");
+    int rc = check_params(p);
+    if (rc) return rc;
+    s->params = *p;
+    s->consts = make_consts(s->n, s->grid, s->params, s->key_bits);
+    return PBF_OK;
+}
+
+int pbf_sph_get_params(const pbf_sph *s, pbf_params *out) {
+    if (!s || !out) return invalid("null argument");
+    *out = s->params;
+    return PBF_OK;
+}
+
+int pbf_sph_upload_state(pbf_sph *s, const float *pos4, const float *vel4, int host) {
+    if (!s || !pos4 || !vel4) return invalid("null argument");
+    const size_t bytes = (size_t)s->n * sizeof(float4);
+    PBF_HIP(hipMemcpyAsync(s->pos, pos4, bytes, to_dev(host), s->stream));
+    PBF_HIP(hipMemcpyAsync(s->vel, vel4, bytes, to_dev(host), s->stream));
+    if (host) PBF_HIP(hipStreamSynchronize(s->stream));
+    return PBF_OK;
+}
+
+int pbf_sph_download_state(pbf_sph *s, float *pos4, float *vel4, int host) {
+    if (!s) return invalid("null SPH handle");
+    const size_t bytes = (size_t)s->n * sizeof(float4);
+    if (pos4) PBF_HIP(hipMemcpyAsync(pos4, s->pos, bytes, from_dev(host), s->stream));
+    if (vel4) PBF_HIP(hipMemcpyAsync(vel4, s->vel, bytes, from_dev(host), s->stream));
+    if (host) PBF_HIP(hipStreamSynchronize(s->stream));
+    return PBF_OK;
+}
+
+int pbf_sph_run(pbf_sph *s) {
+    if (!s) return invalid("null SPH handle");
+    hipStream_t st = s->stream;
+    const SolverConsts &c = s->consts;
+    const int32_t n = (int32_t)s->n;
+    KernelTimer &kt = s->kt;
+#define TIMED(id, launch)                    \
+    do {                                     \
+        PBF_HIP(kt.begin(id, st));           \
+        launch;                              \
+        PBF_HIP(kt.end(st));                 \
+    } while (0)
+    PBF_HIP(hipEventRecord(s->ev[0], st));
+    // predict positions (src/SPH.cpp:247-261) + cell keys
+    TIMED(KID_PREDICT, launch_predict(c, s->pos, s->vel, s->xs, s->keys, st));
+    PBF_HIP(hipEventRecord(s->ev[1], st));
+    // sort (src/SPH.cpp:263-268)
+    TIMED(KID_SORT, radix_sort_perm(s->rs, s->keys, n, s->key_bits, s->perm, st));
+    TIMED(KID_GATHER, launch_gather_records(n, s->perm, s->xs, s->rec, st));
+    PBF_HIP(hipEventRecord(s->ev[2], st));
+    // neighbour cells (src/SPH.cpp:270-275; src/NeighbourCellFinder.cpp:113-148)
+    {
+        CellTables &t = s->cells_t;
+        SolverConsts cc = c;
+        cc.epoch = ++t.epoch;
+        TIMED(KID_CLEAR_START, launch_fill_i32(t.start, -1, t.cells, st));
+        TIMED(KID_FINDCELLS, launch_findcells(cc, s->rec, t.start, t.end64, st));
+        TIMED(KID_NEIGHBOURCELLS, launch_neighbourcells(cc, s->rec, t.start, t.end64, t.nbr9, st));
+    }
+    PBF_HIP(hipEventRecord(s->ev[3], st));
+    // solver iterations (src/SPH.cpp:303-311)
+    for (uint32_t it = 0; it < s->params.iterations; it++) {
+        TIMED(KID_LAMBDA, launch_lambda(c, s->rec, s->cells_t.nbr9, s->lambda, st));
+        TIMED(KID_UPDATEPOS, launch_updatepos(c, s->rec, s->lambda, s->cells_t.nbr9, s->rec_alt, st));
+        std::swap(s->rec, s->rec_alt);
+    }
+    PBF_HIP(hipEventRecord(s->ev[4], st));
+    // update + vorticity (src/SPH.cpp:315-333)
+    TIMED(KID_UPDATE, launch_update(c, s->rec, s->pos, s->vel, s->params.vorticity ? s->vel_sorted : nullptr, st));
+    if (s->params.vorticity) {
+        TIMED(KID_VORTICITY_A,
+              launch_vorticity_a(c, s->rec, s->cells_t.nbr9, s->vel_sorted, s->vel, s->vxsph, s->omega, st));
+        TIMED(KID_VORTICITY_B, launch_vorticity_b(c, s->rec, s->cells_t.nbr9, s->vxsph, s->omega, s->vel, st));
+    }
+#undef TIMED
+    PBF_HIP(hipEventRecord(s->ev[5], st));
+    PBF_HIP(hipGetLastError());
+    s->stepped = true;
+    return PBF_OK;
+}
+
+int pbf_sph_step(pbf_sph *s, int32_t nsteps) {
+    if (!s) return invalid("null SPH handle");
+    if (nsteps < 0) return invalid("negative step count");
+    for (int32_t k = 0; k < nsteps; k++) {
+        int rc = pbf_sph_run(s);
+        if (rc) return rc;
+    }
+    return PBF_OK;
+}
+
+int pbf_sph_positions_device(pbf_sph *s, void **dev, size_t *stride) {
+    if (!s || !dev) return invalid("null argument");
+    *dev = s->pos;
+    if (stride) *stride = sizeof(float4);
+    return PBF_OK;
+}
+
+int pbf_sph_velocities_device(pbf_sph *s, void **dev, size_t *stride) {
+    if (!s || !dev) return invalid("null argument");
+    *dev = s->vel;
+    if (stride) *stride = sizeof(float4);
+    return PBF_OK;
+}
+
+int pbf_sph_timing(pbf_sph *s, pbf_timing *out) {
+    if (!s || !out) return invalid("null argument");
+    std::memset(out, 0, sizeof *out);
+    if (!s->stepped) return PBF_OK;
+    PBF_HIP(hipEventSynchronize(s->ev[5]));
+    float ms[5];
+    for (int k = 0; k < 5; k++) PBF_HIP(hipEventElapsedTime(&ms[k], s->ev[k], s->ev[k + 1]));
+    out->predict_ms = ms[0];
+    out->sort_ms = ms[1];
+    out->neighbour_cells_ms = ms[2];
+    out->solver_ms = ms[3];
+    out->vorticity_ms = ms[4];
+    return PBF_OK;
+}
+
+int pbf_sph_set_kernel_timing(pbf_sph *s, int enable) {
+    if (!s) return invalid("null SPH handle");
+    if (s->kt.on && !enable) PBF_HIP(s->kt.fold());
+    s->kt.on = enable != 0;
+    return PBF_OK;
+}
+
+int pbf_sph_kernel_times(pbf_sph *s, pbf_kernel_time out[PBF_NUM_KERNEL_IDS]) {
+    if (!s || !out) return invalid("null argument");
+    PBF_HIP(s->kt.fold());
+    for (int k = 0; k < KID_COUNT; k++) {
+        out[k].total_ms = s->kt.total_ms[k];
+        out[k].launches = s->kt.launches[k];
+    }
+    s->kt.reset();
+    return PBF_OK;
+}
+
+const char *pbf_kernel_name(int id) { return id >= 0 && id < KID_COUNT ? kKernelNames[id] : nullptr; }
+
+int pbf_sph_set_stream(pbf_sph *s, void *stream) {
+    if (!s) return invalid("null SPH handle");
+    s->stream = stream ? (hipStream_t)stream : s->own_stream;
+    return PBF_OK;
+}
+
+void *pbf_sph_get_stream(const pbf_sph *s) { return s ? (void *)s->stream : nullptr; }
+
+int pbf_sph_get_end_table(pbf_sph *s, int32_t *dst, int host) {
+    if (!s || !dst) return invalid("null argument");
+    int32_t *tmp = dst;
+    if (host) PBF_HIP(dalloc(&tmp, s->cells));
+    launch_end64_to_i32(s->cells, s->cells_t.end64, tmp, s->stream);
+    if (host) {
+        hipError_t e = hipMemcpyAsync(dst, tmp, s->cells * 4, hipMemcpyDeviceToHost, s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        (void)hipFree(tmp);
+        if (e != hipSuccess) return hip_fail(e, "pbf_sph_get_end_table");
+    }
+    return PBF_OK;
+}
+
+int pbf_sph_set_end_table(pbf_sph *s, const int32_t *src, int host) {
+    if (!s || !src) return invalid("null argument");
+    int32_t *tmp = nullptr;
+    PBF_HIP(dalloc(&tmp, s->cells));
+    hipError_t e = hipMemcpyAsync(tmp, src, s->cells * 4, to_dev(host), s->stream);
+    if (e == hipSuccess) {
+        launch_i32_to_end64(s->cells, tmp, s->cells_t.end64, s->stream);
+        e = hipStreamSynchronize(s->stream);
+    }
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return hip_fail(e, "pbf_sph_set_end_table");
+    return PBF_OK;
+}
+
+int pbf_sph_neighbour_table(pbf_sph *s, int32_t *dst12, int host) {
+    if (!s || !dst12) return invalid("null argument");
+    int32_t *tmp = dst12;
+    if (host) PBF_HIP(dalloc(&tmp, (size_t)12 * s->n));
+    launch_nbr9_to_12((int32_t)s->n, s->cells_t.nbr9, tmp, s->stream);
+    if (host) {
+        hipError_t e = hipMemcpyAsync(dst12, tmp, (size_t)48 * s->n, hipMemcpyDeviceToHost, s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        (void)hipFree(tmp);
+        if (e != hipSuccess) return hip_fail(e, "pbf_sph_neighbour_table");
+    }
+    return PBF_OK;
+}
+
+int pbf_sph_sorted_records(pbf_sph *s, float *dst4, int host) {
+    if (!s || !dst4) return invalid("null argument");
+    PBF_HIP(hipMemcpyAsync(dst4, s->rec, (size_t)s->n * sizeof(float4), from_dev(host), s->stream));
+    if (host) PBF_HIP(hipStreamSynchronize(s->stream));
+    return PBF_OK;
+}
+
+// ---------------------------------------------------------------- RadixSort
+int pbf_radixsort_create(pbf_radixsort **out, uint32_t blocksize, uint32_t numblocks, int32_t gx, int32_t gy,
+                         int32_t gz) {
+    if (!out) return invalid("null output handle");
+    *out = nullptr;
+    if (blocksize & 1) {  // src/RadixSort.cpp:41-42
+        set_error("The block size for sorting has to be even.");
+        return PBF_ERR_LOGIC;
+    }
+    const uint64_t n = (uint64_t)blocksize * numblocks;
+    if (n == 0 || n >= (1u << 31)) return invalid("blocksize * numblocks must be in [1, 2^31)");
+    if (!valid_grid(gx, gy, gz)) return invalid("grid dimensions must be positive and below 2^31 cells");
+    if (pbf_device_count() < 1) {
+        set_error("no HIP device");
+        return PBF_ERR_NODEVICE;
+    }
+    pbf_radixsort *r = new (std::nothrow) pbf_radixsort();
+    if (!r) return invalid("host allocation failed");
+    r->n = (uint32_t)n;
+    r->grid[0] = gx;
+    r->grid[1] = gy;
+    r->grid[2] = gz;
+    r->key_bits = key_bits_of(r->grid);
+    hipError_t e;
+    bool ok = (e = hipStreamCreateWithFlags(&r->own_stream, hipStreamNonBlocking)) == hipSuccess &&
+              (e = dalloc(&r->buf, n)) == hipSuccess && (e = dalloc(&r->buf_alt, n)) == hipSuccess &&
+              (e = dalloc(&r->keys, n)) == hipSuccess && (e = dalloc(&r->perm, n)) == hipSuccess;
+    if (ok && !radix_alloc(r->rs, (int32_t)n)) {
+        ok = false;
+        e = hipErrorOutOfMemory;
+    }
+    if (!ok) {
+        int rc = hip_fail(e, "pbf_radixsort_create");
+        pbf_radixsort_destroy(r);
+        return rc;
+    }
+    r->stream = r->own_stream;
+    *out = r;
+    return PBF_OK;
+}
+
+int pbf_radixsort_destroy(pbf_radixsort *r) {
+    if (!r) return PBF_OK;
+    if (r->stream) (void)hipStreamSynchronize(r->stream);
+    dfree(r->buf);
+    dfree(r->buf_alt);
+    dfree(r->keys);
+    dfree(r->perm);
+    radix_free(r->rs);
+    if (r->own_stream) (void)hipStreamDestroy(r->own_stream);
+    delete r;
+    return PBF_OK;
+}
+
+int pbf_radixsort_buffer(pbf_radixsort *r, void **dev) {
+    if (!r || !dev) return invalid("null argument");
+    *dev = r->buf;
+    return PBF_OK;
+}
+
+int pbf_radixsort_run(pbf_radixsort *r) {
+    if (!r) return invalid("null RadixSort handle");
+    pbf_params p;
+    pbf_params_default(&p, r->grid[0], r->grid[1], r->grid[2]);
+    const SolverConsts c = make_consts(r->n, r->grid, p, r->key_bits);
+    launch_keys_of_records(c, r->buf, r->keys, r->stream);
+    radix_sort_perm(r->rs, r->keys, (int32_t)r->n, r->key_bits, r->perm, r->stream);
+    launch_gather_records((int32_t)r->n, r->perm, r->buf, r->buf_alt, r->stream);
+    std::swap(r->buf, r->buf_alt);
+    PBF_HIP(hipGetLastError());
+    return PBF_OK;
+}
+
+int pbf_radixsort_permutation(pbf_radixsort *r, void **dev) {
+    if (!r || !dev) return invalid("null argument");
+    *dev = r->perm;
+    return PBF_OK;
+}
+
+int pbf_radixsort_set_stream(pbf_radixsort *r, void *stream) {
+    if (!r) return invalid("null RadixSort handle");
+    r->stream = stream ? (hipStream_t)stream : r->own_stream;
+    return PBF_OK;
+}
+
+void *pbf_radixsort_get_stream(const pbf_radixsort *r) { return r ? (void *)r->stream : nullptr; }
+
+// ---------------------------------------------------------------- NeighbourCellFinder
+int pbf_ncf_create(pbf_ncf **out, uint32_t n, int32_t gx, int32_t gy, int32_t gz) {
+    if (!out) return invalid("null output handle");
+    *out = nullptr;
+    if (n == 0 || n >= (1u << 24)) return invalid("numparticles must be in [1, 2^24)");
+    if (!valid_grid(gx, gy, gz)) return invalid("grid dimensions must be positive and below 2^31 cells");
+    if (pbf_device_count() < 1) {
+        set_error("no HIP device");
+        return PBF_ERR_NODEVICE;
+    }
+    pbf_ncf *f = new (std::nothrow) pbf_ncf();
+    if (!f) return invalid("host allocation failed");
+    f->n = n;
+    f->grid[0] = gx;
+    f->grid[1] = gy;
+    f->grid[2] = gz;
+    hipError_t e;
+    bool ok = (e = hipStreamCreateWithFlags(&f->own_stream, hipStreamNonBlocking)) == hipSuccess &&
+              (e = f->t.alloc(n, (size_t)gx * gy * gz)) == hipSuccess &&
+              (e = dalloc(&f->nbr12, (size_t)12 * n)) == hipSuccess;
+    if (!ok) {
+        int rc = hip_fail(e, "pbf_ncf_create");
+        pbf_ncf_destroy(f);
+        return rc;
+    }
+    f->stream = f->own_stream;
+    *out = f;
+    return PBF_OK;
+}
+
+int pbf_ncf_destroy(pbf_ncf *f) {
+    if (!f) return PBF_OK;
+    if (f->stream) (void)hipStreamSynchronize(f->stream);
+    f->t.release();
+    dfree(f->nbr12);
+    if (f->own_stream) (void)hipStreamDestroy(f->own_stream);
+    delete f;
+    return PBF_OK;
+}
+
+int pbf_ncf_find(pbf_ncf *f, const void *dev_sorted_records) {
+    if (!f || !dev_sorted_records) return invalid("null argument");
+    pbf_params p;
+    pbf_params_default(&p, f->grid[0], f->grid[1], f->grid[2]);
+    const SolverConsts c = make_consts(f->n, f->grid, p, key_bits_of(f->grid));
+    f->t.find(c, (const float4 *)dev_sorted_records, f->stream);
+    launch_nbr9_to_12((int32_t)f->n, f->t.nbr9, f->nbr12, f->stream);
+    PBF_HIP(hipGetLastError());
+    return PBF_OK;
+}
+
+int pbf_ncf_result(pbf_ncf *f, void **dev12) {
+    if (!f || !dev12) return invalid("null argument");
+    *dev12 = f->nbr12;
+    return PBF_OK;
+}
+
+int pbf_ncf_get_tables(pbf_ncf *f, int32_t *start, int32_t *end, int host) {
+    if (!f) return invalid("null NeighbourCellFinder handle");
+    const size_t bytes = f->t.cells * 4;
+    if (start) PBF_HIP(hipMemcpyAsync(start, f->t.start, bytes, from_dev(host), f->stream));
+    if (end) {
+        int32_t *tmp = end;
+        if (host) PBF_HIP(dalloc(&tmp, f->t.cells));
+        launch_end64_to_i32(f->t.cells, f->t.end64, tmp, f->stream);
+        if (host) {
+            hipError_t e = hipMemcpyAsync(end, tmp, bytes, hipMemcpyDeviceToHost, f->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(f->stream);
+            (void)hipFree(tmp);
+            if (e != hipSuccess) return hip_fail(e, "pbf_ncf_get_tables");
+        }
+    }
+    if (host) PBF_HIP(hipStreamSynchronize(f->stream));
+    return PBF_OK;
+}
+
+int pbf_ncf_set_end_table(pbf_ncf *f, const int32_t *end, int host) {
+    if (!f || !end) return invalid("null argument");
+    int32_t *tmp = nullptr;
+    PBF_HIP(dalloc(&tmp, f->t.cells));
+    hipError_t e = hipMemcpyAsync(tmp, end, f->t.cells * 4, to_dev(host), f->stream);
+    if (e == hipSuccess) {
+        launch_i32_to_end64(f->t.cells, tmp, f->t.end64, f->stream);
+        e = hipStreamSynchronize(f->stream);
+    }
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return hip_fail(e, "pbf_ncf_set_end_table");
+    return PBF_OK;
+}
+
+int pbf_ncf_set_stream(pbf_ncf *f, void *stream) {
+    if (!f) return invalid("null NeighbourCellFinder handle");
+    f->stream = stream ? (hipStream_t)stream : f->own_stream;
+    return PBF_OK;
+}
+
+void *pbf_ncf_get_stream(const pbf_ncf *f) { return f ? (void *)f->stream : nullptr; }
+
+// ---------------------------------------------------------------- stages
+static int stage_check(uint32_t n, const int32_t *grid, const pbf_params *p) {
+    if (n == 0 || n % 256 != 0 || n >= (1u << 24)) return invalid("n must be a positive multiple of 256 below 2^24");
+    if (grid && !valid_grid(grid[0], grid[1], grid[2])) return invalid("bad grid");
+    return check_params(p);
+}
+
+static const int32_t k_no_grid[3] = {1, 1, 1};
+
+int pbf_stage_predict(uint32_t n, const int32_t grid[3], const pbf_params *p, const void *pos4, const void *vel4,
+                      void *rec4_out, void *stream) {
+    int rc = stage_check(n, grid, p);
+    if (rc) return rc;
+    if (!pos4 || !vel4 || !rec4_out) return invalid("null pointer");
+    const SolverConsts c = make_consts(n, grid, *p, key_bits_of(grid));
+    launch_predict(c, (const float4 *)pos4, (const float4 *)vel4, (float4 *)rec4_out, nullptr, (hipStream_t)stream);
+    PBF_HIP(hipGetLastError());
+    return PBF_OK;
+}
+
+// Converts a caller's GetResult-layout table into the kernels' [9][n] layout.
+static int nbr_scratch(uint32_t n, const void *nbr12, int32_t **nbr9, hipStream_t st) {
+    PBF_HIP(dalloc(nbr9, (size_t)9 * n));
+    launch_nbr12_to_9((int32_t)n, (const int32_t *)nbr12, *nbr9, st);
+    return PBF_OK;
+}
+
+static int release_scratch(void *p, hipStream_t st) {
+    hipError_t e = hipStreamSynchronize(st);
+    (void)hipFree(p);
+    if (e != hipSuccess) return hip_fail(e, "stage");
+    PBF_HIP(hipGetLastError());
+    return PBF_OK;
+}
+
+int pbf_stage_lambda(uint32_t n, const int32_t grid[3], const pbf_params *p, const void *rec4, const void *nbr12,
+                     void *lambda_out, void *stream) {
+    int rc = stage_check(n, grid, p);
+    if (rc) return rc;
+    if (!rec4 || !nbr12 || !lambda_out) return invalid("null pointer");
+    hipStream_t st = (hipStream_t)stream;
+    int32_t *nbr9 = nullptr;
+    if ((rc = nbr_scratch(n, nbr12, &nbr9, st))) return rc;
+    const SolverConsts c = make_consts(n, grid, *p, key_bits_of(grid));
+    launch_lambda(c, (const float4 *)rec4, nbr9, (float *)lambda_out, st);
+    return release_scratch(nbr9, st);
+}
+
+int pbf_stage_updatepos(uint32_t n, const int32_t grid[3], const pbf_params *p, const void *rec4_in,
+                        const void *lambda, const void *nbr12, void *rec4_out, void *stream) {
+    int rc = stage_check(n, grid, p);
+    if (rc) return rc;
+    if (!rec4_in || !lambda || !nbr12 || !rec4_out) return invalid("null pointer");
+    if (rec4_in == rec4_out) return invalid("updatepos is race-free: input and output must differ");
+    hipStream_t st = (hipStream_t)stream;
+    int32_t *nbr9 = nullptr;
+    if ((rc = nbr_scratch(n, nbr12, &nbr9, st))) return rc;
+    const SolverConsts c = make_consts(n, grid, *p, key_bits_of(grid));
+    launch_updatepos(c, (const float4 *)rec4_in, (const float *)lambda, nbr9, (float4 *)rec4_out, st);
+    return release_scratch(nbr9, st);
+}
+
+int pbf_stage_update(uint32_t n, const pbf_params *p, const void *rec4, void *pos4, void *vel4, void *stream) {
+    int rc = stage_check(n, nullptr, p);
+    if (rc) return rc;
+    if (!rec4 || !pos4 || !vel4) return invalid("null pointer");
+    const SolverConsts c = make_consts(n, k_no_grid, *p, 1);
+    launch_update(c, (const float4 *)rec4, (float4 *)pos4, (float4 *)vel4, nullptr, (hipStream_t)stream);
+    PBF_HIP(hipGetLastError());
+    return PBF_OK;
+}
+
+int pbf_stage_vorticity(uint32_t n, const int32_t grid[3], const pbf_params *p, const void *rec4,
+                        const void *nbr12, void *vel4, void *vort_out, void *stream) {
+    int rc = stage_check(n, grid, p);
+    if (rc) return rc;
+    if (!rec4 || !nbr12 || !vel4 || !vort_out) return invalid("null pointer");
+    hipStream_t st = (hipStream_t)stream;
+    int32_t *nbr9 = nullptr;
+    float4 *tmp = nullptr;  // sorted velocities, XSPH velocities, omega
+    if ((rc = nbr_scratch(n, nbr12, &nbr9, st))) return rc;
+    hipError_t e = dalloc(&tmp, (size_t)3 * n);
+    if (e != hipSuccess) {
+        (void)hipFree(nbr9);
+        return hip_fail(e, "pbf_stage_vorticity");
+    }
+    const SolverConsts c = make_consts(n, grid, *p, key_bits_of(grid));
+    const float4 *rec = (const float4 *)rec4;
+    float4 *vel = (float4 *)vel4;
+    float4 *vs = tmp, *vx = tmp + n, *om = tmp + 2 * (size_t)n;
+    // sorted copy of the velocities, vs[g] = vel[id(rec[g])] (SPH::Run keeps
+    // one from the update pass)
+    launch_gather_by_id((int32_t)n, rec, vel, vs, st);
+    launch_vorticity_a(c, rec, nbr9, vs, vel, vx, om, st);
+    launch_vorticity_b(c, rec, nbr9, vx, om, vel, st);
+    launch_omega_norm((int32_t)n, om, (float *)vort_out, st);
+    e = hipStreamSynchronize(st);
+    (void)hipFree(nbr9);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return hip_fail(e, "pbf_stage_vorticity");
+    PBF_HIP(hipGetLastError());
+    return PBF_OK;
+}
+
+}  // extern "C"

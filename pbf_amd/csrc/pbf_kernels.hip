@@ -1,0 +1,490 @@
+// gfx950 kernels of the pbf simulation step (everything except the sort).
+//
+// Built with -ffp-contract=off: every float expression below is evaluated
+// with IEEE round-to-nearest in the written order, and division / sqrtf are
+// the correctly rounded sequences. The order of operations is the one the
+// reference's GLSL has when compiled for llvmpipe (Mesa 23.2.1): dot products
+// reduced from the last component, r*r of r = sqrt(sq) folded to sq,
+// 315/(64 pi)*t*t*t evaluated as (t*t)*(t*315/(64 pi)), and acc + (a - b) in
+// the vorticity sum as (acc + a) - b. The oracle restates the same order and
+// is pinned bit-exact to llvmpipe runs of the reference's shaders.
+//
+// Layouts (HBM): records are float4 {x, y, z, id bits} (ParticleKey,
+// predictpos.glsl:3-6); id-order state is float4 pos/vel with w = 0
+// (src/SPH.cpp:119-133); the neighbour table is int32[9][n] (row-major over
+// the reference's 9 (dy,dz) rows, each start + (count << 24)); the end table
+// is uint64 {step tag, gid} so the newest step's highest gid wins.
+#include "pbf_internal.h"
+
+namespace pbf {
+namespace {
+
+constexpr float POLY6 = 1.56668147106f;                // calclambda.glsl:46
+constexpr float SPIKY_GRAD = -3.0f * 4.774648292756860f; // calclambda.glsl:63, folded in FP32
+constexpr int BLOCK = 256;
+
+__device__ __forceinline__ float sq_len(float x, float y, float z) { return (z * z + y * y) + x * x; }
+
+// Wpoly6 (calclambda.glsl:41-47) of r = sqrt(sq).
+__device__ __forceinline__ float wpoly6(const SolverConsts &c, float sq, float r) {
+    if (r > c.h) return 0.0f;
+    float t = c.hh - sq;
+    float w = (t * t) * (t * POLY6);
+    return c.h_is_one ? w : w / c.h9;
+}
+
+// gradWspiky (calclambda.glsl:57-64) for l = sqrt(sq) with 0 < l <= h.
+__device__ __forceinline__ void grad_spiky(const SolverConsts &c, float dx, float dy, float dz, float l, float &gx,
+                                           float &gy, float &gz) {
+    float t = c.h - l;
+    float s = (SPIKY_GRAD * t) * t;
+    float den = l;
+    if (!c.h_is_one) den = (((((l * c.h) * c.h) * c.h) * c.h) * c.h) * c.h;
+    gx = (s * dx) / den;
+    gy = (s * dy) / den;
+    gz = (s * dz) / den;
+}
+
+__device__ __forceinline__ float clamp_glsl(float x, float lo, float hi) {
+    float m = x < lo ? lo : x;
+    return m > hi ? hi : m;
+}
+
+// GetHash, counting.glsl:53-57: FP32 dot of the clamped integer cell.
+__device__ __forceinline__ uint32_t cell_hash(const SolverConsts &c, float x, float y, float z) {
+    int32_t ix = (int32_t)clamp_glsl(x, 0.0f, (float)c.gx);
+    int32_t iy = (int32_t)clamp_glsl(y, 0.0f, (float)c.gy);
+    int32_t iz = (int32_t)clamp_glsl(z, 0.0f, (float)c.gz);
+    const float w1 = (float)(c.gx * c.gz), w2 = (float)c.gx;
+    float d = (float)ix * 1.0f + (float)iy * w1 + (float)iz * w2;
+    return (uint32_t)d;
+}
+
+__device__ __forceinline__ int32_t id_of(const float4 &r) { return __float_as_int(r.w); }
+
+// ----------------------------------------------------------------- predict
+// predictpos.glsl:18-38 fused with the key of counting.glsl:70.
+__global__ __launch_bounds__(BLOCK) void k_predict(SolverConsts c, const float4 *__restrict__ pos,
+                                                   const float4 *__restrict__ vel, float4 *__restrict__ rec,
+                                                   uint32_t *__restrict__ keys) {
+    const int32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= c.n) return;
+    const float4 p = pos[i];
+    const float4 v = vel[i];
+    float vx = v.x, vy = v.y, vz = v.z;
+    if (c.extforce && p.z > (float)c.gz / 2.0f) {
+        const float g2 = 2.0f * c.gravity;
+        vx = vx + (g2 * 0.0f) * c.dt;
+        vy = vy + (g2 * 0.0f) * c.dt;
+        vz = vz + (g2 * -1.0f) * c.dt;
+    }
+    vx = vx + (c.gravity * c.gdir[0]) * c.dt;
+    vy = vy + (c.gravity * c.gdir[1]) * c.dt;
+    vz = vz + (c.gravity * c.gdir[2]) * c.dt;
+    const float x = p.x + c.dt * vx, y = p.y + c.dt * vy, z = p.z + c.dt * vz;
+    rec[i] = make_float4(x, y, z, __int_as_float(i));
+    if (keys) keys[i] = cell_hash(c, x, y, z) & c.key_mask;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_keys(SolverConsts c, const float4 *__restrict__ rec,
+                                                uint32_t *__restrict__ keys) {
+    const int32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= c.n) return;
+    const float4 r = rec[i];
+    keys[i] = cell_hash(c, r.x, r.y, r.z) & c.key_mask;
+}
+
+// globalsort.glsl:64 as a gather: out[g] = in[perm[g]] (bit-preserving).
+__global__ __launch_bounds__(BLOCK) void k_gather(int32_t n, const uint32_t *__restrict__ perm,
+                                                  const float4 *__restrict__ in, float4 *__restrict__ out) {
+    const int32_t g = blockIdx.x * BLOCK + threadIdx.x;
+    if (g >= n) return;
+    out[g] = in[perm[g]];
+}
+
+__global__ __launch_bounds__(BLOCK) void k_gather_by_id(int32_t n, const float4 *__restrict__ rec,
+                                                        const float4 *__restrict__ src, float4 *__restrict__ out) {
+    const int32_t g = blockIdx.x * BLOCK + threadIdx.x;
+    if (g >= n) return;
+    out[g] = src[id_of(rec[g])];
+}
+
+__global__ __launch_bounds__(BLOCK) void k_fill_i32(int32_t *__restrict__ p, int32_t v, size_t count) {
+    const size_t stride = (size_t)gridDim.x * BLOCK * 4;
+    for (size_t i = ((size_t)blockIdx.x * BLOCK + threadIdx.x) * 4; i < count; i += stride) {
+        if (i + 4 <= count) {
+            *reinterpret_cast<int4 *>(p + i) = make_int4(v, v, v, v);
+        } else {
+            for (size_t k = i; k < count; k++) p[k] = v;
+        }
+    }
+}
+
+// ----------------------------------------------------------------- cells
+__device__ __forceinline__ int64_t clamped_cell_index(const SolverConsts &c, const float4 &r) {
+    const int32_t x = (int32_t)clamp_glsl(r.x, 0.0f, (float)c.gx);
+    const int32_t y = (int32_t)clamp_glsl(r.y, 0.0f, (float)c.gy);
+    const int32_t z = (int32_t)clamp_glsl(r.z, 0.0f, (float)c.gz);
+    if (x >= c.gx || y >= c.gy || z >= c.gz) return -1;  // imageStore out of range: dropped
+    return (int64_t)x + (int64_t)c.gx * ((int64_t)y + (int64_t)c.gy * z);
+}
+
+__device__ __forceinline__ bool same_cell(const SolverConsts &c, const float4 &a, const float4 &b) {
+    return (int32_t)clamp_glsl(a.x, 0.0f, (float)c.gx) == (int32_t)clamp_glsl(b.x, 0.0f, (float)c.gx) &&
+           (int32_t)clamp_glsl(a.y, 0.0f, (float)c.gy) == (int32_t)clamp_glsl(b.y, 0.0f, (float)c.gy) &&
+           (int32_t)clamp_glsl(a.z, 0.0f, (float)c.gz) == (int32_t)clamp_glsl(b.z, 0.0f, (float)c.gz);
+}
+
+// findcells.glsl:34-53. The reference writes in dispatch order, so with
+// colliding keys the highest gid is the last writer: atomicMax reproduces it
+// (start is cleared to -1 first; end carries the step tag in its high word).
+__global__ __launch_bounds__(BLOCK) void k_findcells(SolverConsts c, const float4 *__restrict__ sorted,
+                                                     int32_t *__restrict__ start,
+                                                     unsigned long long *__restrict__ end64) {
+    const int32_t g = blockIdx.x * BLOCK + threadIdx.x;
+    if (g >= c.n) return;
+    if (g == 0) {
+        atomicMax(&start[0], 0);  // imageStore (gridtexture, ivec3 (0, 0, 0), 0)
+        return;
+    }
+    const float4 a = sorted[g], b = sorted[g - 1];
+    if (same_cell(c, a, b)) return;
+    const int64_t ca = clamped_cell_index(c, a), cb = clamped_cell_index(c, b);
+    if (ca >= 0) atomicMax(&start[ca], g);
+    if (cb >= 0) atomicMax(&end64[cb], ((unsigned long long)c.epoch << 32) | (unsigned long long)(uint32_t)g);
+}
+
+__device__ __forceinline__ bool in_grid(const SolverConsts &c, int32_t x, int32_t y, int32_t z) {
+    return x >= 0 && y >= 0 && z >= 0 && x < c.gx && y < c.gy && z < c.gz;
+}
+
+// neighbourcells.glsl:52-92: per particle, 9 (dy,dz) rows merging x-1..x+1;
+// texel fetches outside the grid read 0 (as llvmpipe does).
+__global__ __launch_bounds__(BLOCK) void k_neighbourcells(SolverConsts c, const float4 *__restrict__ sorted,
+                                                          const int32_t *__restrict__ start,
+                                                          const unsigned long long *__restrict__ end64,
+                                                          int32_t *__restrict__ nbr9) {
+    const int32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= c.n) return;
+    const float4 p = sorted[i];
+    const int32_t px = (int32_t)p.x, py = (int32_t)p.y, pz = (int32_t)p.z;  // ivec3 (pos): truncation
+#pragma unroll
+    for (int o = 0; o < 9; o++) {
+        const int32_t y = py + (o / 3 - 1), z = pz + (o % 3 - 1);
+        int32_t cell = -1, entries = 0;
+#pragma unroll
+        for (int dx = -1; dx <= 1; dx++) {
+            const int32_t x = px + dx;
+            const bool inb = in_grid(c, x, y, z);
+            const size_t idx = inb ? (size_t)x + (size_t)c.gx * ((size_t)y + (size_t)c.gy * (size_t)z) : 0;
+            const int32_t s = inb ? start[idx] : 0;
+            if (cell == -1) cell = s;
+            if (s != -1) {
+                const int32_t e = inb ? (int32_t)(uint32_t)(end64[idx] & 0xFFFFFFFFull) : 0;
+                entries += e - s;
+            }
+        }
+        nbr9[(size_t)o * c.n + i] = (int32_t)((uint32_t)cell + ((uint32_t)entries << 24));
+    }
+}
+
+// ----------------------------------------------------------------- solver
+// Neighbour record j of the packed ranges; indices past n (a wrapped count)
+// read a zero record, as the reference's robust buffer access does.
+__device__ __forceinline__ float4 rec_at(const float4 *__restrict__ rec, int32_t j, int32_t n) {
+    return j < n ? rec[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// calclambda.glsl:66-103 (FOR_EACH_NEIGHBOUR, foreachneighbour.glsl:1-10).
+// Candidates with fl(sqrt(sq)) > h add exact zeros in the reference and are
+// skipped here; the cheap sq > h2cut test only ever skips such candidates.
+__global__ __launch_bounds__(BLOCK) void k_lambda(SolverConsts c, const float4 *__restrict__ rec,
+                                                  const int32_t *__restrict__ nbr9, float *__restrict__ lambda) {
+    const int32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= c.n) return;
+    const float4 pi = rec[i];
+    float rho = 0.0f, sk = 0.0f, gix = 0.0f, giy = 0.0f, giz = 0.0f;
+    for (int o = 0; o < 9; o++) {
+        const int32_t d = nbr9[(size_t)o * c.n + i];
+        const int32_t e = d >> 24, s = d & 0xFFFFFF;
+        for (int32_t j = s; j < s + e; j++) {
+            if (j == i) continue;
+            const float4 pj = rec_at(rec, j, c.n);
+            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const float sq = sq_len(dx, dy, dz);
+            if (sq > c.h2cut) continue;
+            const float r = sqrtf(sq);
+            if (r > c.h) continue;
+            rho += wpoly6(c, sq, r);
+            if (r == 0.0f) continue;
+            float gx, gy, gz;
+            grad_spiky(c, dx, dy, dz, r, gx, gy, gz);
+            gx = gx * c.inv_rho0;
+            gy = gy * c.inv_rho0;
+            gz = gz * c.inv_rho0;
+            sk += sq_len(gx, gy, gz);
+            gix += gx;
+            giy += gy;
+            giz += gz;
+        }
+    }
+    sk += sq_len(gix, giy, giz);
+    const float ci = rho * c.inv_rho0 - 1.0f;
+    lambda[i] = -ci / (sk + c.eps);
+}
+
+// updatepos.glsl:42-105, race-free: neighbours read from rec_in, result to rec_out.
+__global__ __launch_bounds__(BLOCK) void k_updatepos(SolverConsts c, const float4 *__restrict__ rec_in,
+                                                     const float *__restrict__ lambda,
+                                                     const int32_t *__restrict__ nbr9,
+                                                     float4 *__restrict__ rec_out) {
+    const int32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= c.n) return;
+    const float4 pi = rec_in[i];
+    const float li = lambda[i];
+    float dpx = 0.0f, dpy = 0.0f, dpz = 0.0f;
+    for (int o = 0; o < 9; o++) {
+        const int32_t d = nbr9[(size_t)o * c.n + i];
+        const int32_t e = d >> 24, s = d & 0xFFFFFF;
+        for (int32_t j = s; j < s + e; j++) {
+            if (j == i) continue;
+            const float4 pj = rec_at(rec_in, j, c.n);
+            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const float sq = sq_len(dx, dy, dz);
+            if (sq > c.h2cut) continue;
+            const float r = sqrtf(sq);
+            if (r > c.h || r == 0.0f) continue;
+            float sc = c.scale * wpoly6(c, sq, r);
+            sc *= sc;
+            sc *= sc;
+            sc = -c.k * sc;
+            const float lj = j < c.n ? lambda[j] : 0.0f;
+            float gx, gy, gz;
+            grad_spiky(c, dx, dy, dz, r, gx, gy, gz);
+            const float f = li + lj + sc;
+            dpx += f * gx;
+            dpy += f * gy;
+            dpz += f * gz;
+        }
+    }
+    float x = pi.x + c.inv_rho0 * dpx, y = pi.y + c.inv_rho0 * dpy, z = pi.z + c.inv_rho0 * dpz;
+    x = clamp_glsl(x, c.lo[0], c.hi[0]);
+    y = clamp_glsl(y, c.lo[1], c.hi[1]);
+    z = clamp_glsl(z, c.lo[2], c.hi[2]);
+    rec_out[i] = make_float4(x, y, z, pi.w);
+}
+
+// update.glsl:16-28; also keeps the velocities in sorted order for the
+// vorticity passes.
+__global__ __launch_bounds__(BLOCK) void k_update(SolverConsts c, const float4 *__restrict__ rec,
+                                                  float4 *__restrict__ pos, float4 *__restrict__ vel,
+                                                  float4 *__restrict__ vel_sorted) {
+    const int32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= c.n) return;
+    const float4 r = rec[i];
+    const int32_t id = id_of(r);
+    const float4 old = pos[id];
+    const float4 v = make_float4((r.x - old.x) / c.dt, (r.y - old.y) / c.dt, (r.z - old.z) / c.dt, 0.0f);
+    pos[id] = make_float4(r.x, r.y, r.z, 0.0f);
+    vel[id] = v;
+    if (vel_sorted) vel_sorted[i] = v;
+}
+
+// vorticity.glsl:34-60 (first half): XSPH and omega from the velocities before
+// this pass; |omega| kept in omega.w.
+__global__ __launch_bounds__(BLOCK) void k_vorticity_a(SolverConsts c, const float4 *__restrict__ rec,
+                                                       const int32_t *__restrict__ nbr9,
+                                                       const float4 *__restrict__ vel_sorted,
+                                                       const float4 *__restrict__ vel_id,
+                                                       float4 *__restrict__ vxsph, float4 *__restrict__ omega) {
+    const int32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= c.n) return;
+    const float4 pi = rec[i];
+    const float4 vi = vel_sorted[i];
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f, wx = 0.0f, wy = 0.0f, wz = 0.0f;
+    for (int o = 0; o < 9; o++) {
+        const int32_t d = nbr9[(size_t)o * c.n + i];
+        const int32_t e = d >> 24, s = d & 0xFFFFFF;
+        for (int32_t j = s; j < s + e; j++) {
+            if (j == i) continue;
+            const float4 pj = rec_at(rec, j, c.n);
+            const float px = pi.x - pj.x, py = pi.y - pj.y, pz = pi.z - pj.z;
+            const float sq = sq_len(px, py, pz);
+            if (sq > c.h2cut) continue;
+            const float r = sqrtf(sq);
+            if (r > c.h) continue;
+            // a zero record past n has id 0: imageLoad (velocitytexture, 0)
+            const float4 vj = j < c.n ? vel_sorted[j] : vel_id[0];
+            const float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
+            const float w = wpoly6(c, sq, r);
+            sx += ux * w;
+            sy += uy * w;
+            sz += uz * w;
+            if (r == 0.0f) continue;
+            float gx, gy, gz;
+            grad_spiky(c, px, py, pz, r, gx, gy, gz);
+            wx = (wx + uy * gz) - uz * gy;
+            wy = (wy + uz * gx) - ux * gz;
+            wz = (wz + ux * gy) - uy * gx;
+        }
+    }
+    vxsph[i] = make_float4(vi.x + c.xsph_c * sx, vi.y + c.xsph_c * sy, vi.z + c.xsph_c * sz, 0.0f);
+    omega[i] = make_float4(wx, wy, wz, sqrtf(sq_len(wx, wy, wz)));
+}
+
+// vorticity.glsl:65-85 (second half): confinement from this step's |omega|.
+__global__ __launch_bounds__(BLOCK) void k_vorticity_b(SolverConsts c, const float4 *__restrict__ rec,
+                                                       const int32_t *__restrict__ nbr9,
+                                                       const float4 *__restrict__ vxsph,
+                                                       const float4 *__restrict__ omega,
+                                                       float4 *__restrict__ vel_id) {
+    const int32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= c.n) return;
+    const float4 pi = rec[i];
+    float nx = 0.0f, ny = 0.0f, nz = 0.0f;
+    for (int o = 0; o < 9; o++) {
+        const int32_t d = nbr9[(size_t)o * c.n + i];
+        const int32_t e = d >> 24, s = d & 0xFFFFFF;
+        for (int32_t j = s; j < s + e; j++) {
+            if (j == i) continue;
+            const float4 pj = rec_at(rec, j, c.n);
+            const float px = pi.x - pj.x, py = pi.y - pj.y, pz = pi.z - pj.z;
+            const float sq = sq_len(px, py, pz);
+            if (sq > c.h2cut) continue;
+            const float r = sqrtf(sq);
+            if (r > c.h || r == 0.0f) continue;
+            float gx, gy, gz;
+            grad_spiky(c, px, py, pz, r, gx, gy, gz);
+            const float om = j < c.n ? omega[j].w : 0.0f;
+            nx += om * gx;
+            ny += om * gy;
+            nz += om * gz;
+        }
+    }
+    const float l = sqrtf(sq_len(nx, ny, nz));
+    if (l > 0.0f) {
+        nx = nx / l;
+        ny = ny / l;
+        nz = nz / l;
+    }
+    const float4 w = omega[i];
+    const float4 v = vxsph[i];
+    const float s = c.dt * c.vort_eps;
+    const float ox = v.x + s * (ny * w.z - nz * w.y);
+    const float oy = v.y + s * (nz * w.x - nx * w.z);
+    const float oz = v.z + s * (nx * w.y - ny * w.x);
+    vel_id[id_of(pi)] = make_float4(ox, oy, oz, 0.0f);
+}
+
+// ----------------------------------------------------------------- layout conversions
+__global__ __launch_bounds__(BLOCK) void k_nbr9_to_12(int32_t n, const int32_t *__restrict__ nbr9,
+                                                      int32_t *__restrict__ nbr12) {
+    const int32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    for (int r = 0; r < 3; r++) {
+        int4 v = make_int4(nbr9[(size_t)(3 * r) * n + i], nbr9[(size_t)(3 * r + 1) * n + i],
+                           nbr9[(size_t)(3 * r + 2) * n + i], 0);
+        reinterpret_cast<int4 *>(nbr12)[(size_t)3 * i + r] = v;
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_nbr12_to_9(int32_t n, const int32_t *__restrict__ nbr12,
+                                                      int32_t *__restrict__ nbr9) {
+    const int32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    for (int r = 0; r < 3; r++) {
+        const int4 v = reinterpret_cast<const int4 *>(nbr12)[(size_t)3 * i + r];
+        nbr9[(size_t)(3 * r) * n + i] = v.x;
+        nbr9[(size_t)(3 * r + 1) * n + i] = v.y;
+        nbr9[(size_t)(3 * r + 2) * n + i] = v.z;
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_end64_to_i32(size_t cells, const unsigned long long *__restrict__ e64,
+                                                        int32_t *__restrict__ e) {
+    for (size_t k = (size_t)blockIdx.x * BLOCK + threadIdx.x; k < cells; k += (size_t)gridDim.x * BLOCK)
+        e[k] = (int32_t)(uint32_t)(e64[k] & 0xFFFFFFFFull);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_i32_to_end64(size_t cells, const int32_t *__restrict__ e,
+                                                        unsigned long long *__restrict__ e64) {
+    for (size_t k = (size_t)blockIdx.x * BLOCK + threadIdx.x; k < cells; k += (size_t)gridDim.x * BLOCK)
+        e64[k] = (unsigned long long)(uint32_t)e[k];
+}
+
+__global__ __launch_bounds__(BLOCK) void k_omega_norm(int32_t n, const float4 *__restrict__ omega,
+                                                      float *__restrict__ vort) {
+    const int32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    vort[i] = omega[i].w;
+}
+
+inline unsigned blocks_for(size_t n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
+inline unsigned grid_stride_blocks(size_t n) {
+    size_t b = (n + BLOCK - 1) / BLOCK;
+    return (unsigned)(b > 65536 ? 65536 : (b == 0 ? 1 : b));
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------- launchers
+void launch_predict(const SolverConsts &c, const float4 *pos, const float4 *vel, float4 *rec, uint32_t *keys,
+                    hipStream_t st) {
+    if (c.n > 0) k_predict<<<blocks_for(c.n), BLOCK, 0, st>>>(c, pos, vel, rec, keys);
+}
+void launch_keys_of_records(const SolverConsts &c, const float4 *rec, uint32_t *keys, hipStream_t st) {
+    if (c.n > 0) k_keys<<<blocks_for(c.n), BLOCK, 0, st>>>(c, rec, keys);
+}
+void launch_gather_records(int32_t n, const uint32_t *perm, const float4 *in, float4 *out, hipStream_t st) {
+    if (n > 0) k_gather<<<blocks_for(n), BLOCK, 0, st>>>(n, perm, in, out);
+}
+void launch_gather_by_id(int32_t n, const float4 *rec, const float4 *src, float4 *out, hipStream_t st) {
+    if (n > 0) k_gather_by_id<<<blocks_for(n), BLOCK, 0, st>>>(n, rec, src, out);
+}
+void launch_fill_i32(int32_t *p, int32_t value, size_t count, hipStream_t st) {
+    if (count > 0) k_fill_i32<<<grid_stride_blocks((count + 3) / 4), BLOCK, 0, st>>>(p, value, count);
+}
+void launch_findcells(const SolverConsts &c, const float4 *sorted, int32_t *start, unsigned long long *end64,
+                      hipStream_t st) {
+    if (c.n > 0) k_findcells<<<blocks_for(c.n), BLOCK, 0, st>>>(c, sorted, start, end64);
+}
+void launch_neighbourcells(const SolverConsts &c, const float4 *sorted, const int32_t *start,
+                           const unsigned long long *end64, int32_t *nbr9, hipStream_t st) {
+    if (c.n > 0) k_neighbourcells<<<blocks_for(c.n), BLOCK, 0, st>>>(c, sorted, start, end64, nbr9);
+}
+void launch_lambda(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, float *lambda, hipStream_t st) {
+    if (c.n > 0) k_lambda<<<blocks_for(c.n), BLOCK, 0, st>>>(c, rec, nbr9, lambda);
+}
+void launch_updatepos(const SolverConsts &c, const float4 *rec_in, const float *lambda, const int32_t *nbr9,
+                      float4 *rec_out, hipStream_t st) {
+    if (c.n > 0) k_updatepos<<<blocks_for(c.n), BLOCK, 0, st>>>(c, rec_in, lambda, nbr9, rec_out);
+}
+void launch_update(const SolverConsts &c, const float4 *rec, float4 *pos, float4 *vel, float4 *vel_sorted,
+                   hipStream_t st) {
+    if (c.n > 0) k_update<<<blocks_for(c.n), BLOCK, 0, st>>>(c, rec, pos, vel, vel_sorted);
+}
+void launch_vorticity_a(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, const float4 *vel_sorted,
+                        const float4 *vel_id, float4 *vxsph, float4 *omega, hipStream_t st) {
+    if (c.n > 0) k_vorticity_a<<<blocks_for(c.n), BLOCK, 0, st>>>(c, rec, nbr9, vel_sorted, vel_id, vxsph, omega);
+}
+void launch_vorticity_b(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, const float4 *vxsph,
+                        const float4 *omega, float4 *vel_id, hipStream_t st) {
+    if (c.n > 0) k_vorticity_b<<<blocks_for(c.n), BLOCK, 0, st>>>(c, rec, nbr9, vxsph, omega, vel_id);
+}
+void launch_nbr9_to_12(int32_t n, const int32_t *nbr9, int32_t *nbr12, hipStream_t st) {
+    if (n > 0) k_nbr9_to_12<<<blocks_for(n), BLOCK, 0, st>>>(n, nbr9, nbr12);
+}
+void launch_nbr12_to_9(int32_t n, const int32_t *nbr12, int32_t *nbr9, hipStream_t st) {
+    if (n > 0) k_nbr12_to_9<<<blocks_for(n), BLOCK, 0, st>>>(n, nbr12, nbr9);
+}
+void launch_end64_to_i32(size_t cells, const unsigned long long *end64, int32_t *end, hipStream_t st) {
+    if (cells > 0) k_end64_to_i32<<<grid_stride_blocks(cells), BLOCK, 0, st>>>(cells, end64, end);
+}
+void launch_i32_to_end64(size_t cells, const int32_t *end, unsigned long long *end64, hipStream_t st) {
+    if (cells > 0) k_i32_to_end64<<<grid_stride_blocks(cells), BLOCK, 0, st>>>(cells, end, end64);
+}
+void launch_omega_norm(int32_t n, const float4 *omega, float *vort, hipStream_t st) {
+    if (n > 0) k_omega_norm<<<blocks_for(n), BLOCK, 0, st>>>(n, omega, vort);
+}
+
+}  // namespace pbf

@@ -1,0 +1,183 @@
+// Stable LSD radix sort of (cell key, particle index) pairs for gfx950.
+//
+// Replaces the reference's 2-bit LSD sort (RadixSort::Run / SortBits,
+// src/RadixSort.cpp:124-200; counting.glsl, blockscan.glsl, addblocksum.glsl,
+// globalsort.glsl). Both are stable sorts by the key bits [0, key_bits), so the
+// permutation is identical (ties in ascending index). Here: 8-bit digits, so
+// ceil(key_bits / 8) passes instead of key_bits / 2, and each pass is
+//   upsweep   per-tile digit counts (wave64 ballot match, one LDS add per
+//             distinct digit per wave), digit-major like the reference's
+//             blocksum layout (counting.glsl:96-101);
+//   scan      one exclusive scan of the 256 x tiles counts;
+//   downsweep stable in-tile ranks (ballot match + per-(item, wave) digit
+//             offsets scanned in LDS) and the scatter.
+// A tile is 256 threads x 8 items = 2048 keys, element order (item, wave, lane).
+#include "pbf_internal.h"
+
+namespace pbf {
+namespace {
+
+constexpr int RS_THREADS = 256;
+constexpr int RS_WAVES = RS_THREADS / 64;
+constexpr int RS_ITEMS = 8;
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
+constexpr int SCAN_THREADS = 1024;
+
+__device__ __forceinline__ unsigned lane_id() { return threadIdx.x & 63u; }
+
+// Lanes of the wave whose 8-bit digit equals this lane's (valid lanes only).
+__device__ __forceinline__ unsigned long long match_digit(uint32_t d, bool valid) {
+    unsigned long long m = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+        const bool bit = (d >> b) & 1u;
+        const unsigned long long bb = __ballot(bit);
+        m &= bit ? bb : ~bb;
+    }
+    return m;
+}
+
+__global__ __launch_bounds__(RS_THREADS) void k_upsweep(const uint32_t *__restrict__ keys, int32_t n, int shift,
+                                                        int32_t tiles, uint32_t *__restrict__ counts) {
+    __shared__ uint32_t hist[256];
+    hist[threadIdx.x] = 0;
+    __syncthreads();
+    const int32_t base = blockIdx.x * RS_TILE;
+    const unsigned long long lt = (1ull << lane_id()) - 1ull;
+#pragma unroll
+    for (int k = 0; k < RS_ITEMS; k++) {
+        const int32_t idx = base + k * RS_THREADS + threadIdx.x;
+        const bool valid = idx < n;
+        const uint32_t d = valid ? (keys[idx] >> shift) & 255u : 0u;
+        const unsigned long long m = match_digit(d, valid);
+        if (valid && (m & lt) == 0) atomicAdd(&hist[d], (uint32_t)__popcll(m));
+    }
+    __syncthreads();
+    counts[(size_t)threadIdx.x * tiles + blockIdx.x] = hist[threadIdx.x];
+}
+
+// Exclusive scan of counts[0, total) in one workgroup (contiguous chunk per thread).
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan(uint32_t *__restrict__ counts, int32_t total) {
+    __shared__ uint32_t wsum[SCAN_THREADS / 64];
+    const int32_t chunk = (total + SCAN_THREADS - 1) / SCAN_THREADS;
+    const int32_t lo = threadIdx.x * chunk;
+    const int32_t hi = lo + chunk < total ? lo + chunk : total;
+    uint32_t sum = 0;
+    for (int32_t k = lo; k < hi; k++) sum += counts[k];
+    // inclusive scan of the thread sums: within the wave, then across waves
+    uint32_t inc = sum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(inc, off, 64);
+        if (lane_id() >= (unsigned)off) inc += t;
+    }
+    const int w = threadIdx.x >> 6;
+    if (lane_id() == 63) wsum[w] = inc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int k = 0; k < SCAN_THREADS / 64; k++) {
+            const uint32_t t = wsum[k];
+            wsum[k] = run;
+            run += t;
+        }
+    }
+    __syncthreads();
+    uint32_t run = wsum[w] + inc - sum;
+    for (int32_t k = lo; k < hi; k++) {
+        const uint32_t t = counts[k];
+        counts[k] = run;
+        run += t;
+    }
+}
+
+__global__ __launch_bounds__(RS_THREADS) void k_downsweep(const uint32_t *__restrict__ keys_in,
+                                                          const uint32_t *__restrict__ vals_in, int32_t n, int shift,
+                                                          int32_t tiles, const uint32_t *__restrict__ offsets,
+                                                          uint32_t *__restrict__ keys_out,
+                                                          uint32_t *__restrict__ vals_out) {
+    __shared__ uint32_t cnt[RS_ITEMS * RS_WAVES][256];
+    for (int s = 0; s < RS_ITEMS * RS_WAVES; s++) cnt[s][threadIdx.x] = 0;
+    const uint32_t tile_base = offsets[(size_t)threadIdx.x * tiles + blockIdx.x];
+    __syncthreads();
+    const int32_t base = blockIdx.x * RS_TILE;
+    const int w = threadIdx.x >> 6;
+    const unsigned long long lt = (1ull << lane_id()) - 1ull;
+    uint32_t key[RS_ITEMS], val[RS_ITEMS], rank[RS_ITEMS];
+#pragma unroll
+    for (int k = 0; k < RS_ITEMS; k++) {
+        const int32_t idx = base + k * RS_THREADS + threadIdx.x;
+        const bool valid = idx < n;
+        key[k] = valid ? keys_in[idx] : 0u;
+        val[k] = valid ? (vals_in ? vals_in[idx] : (uint32_t)idx) : 0u;
+        const uint32_t d = (key[k] >> shift) & 255u;
+        const unsigned long long m = match_digit(d, valid);
+        rank[k] = (uint32_t)__popcll(m & lt);
+        if (valid && rank[k] == 0) cnt[k * RS_WAVES + w][d] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    {
+        uint32_t run = tile_base;
+        for (int s = 0; s < RS_ITEMS * RS_WAVES; s++) {
+            const uint32_t t = cnt[s][threadIdx.x];
+            cnt[s][threadIdx.x] = run;
+            run += t;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < RS_ITEMS; k++) {
+        const int32_t idx = base + k * RS_THREADS + threadIdx.x;
+        if (idx >= n) continue;
+        const uint32_t d = (key[k] >> shift) & 255u;
+        const uint32_t dst = cnt[k * RS_WAVES + w][d] + rank[k];
+        if (keys_out) keys_out[dst] = key[k];
+        vals_out[dst] = val[k];
+    }
+}
+
+}  // namespace
+
+bool radix_alloc(RadixScratch &s, int32_t n) {
+    radix_free(s);
+    const int32_t tiles = (n + RS_TILE - 1) / RS_TILE;
+    const size_t nn = (size_t)(n > 0 ? n : 1);
+    if (hipMalloc(&s.keys_alt, nn * 4) != hipSuccess) return false;
+    if (hipMalloc(&s.vals, nn * 4) != hipSuccess) return false;
+    if (hipMalloc(&s.vals_alt, nn * 4) != hipSuccess) return false;
+    if (hipMalloc(&s.counts, (size_t)256 * (tiles > 0 ? tiles : 1) * 4) != hipSuccess) return false;
+    s.capacity = n;
+    s.tiles = tiles;
+    return true;
+}
+
+void radix_free(RadixScratch &s) {
+    if (s.keys_alt) (void)hipFree(s.keys_alt);
+    if (s.vals) (void)hipFree(s.vals);
+    if (s.vals_alt) (void)hipFree(s.vals_alt);
+    if (s.counts) (void)hipFree(s.counts);
+    s = RadixScratch();
+}
+
+void radix_sort_perm(RadixScratch &s, uint32_t *keys, int32_t n, int key_bits, uint32_t *perm, hipStream_t st) {
+    if (n <= 0) return;
+    const int32_t tiles = (n + RS_TILE - 1) / RS_TILE;
+    const int passes = key_bits <= 8 ? 1 : (key_bits + 7) / 8;
+    uint32_t *kin = keys, *kout = s.keys_alt;
+    const uint32_t *vin = nullptr;
+    uint32_t *vbuf[2] = {s.vals, s.vals_alt};
+    for (int p = 0; p < passes; p++) {
+        const bool last = p == passes - 1;
+        uint32_t *vout = last ? perm : vbuf[p & 1];
+        hipLaunchKernelGGL(k_upsweep, dim3(tiles), dim3(RS_THREADS), 0, st, kin, n, 8 * p, tiles, s.counts);
+        hipLaunchKernelGGL(k_scan, dim3(1), dim3(SCAN_THREADS), 0, st, s.counts, 256 * tiles);
+        hipLaunchKernelGGL(k_downsweep, dim3(tiles), dim3(RS_THREADS), 0, st, kin, vin, n, 8 * p, tiles,
+                           s.counts, last ? nullptr : kout, vout);
+        vin = vout;
+        uint32_t *t = kin;
+        kin = kout;
+        kout = t;
+    }
+}
+
+}  // namespace pbf

@@ -1,0 +1,172 @@
+"""HIP path against the golden fixtures (the reference's shaders on llvmpipe) and
+the oracle, through the C ABI.
+
+Bar: bit-exact everywhere. Keys, permutation and tables are integers; the FP32
+stages follow the reference's operation order without fused multiply-add, so
+the race-free stages equal the fixtures bit for bit and the race-free
+replacements of the reference's in-place passes (updatepos, vorticity) equal
+the oracle's race-free order bit for bit."""
+import numpy as np
+import pytest
+
+import oracle as O
+from pbf_amd import scene as S
+from pbf_amd import sph as P
+from tests import golden_util as G
+from tests import gpu_util as U
+
+pytestmark = pytest.mark.gpu
+
+FIXTURES = G.NAMES
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def bits(a):
+    a = np.ascontiguousarray(a)
+    return a.view(np.int32) if a.dtype == np.float32 else a
+
+
+def assert_bitexact(got, want, what):
+    got, want = bits(got), bits(want)
+    assert got.shape == want.shape, f"{what}: shape {got.shape} vs {want.shape}"
+    bad = np.nonzero(got.ravel() != want.ravel())[0]
+    assert bad.size == 0, f"{what}: {bad.size} of {got.size} differ, first at {bad[:8]}"
+
+
+def fixture_params(g):
+    p = G.params_of(g)
+    sc = S.Scene("fx", tuple(int(x) for x in g["grid"]), p, g["pos0"], g["vel0"])
+    return P.scene_params(sc), O.make_cfg(g["pos0"].shape[0], sc.grid, p)
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_stages_against_fixtures(gpu_lib, name):
+    g = G.load(name)
+    pp, cfg = fixture_params(g)
+    grid = tuple(int(x) for x in g["grid"])
+    n = g["pos0"].shape[0]
+    st = U.stream()
+    for s in [int(x) for x in g["steps_dumped"]]:
+        p = f"s{s}_"
+        say(f"[{name} step {s}] n={n} grid={grid}")
+        pos, vel, end0, vort0 = G.step_inputs(g, s)
+        # predict
+        dpos, dvel = U.dev(pos), U.dev(vel)
+        rec = U.empty_like_host((n, 4), np.float32)
+        P.stage_predict(n, grid, pp, U.ptr(dpos), U.ptr(dvel), U.ptr(rec), st)
+        assert_bitexact(U.host(rec), g[p + "predict"], f"{name} {s} predict")
+        # sort (RadixSort class)
+        rs = P.RadixSort(512, n // 512, grid)
+        U.copy_to_device(rs.GetBuffer(), g[p + "predict"], rs.stream())
+        rs.Run()
+        sorted_rec = U.copy_from_device(rs.GetBuffer(), (n, 4), np.float32, rs.stream())
+        assert_bitexact(sorted_rec, G.sorted_records(g, s), f"{name} {s} sorted records")
+        perm = U.copy_from_device(rs.permutation_ptr(), (n,), np.int32, rs.stream())
+        assert_bitexact(perm, g[p + "perm"], f"{name} {s} permutation")
+        rs.close()
+        # cells (NeighbourCellFinder class)
+        f = P.NeighbourCellFinder(n, grid)
+        f.set_end_table(end0)
+        dsorted = U.dev(G.sorted_records(g, s))
+        f.FindNeighbourCells(U.ptr(dsorted))
+        start, end = f.tables()
+        assert_bitexact(start, G.dense(g, p, "start", -1), f"{name} {s} start")
+        assert_bitexact(end, G.dense(g, p, "end", 0), f"{name} {s} end")
+        nbr = U.copy_from_device(f.GetResult(), (n, 3, 4), np.int32, f.stream())
+        assert_bitexact(nbr, g[p + "nbr"], f"{name} {s} neighbour table")
+        f.close()
+        # solver stages from the fixture's own inputs
+        dnbr = U.dev(g[p + "nbr"])
+        prev = G.sorted_records(g, s)
+        lam = U.empty_like_host((n,), np.float32)
+        out = U.empty_like_host((n, 4), np.float32)
+        for it in range(int(g["iterations"])):
+            dprev = U.dev(prev)
+            P.stage_lambda(n, grid, pp, U.ptr(dprev), U.ptr(dnbr), U.ptr(lam), st)
+            assert_bitexact(U.host(lam), g[p + "lambda"][it], f"{name} {s} lambda {it}")
+            dlam = U.dev(g[p + "lambda"][it])
+            P.stage_updatepos(n, grid, pp, U.ptr(dprev), U.ptr(dlam), U.ptr(dnbr), U.ptr(out), st)
+            want = O.updatepos(cfg, prev, g[p + "lambda"][it], g[p + "nbr"], O.ORDER_JACOBI)
+            assert_bitexact(U.host(out), want, f"{name} {s} updatepos {it} (race-free)")
+            prev = G.solved_records(g, s, it)
+        dprev = U.dev(prev)
+        dpos, dvel = U.dev(pos), U.dev(vel)
+        P.stage_update(n, pp, U.ptr(dprev), U.ptr(dpos), U.ptr(dvel), st)
+        assert_bitexact(U.host(dpos), g[p + "upd_pos"], f"{name} {s} update pos")
+        assert_bitexact(U.host(dvel), g[p + "upd_vel"], f"{name} {s} update vel")
+        if p + "vort" in g:
+            dvel = U.dev(g[p + "upd_vel"])
+            vort = U.empty_like_host((n,), np.float32)
+            P.stage_vorticity(n, grid, pp, U.ptr(dprev), U.ptr(dnbr), U.ptr(dvel), U.ptr(vort), st)
+            wv, wo = O.vorticity(cfg, prev, g[p + "nbr"], g[p + "upd_vel"], vort0, O.ORDER_JACOBI)
+            assert_bitexact(U.host(vort), wo, f"{name} {s} |omega| (race-free)")
+            assert_bitexact(U.host(dvel), wv, f"{name} {s} vorticity velocity (race-free)")
+
+
+def run_sph(scene, steps, pparams=None):
+    sph = P.SPH(scene.n, scene.grid)
+    sph.set_params(pparams or P.scene_params(scene))
+    sph.upload(scene.pos, scene.vel)
+    for _ in range(steps):
+        sph.Run()
+    pos, vel = sph.download()
+    return sph, pos, vel
+
+
+def oracle_rollout(scene, steps):
+    cfg = O.cfg_for(scene)
+    pos, vel, end, vort = scene.pos, scene.vel, None, None
+    for _ in range(steps):
+        pos, vel, end, vort = O.step(cfg, pos, vel, end, vort, O.ORDER_JACOBI)
+    return pos, vel, end
+
+
+@pytest.mark.parametrize("name,steps", [("dam4k", 40), ("extforce", 3), ("reference_default", 3)])
+def test_rollout_matches_oracle(gpu_lib, name, steps):
+    """Whole SPH::Run steps on the GPU equal the oracle's race-free rollout bit for bit."""
+    g = G.load(name)
+    p = G.params_of(g)
+    sc = S.Scene(name, tuple(int(x) for x in g["grid"]), p, g["pos0"], g["vel0"])
+    say(f"[rollout {name}] n={sc.n} steps={steps}")
+    sph, pos, vel = run_sph(sc, steps)
+    wpos, wvel, wend = oracle_rollout(sc, steps)
+    assert_bitexact(pos, wpos, f"{name} positions after {steps} steps")
+    assert_bitexact(vel, wvel, f"{name} velocities after {steps} steps")
+    assert_bitexact(sph.end_table(), wend, f"{name} end table after {steps} steps")
+
+
+def test_dam_32k_rollout(gpu_lib):
+    """The 32,768-particle dam break (grid 64^3, 3 iterations): 25 steps bit-exact."""
+    sc = S.cfg_dam_32k()
+    say(f"[dam_32k] n={sc.n}")
+    _, pos, vel = run_sph(sc, 25)
+    wpos, wvel, _ = oracle_rollout(sc, 25)
+    assert_bitexact(pos, wpos, "dam_32k positions")
+    assert_bitexact(vel, wvel, "dam_32k velocities")
+
+
+def test_block_1m_one_step(gpu_lib):
+    """The 1,048,576-particle block (grid 256x128x256, 4 iterations): one step
+    bit-exact against the oracle, plus size-independent checks of the sort."""
+    sc = S.cfg_block_1m()
+    say(f"[block_1m] n={sc.n}")
+    sph, pos, vel = run_sph(sc, 1)
+    wpos, wvel, _ = oracle_rollout(sc, 1)
+    assert_bitexact(pos, wpos, "block_1m positions")
+    assert_bitexact(vel, wvel, "block_1m velocities")
+    rec = sph.sorted_records()
+    ids = rec[:, 3].view(np.int32)
+    assert np.array_equal(np.sort(ids), np.arange(sc.n)), "sorted ids are a permutation"
+
+
+def test_error_paths(gpu_lib):
+    """C-ABI errors: bad sizes and odd sort blocks fail with the reference's error kinds."""
+    with pytest.raises(P._lib.PbfError):
+        P.SPH(1000, (64, 64, 64))
+    with pytest.raises(P._lib.PbfLogicError):
+        P.RadixSort(511, 2, (64, 64, 64))
+    with pytest.raises(P._lib.PbfError):
+        P.SPH(512, (0, 64, 64))
