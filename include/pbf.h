@@ -94,6 +94,13 @@ float pbf_wpoly6(float r, float h);
  * src/RadixSort.cpp:24-30, 44, 127. */
 int pbf_key_bits(int32_t gx, int32_t gy, int32_t gz);
 
+/* Self-test of the fast correctly rounded FP32 primitives of the solver kernels
+ * against the general IEEE sequences on the current device (no reference
+ * counterpart): reciprocal over every float of [2^-60, 2^60) of either sign,
+ * square root over every float of [2^-96, 2^8), division over div_samples
+ * pseudo-random operand pairs. out[0..2]: mismatch counts, out[3..5]: cases. */
+int pbf_selftest_math(uint64_t div_samples, uint64_t seed, uint64_t out[6]);
+
 /* ---------------------------------------------------------------- SPH */
 /* SPH::SPH (src/SPH.h:40, src/SPH.cpp:24-151). numparticles must be a positive
  * multiple of 512, as the reference's dispatch sizes require (src/Simulation.cpp:202). */

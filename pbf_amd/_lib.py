@@ -49,6 +49,7 @@ SIGNATURES = {
     "pbf_params_default": ([pp, i32, i32, i32], None),
     "pbf_wpoly6": ([C.c_float, C.c_float], C.c_float),
     "pbf_key_bits": ([i32, i32, i32], C.c_int),
+    "pbf_selftest_math": ([C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64)], C.c_int),
     "pbf_sph_create": ([C.POINTER(vp), u32, i32, i32, i32], C.c_int),
     "pbf_sph_destroy": ([vp], C.c_int),
     "pbf_sph_set_params": ([vp, pp], C.c_int),

@@ -11,7 +11,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libpbf_amd.so")
-SOURCES = ["pbf_kernels.hip", "pbf_sort.hip", "pbf_batched.hip", "pbf_adjoint.hip", "pbf_capi.hip"]
+SOURCES = ["pbf_kernels.hip", "pbf_sort.hip", "pbf_selftest.hip", "pbf_batched.hip", "pbf_adjoint.hip",
+           "pbf_capi.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

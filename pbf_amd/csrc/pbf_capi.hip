@@ -34,6 +34,10 @@ int key_bits_of(const int32_t g[3]) {
     return (int)(2 * ((nb + 1) >> 1));
 }
 
+// Capacity of the per-particle neighbour list that the lambda pass records for
+// the position update (particles with more neighbours within h scan instead).
+constexpr int32_t kListCap = 64;
+
 SolverConsts make_consts(uint32_t n, const int32_t grid[3], const pbf_params &p, int key_bits) {
     SolverConsts c;
     std::memset(&c, 0, sizeof c);
@@ -48,6 +52,7 @@ SolverConsts make_consts(uint32_t n, const int32_t grid[3], const pbf_params &p,
     c.h_is_one = h == 1.0f;
     // Any float sq above h2cut has fl(sqrt(sq)) > h: h^2 (1 + 1e-5), rounded up.
     c.h2cut = std::nextafter((float)((double)h * (double)h * (1.0 + 1e-5)), INFINITY);
+    c.h2loose = std::nextafter((float)((double)c.h2cut * (1.0 + 0x1p-20)), INFINITY);
     c.inv_rho0 = p.one_over_rho_0;
     c.eps = p.epsilon;
     c.gravity = p.gravity;
@@ -215,9 +220,12 @@ struct pbf_sph {
     hipStream_t stream = nullptr;
     float4 *pos = nullptr, *vel = nullptr;       // id-order state
     float4 *xs = nullptr;                        // predicted records, id order
-    float4 *rec = nullptr, *rec_alt = nullptr;   // sorted records, Jacobi double buffer
+    float4 *rec0 = nullptr;                      // sorted records at x*
+    float4 *rec_a = nullptr, *rec_b = nullptr;   // sorted records of the solver iterations (Jacobi)
+    float4 *rec = nullptr;                       // sorted records after the last iteration (borrowed)
+    uint16_t *list_entries = nullptr, *list_count = nullptr;  // neighbours within h, lambda -> update
     float4 *vel_sorted = nullptr, *vxsph = nullptr, *omega = nullptr;
-    float *lambda = nullptr;
+    float *lambda = nullptr, *vort = nullptr;
     uint32_t *keys = nullptr, *perm = nullptr;
     CellTables cells_t;
     RadixScratch rs;
@@ -230,12 +238,17 @@ struct pbf_sph {
         dfree(pos);
         dfree(vel);
         dfree(xs);
-        dfree(rec);
-        dfree(rec_alt);
+        dfree(rec0);
+        dfree(rec_a);
+        dfree(rec_b);
+        rec = nullptr;
+        dfree(list_entries);
+        dfree(list_count);
         dfree(vel_sorted);
         dfree(vxsph);
         dfree(omega);
         dfree(lambda);
+        dfree(vort);
         dfree(keys);
         dfree(perm);
         cells_t.release();
@@ -361,12 +374,17 @@ int pbf_sph_create(pbf_sph **out, uint32_t n, int32_t gx, int32_t gy, int32_t gz
     TRY(dalloc(&s->pos, n));
     TRY(dalloc(&s->vel, n));
     TRY(dalloc(&s->xs, n));
-    TRY(dalloc(&s->rec, n));
-    TRY(dalloc(&s->rec_alt, n));
+    TRY(dalloc(&s->rec0, n));
+    TRY(dalloc(&s->rec_a, n));
+    TRY(dalloc(&s->rec_b, n));
+    s->rec = s->rec0;
+    TRY(dalloc(&s->list_entries, (size_t)kListCap * n));
+    TRY(dalloc(&s->list_count, n));
     TRY(dalloc(&s->vel_sorted, n));
     TRY(dalloc(&s->vxsph, n));
     TRY(dalloc(&s->omega, n));
     TRY(dalloc(&s->lambda, n));
+    TRY(dalloc(&s->vort, n));
     TRY(dalloc(&s->keys, n));
     TRY(dalloc(&s->perm, n));
     TRY(s->cells_t.alloc(n, s->cells));
@@ -448,31 +466,38 @@ int pbf_sph_run(pbf_sph *s) {
     PBF_HIP(hipEventRecord(s->ev[1], st));
     // sort (src/SPH.cpp:263-268)
     TIMED(KID_SORT, radix_sort_perm(s->rs, s->keys, n, s->key_bits, s->perm, st));
-    TIMED(KID_GATHER, launch_gather_records(n, s->perm, s->xs, s->rec, st));
+    TIMED(KID_GATHER, launch_gather_records(n, s->perm, s->xs, s->rec0, st));
     PBF_HIP(hipEventRecord(s->ev[2], st));
     // neighbour cells (src/SPH.cpp:270-275; src/NeighbourCellFinder.cpp:113-148)
+    CellTables &t = s->cells_t;
     {
-        CellTables &t = s->cells_t;
         SolverConsts cc = c;
         cc.epoch = ++t.epoch;
         TIMED(KID_CLEAR_START, launch_fill_i32(t.start, -1, t.cells, st));
-        TIMED(KID_FINDCELLS, launch_findcells(cc, s->rec, t.start, t.end64, st));
-        TIMED(KID_NEIGHBOURCELLS, launch_neighbourcells(cc, s->rec, t.start, t.end64, t.nbr9, st));
+        TIMED(KID_FINDCELLS, launch_findcells(cc, s->rec0, t.start, t.end64, st));
+        TIMED(KID_NEIGHBOURCELLS, launch_neighbourcells(cc, s->rec0, t.start, t.end64, t.nbr9, st));
     }
     PBF_HIP(hipEventRecord(s->ev[3], st));
-    // solver iterations (src/SPH.cpp:303-311)
+    // solver iterations (src/SPH.cpp:303-311), race-free: x* -> a -> b -> a ...
+    PairList pl;
+    pl.entries = s->list_entries;
+    pl.count = s->list_count;
+    pl.cap = kListCap;
+    const float4 *cur = s->rec0;
     for (uint32_t it = 0; it < s->params.iterations; it++) {
-        TIMED(KID_LAMBDA, launch_lambda(c, s->rec, s->cells_t.nbr9, s->lambda, st));
-        TIMED(KID_UPDATEPOS, launch_updatepos(c, s->rec, s->lambda, s->cells_t.nbr9, s->rec_alt, st));
-        std::swap(s->rec, s->rec_alt);
+        float4 *dst = (it & 1) ? s->rec_b : s->rec_a;
+        TIMED(KID_LAMBDA, launch_lambda(c, cur, t.nbr9, s->lambda, pl, st));
+        TIMED(KID_UPDATEPOS, launch_updatepos(c, cur, s->lambda, t.nbr9, pl, dst, st));
+        cur = dst;
     }
+    s->rec = const_cast<float4 *>(cur);
     PBF_HIP(hipEventRecord(s->ev[4], st));
     // update + vorticity (src/SPH.cpp:315-333)
     TIMED(KID_UPDATE, launch_update(c, s->rec, s->pos, s->vel, s->params.vorticity ? s->vel_sorted : nullptr, st));
     if (s->params.vorticity) {
         TIMED(KID_VORTICITY_A,
-              launch_vorticity_a(c, s->rec, s->cells_t.nbr9, s->vel_sorted, s->vel, s->vxsph, s->omega, st));
-        TIMED(KID_VORTICITY_B, launch_vorticity_b(c, s->rec, s->cells_t.nbr9, s->vxsph, s->omega, s->vel, st));
+              launch_vorticity_a(c, s->rec, t.nbr9, s->vel_sorted, s->vel, s->vxsph, s->omega, s->vort, pl, st));
+        TIMED(KID_VORTICITY_B, launch_vorticity_b(c, s->rec, t.nbr9, pl, s->vxsph, s->omega, s->vort, s->vel, st));
     }
 #undef TIMED
     PBF_HIP(hipEventRecord(s->ev[5], st));
@@ -826,7 +851,7 @@ int pbf_stage_lambda(uint32_t n, const int32_t grid[3], const pbf_params *p, con
     int32_t *nbr9 = nullptr;
     if ((rc = nbr_scratch(n, nbr12, &nbr9, st))) return rc;
     const SolverConsts c = make_consts(n, grid, *p, key_bits_of(grid));
-    launch_lambda(c, (const float4 *)rec4, nbr9, (float *)lambda_out, st);
+    launch_lambda(c, (const float4 *)rec4, nbr9, (float *)lambda_out, PairList(), st);
     return release_scratch(nbr9, st);
 }
 
@@ -840,7 +865,7 @@ int pbf_stage_updatepos(uint32_t n, const int32_t grid[3], const pbf_params *p, 
     int32_t *nbr9 = nullptr;
     if ((rc = nbr_scratch(n, nbr12, &nbr9, st))) return rc;
     const SolverConsts c = make_consts(n, grid, *p, key_bits_of(grid));
-    launch_updatepos(c, (const float4 *)rec4_in, (const float *)lambda, nbr9, (float4 *)rec4_out, st);
+    launch_updatepos(c, (const float4 *)rec4_in, (const float *)lambda, nbr9, PairList(), (float4 *)rec4_out, st);
     return release_scratch(nbr9, st);
 }
 
@@ -875,9 +900,8 @@ int pbf_stage_vorticity(uint32_t n, const int32_t grid[3], const pbf_params *p, 
     // sorted copy of the velocities, vs[g] = vel[id(rec[g])] (SPH::Run keeps
     // one from the update pass)
     launch_gather_by_id((int32_t)n, rec, vel, vs, st);
-    launch_vorticity_a(c, rec, nbr9, vs, vel, vx, om, st);
-    launch_vorticity_b(c, rec, nbr9, vx, om, vel, st);
-    launch_omega_norm((int32_t)n, om, (float *)vort_out, st);
+    launch_vorticity_a(c, rec, nbr9, vs, vel, vx, om, (float *)vort_out, PairList(), st);
+    launch_vorticity_b(c, rec, nbr9, PairList(), vx, om, (const float *)vort_out, vel, st);
     e = hipStreamSynchronize(st);
     (void)hipFree(nbr9);
     (void)hipFree(tmp);

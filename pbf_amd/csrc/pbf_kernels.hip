@@ -15,6 +15,7 @@
 // the reference's 9 (dy,dz) rows, each start + (count << 24)); the end table
 // is uint64 {step tag, gid} so the newest step's highest gid wins.
 #include "pbf_internal.h"
+#include "pbf_math.h"
 
 namespace pbf {
 namespace {
@@ -25,24 +26,26 @@ constexpr int BLOCK = 256;
 
 __device__ __forceinline__ float sq_len(float x, float y, float z) { return (z * z + y * y) + x * x; }
 
-// Wpoly6 (calclambda.glsl:41-47) of r = sqrt(sq).
+// Wpoly6 (calclambda.glsl:41-47) of r = sqrt(sq). H1: h == 1, where the
+// reference's division by h^9 is a division by 1.
+template <bool H1>
 __device__ __forceinline__ float wpoly6(const SolverConsts &c, float sq, float r) {
     if (r > c.h) return 0.0f;
-    float t = c.hh - sq;
-    float w = (t * t) * (t * POLY6);
-    return c.h_is_one ? w : w / c.h9;
+    const float t = c.hh - sq;
+    const float w = (t * t) * (t * POLY6);
+    return H1 ? w : w / c.h9;
 }
 
-// gradWspiky (calclambda.glsl:57-64) for l = sqrt(sq) with 0 < l <= h.
+// gradWspiky (calclambda.glsl:57-64) for l = sqrt(sq) with 0 < l <= h:
+// (-3 * 15/pi * t * t) * r / (l * h*h*h*h*h*h), t = h - l; the three
+// quotients correctly rounded (pbf_math.h).
+template <bool H1>
 __device__ __forceinline__ void grad_spiky(const SolverConsts &c, float dx, float dy, float dz, float l, float &gx,
                                            float &gy, float &gz) {
-    float t = c.h - l;
-    float s = (SPIKY_GRAD * t) * t;
-    float den = l;
-    if (!c.h_is_one) den = (((((l * c.h) * c.h) * c.h) * c.h) * c.h) * c.h;
-    gx = (s * dx) / den;
-    gy = (s * dy) / den;
-    gz = (s * dz) / den;
+    const float t = c.h - l;
+    const float s = (SPIKY_GRAD * t) * t;
+    const float den = H1 ? l : (((((l * c.h) * c.h) * c.h) * c.h) * c.h) * c.h;
+    div3_rn(s * dx, s * dy, s * dz, den, gx, gy, gz);
 }
 
 __device__ __forceinline__ float clamp_glsl(float x, float lo, float hi) {
@@ -61,6 +64,16 @@ __device__ __forceinline__ uint32_t cell_hash(const SolverConsts &c, float x, fl
 }
 
 __device__ __forceinline__ int32_t id_of(const float4 &r) { return __float_as_int(r.w); }
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
+// Renumbering them so that every XCD runs one contiguous range of particles
+// keeps that range's neighbour rows in its L2 instead of all XCDs sharing a
+// strided view of the whole array. Bijective on [0, gridDim.x).
+__device__ __forceinline__ uint32_t xcd_block() {
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    const uint32_t x = b % 8u, k = b / 8u, q = nb / 8u, r = nb % 8u;
+    return x * q + (x < r ? x : r) + k;
+}
 
 // ----------------------------------------------------------------- predict
 // predictpos.glsl:18-38 fused with the key of counting.glsl:70.
@@ -189,84 +202,214 @@ __global__ __launch_bounds__(BLOCK) void k_neighbourcells(SolverConsts c, const 
 }
 
 // ----------------------------------------------------------------- solver
-// Neighbour record j of the packed ranges; indices past n (a wrapped count)
+// Neighbour records j of the packed ranges; indices past n (a wrapped count)
 // read a zero record, as the reference's robust buffer access does.
 __device__ __forceinline__ float4 rec_at(const float4 *__restrict__ rec, int32_t j, int32_t n) {
     return j < n ? rec[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
-// calclambda.glsl:66-103 (FOR_EACH_NEIGHBOUR, foreachneighbour.glsl:1-10).
-// Candidates with fl(sqrt(sq)) > h add exact zeros in the reference and are
-// skipped here; the cheap sq > h2cut test only ever skips such candidates.
-__global__ __launch_bounds__(BLOCK) void k_lambda(SolverConsts c, const float4 *__restrict__ rec,
-                                                  const int32_t *__restrict__ nbr9, float *__restrict__ lambda) {
-    const int32_t i = blockIdx.x * BLOCK + threadIdx.x;
+// Neighbour iteration in two phases. FOR_EACH_NEIGHBOUR (foreachneighbour.glsl:
+// 1-10) visits the particles of 27 cells, of which about one in six lies
+// within h. Evaluating the kernels under a per-candidate branch runs them for
+// the whole wave on nearly every candidate (some lane is almost always in
+// range), so each thread first scans its candidates with a cheap conservative
+// distance test, four loads in flight at a time (phase 1), and appends the
+// survivors, in the reference's visiting order, to its own column of an LDS
+// list as 16-bit (row, offset) codes; phase 2 runs the exact arithmetic over
+// the list. A column that fills up is evaluated early and reused. Sums keep the
+// reference's order, and candidates with fl(sqrt(sq)) > h add exact zeros
+// there, so skipping them changes no bit of the result.
+constexpr int NB_BLOCK = 256;
+constexpr int LCAP = 48;
+
+struct NbrLds {
+    uint16_t list[(LCAP + 1) * NB_BLOCK];  // row LCAP: sink of the branch-free appends
+    int32_t rowstart[9 * NB_BLOCK];
+};
+
+// True when the candidate may satisfy sq <= h2cut: the fused multiply-add
+// estimate is compared with h2loose = h2cut (1 + 2^-20), which bounds its error.
+__device__ __forceinline__ bool maybe_near(const SolverConsts &c, const float4 &pi, const float4 &pj) {
+    const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+    return fmaf(dx, dx, fmaf(dy, dy, dz * dz)) <= c.h2loose;
+}
+
+// Full candidate scan; body(j, code) for every candidate that may lie within h.
+template <class Body>
+__device__ __forceinline__ void scan_neighbours(const SolverConsts &c, const float4 *__restrict__ rec,
+                                                const int32_t *__restrict__ nbr9, int32_t i, const float4 &pi,
+                                                NbrLds &lds, Body &&body) {
+    const int t = threadIdx.x;
+    int cnt = 0;
+    auto flush = [&]() {
+        for (int k = 0; k < cnt; k++) {
+            const uint32_t e = lds.list[k * NB_BLOCK + t];
+            body(lds.rowstart[(e >> 8) * NB_BLOCK + t] + (int32_t)(e & 255u), e);
+        }
+        cnt = 0;
+    };
+    for (int o = 0; o < 9; o++) {
+        const int32_t d = nbr9[(size_t)o * c.n + i];
+        const int32_t s0 = d & 0xFFFFFF, e = d >> 24;
+        lds.rowstart[o * NB_BLOCK + t] = s0;
+        // Loads stay in bounds: indices past the row or past n repeat the last
+        // valid one, and candidates past n are the zero record.
+        const int32_t last = max(min(s0 + e, c.n) - 1, 0);
+        for (int32_t q = 0; q < e; q += 4) {
+            if (cnt > LCAP - 4) flush();
+            const int32_t j0 = s0 + q;
+            float4 p[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) p[u] = rec[min(j0 + u, last)];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int32_t j = j0 + u;
+                const float4 pj = j < c.n ? p[u] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                const bool f = (q + u < e) & (j != i) & maybe_near(c, pi, pj);
+                lds.list[(f ? cnt : LCAP) * NB_BLOCK + t] = (uint16_t)((o << 8) | (q + u));
+                cnt += f;
+            }
+        }
+    }
+    flush();
+}
+
+// Per-particle list of the neighbours within h at one set of positions, in
+// visiting order, as (row, offset) codes: entries[k * n + i], k < count[i].
+// The lambda pass (and vorticity pass A) writes it while scanning; the
+// position update (and vorticity pass B), which reads the same positions,
+// replays it instead of scanning. count LIST_NONE: more than cap neighbours,
+// the replaying pass scans.
+constexpr uint16_t LIST_NONE = 0xFFFF;
+
+struct PairListOut {
+    uint16_t *entries;
+    uint16_t *count;
+    int32_t cap;
+    int32_t cnt;
+    __device__ __forceinline__ void add(int32_t i, int32_t n, uint32_t code) {
+        if (entries) {
+            if (cnt < cap) entries[(size_t)cnt * n + i] = (uint16_t)code;
+            cnt++;
+        }
+    }
+    __device__ __forceinline__ void finish(int32_t i) {
+        if (entries) count[i] = cnt <= cap ? (uint16_t)cnt : LIST_NONE;
+    }
+};
+
+// body(j, rec[j], extra(j), listed) for i's neighbours within h (listed: the
+// recording pass saw these exact positions, so the range tests are known to
+// pass) or for candidates that may be within h (scanned); extra(j) gathers the
+// kernel's per-neighbour operand so that, on the list path, the loads of four
+// entries are in flight together.
+template <class Extra, class Body>
+__device__ __forceinline__ void replay_neighbours(const SolverConsts &c, const float4 *__restrict__ rec,
+                                                  const int32_t *__restrict__ nbr9, const uint16_t *__restrict__ entries,
+                                                  const uint16_t *__restrict__ count, int32_t i, const float4 &pi,
+                                                  NbrLds &lds, Extra &&extra, Body &&body) {
+    const int t = threadIdx.x;
+    const uint32_t cnt = entries ? count[i] : LIST_NONE;
+    if (cnt == LIST_NONE) {
+        scan_neighbours(c, rec, nbr9, i, pi, lds,
+                        [&](int32_t j, uint32_t) { body(j, rec_at(rec, j, c.n), extra(j), false); });
+        return;
+    }
+    for (int o = 0; o < 9; o++) lds.rowstart[o * NB_BLOCK + t] = nbr9[(size_t)o * c.n + i] & 0xFFFFFF;
+    using X = decltype(extra(0));
+    for (uint32_t k = 0; k < cnt; k += 4) {
+        uint32_t e[4];
+        int32_t j[4];
+        float4 pj[4];
+        X x[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) e[u] = k + u < cnt ? entries[(size_t)(k + u) * c.n + i] : 0u;
+#pragma unroll
+        for (int u = 0; u < 4; u++) j[u] = lds.rowstart[(e[u] >> 8) * NB_BLOCK + t] + (int32_t)(e[u] & 255u);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const bool v = k + u < cnt;
+            pj[u] = v ? rec_at(rec, j[u], c.n) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            x[u] = v ? extra(j[u]) : X();
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (k + u < cnt) body(j[u], pj[u], x[u], true);
+    }
+}
+
+// calclambda.glsl:66-103; records the neighbours within h for the position update.
+template <bool H1>
+__global__ __launch_bounds__(NB_BLOCK) void k_lambda(SolverConsts c, const float4 *__restrict__ rec,
+                                                     const int32_t *__restrict__ nbr9, float *__restrict__ lambda,
+                                                     uint16_t *__restrict__ list_entries,
+                                                     uint16_t *__restrict__ list_count, int32_t list_cap) {
+    __shared__ NbrLds lds;
+    const int32_t i = xcd_block() * NB_BLOCK + threadIdx.x;
     if (i >= c.n) return;
     const float4 pi = rec[i];
     float rho = 0.0f, sk = 0.0f, gix = 0.0f, giy = 0.0f, giz = 0.0f;
-    for (int o = 0; o < 9; o++) {
-        const int32_t d = nbr9[(size_t)o * c.n + i];
-        const int32_t e = d >> 24, s = d & 0xFFFFFF;
-        for (int32_t j = s; j < s + e; j++) {
-            if (j == i) continue;
-            const float4 pj = rec_at(rec, j, c.n);
-            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            const float sq = sq_len(dx, dy, dz);
-            if (sq > c.h2cut) continue;
-            const float r = sqrtf(sq);
-            if (r > c.h) continue;
-            rho += wpoly6(c, sq, r);
-            if (r == 0.0f) continue;
-            float gx, gy, gz;
-            grad_spiky(c, dx, dy, dz, r, gx, gy, gz);
-            gx = gx * c.inv_rho0;
-            gy = gy * c.inv_rho0;
-            gz = gz * c.inv_rho0;
-            sk += sq_len(gx, gy, gz);
-            gix += gx;
-            giy += gy;
-            giz += gz;
-        }
-    }
+    PairListOut out = {list_entries, list_count, list_cap, 0};
+    scan_neighbours(c, rec, nbr9, i, pi, lds, [&](int32_t j, uint32_t code) {
+        const float4 pj = rec_at(rec, j, c.n);
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const float sq = sq_len(dx, dy, dz);
+        if (sq > c.h2cut) return;
+        const float r = sqrt_rn(sq);
+        if (r > c.h) return;
+        out.add(i, c.n, code);
+        rho += wpoly6<H1>(c, sq, r);
+        if (r == 0.0f) return;
+        float gx, gy, gz;
+        grad_spiky<H1>(c, dx, dy, dz, r, gx, gy, gz);
+        gx = gx * c.inv_rho0;
+        gy = gy * c.inv_rho0;
+        gz = gz * c.inv_rho0;
+        sk += sq_len(gx, gy, gz);
+        gix += gx;
+        giy += gy;
+        giz += gz;
+    });
+    out.finish(i);
     sk += sq_len(gix, giy, giz);
     const float ci = rho * c.inv_rho0 - 1.0f;
     lambda[i] = -ci / (sk + c.eps);
 }
 
-// updatepos.glsl:42-105, race-free: neighbours read from rec_in, result to rec_out.
-__global__ __launch_bounds__(BLOCK) void k_updatepos(SolverConsts c, const float4 *__restrict__ rec_in,
-                                                     const float *__restrict__ lambda,
-                                                     const int32_t *__restrict__ nbr9,
-                                                     float4 *__restrict__ rec_out) {
-    const int32_t i = blockIdx.x * BLOCK + threadIdx.x;
+// updatepos.glsl:42-105, race-free: neighbours read from rec_in, result to
+// rec_out; replays the lambda pass's neighbour list (same positions).
+template <bool H1>
+__global__ __launch_bounds__(NB_BLOCK) void k_updatepos(SolverConsts c, const float4 *__restrict__ rec_in,
+                                                        const float *__restrict__ lambda,
+                                                        const int32_t *__restrict__ nbr9,
+                                                        const uint16_t *__restrict__ list_entries,
+                                                        const uint16_t *__restrict__ list_count,
+                                                        float4 *__restrict__ rec_out) {
+    __shared__ NbrLds lds;
+    const int32_t i = xcd_block() * NB_BLOCK + threadIdx.x;
     if (i >= c.n) return;
     const float4 pi = rec_in[i];
     const float li = lambda[i];
     float dpx = 0.0f, dpy = 0.0f, dpz = 0.0f;
-    for (int o = 0; o < 9; o++) {
-        const int32_t d = nbr9[(size_t)o * c.n + i];
-        const int32_t e = d >> 24, s = d & 0xFFFFFF;
-        for (int32_t j = s; j < s + e; j++) {
-            if (j == i) continue;
-            const float4 pj = rec_at(rec_in, j, c.n);
-            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            const float sq = sq_len(dx, dy, dz);
-            if (sq > c.h2cut) continue;
-            const float r = sqrtf(sq);
-            if (r > c.h || r == 0.0f) continue;
-            float sc = c.scale * wpoly6(c, sq, r);
-            sc *= sc;
-            sc *= sc;
-            sc = -c.k * sc;
-            const float lj = j < c.n ? lambda[j] : 0.0f;
-            float gx, gy, gz;
-            grad_spiky(c, dx, dy, dz, r, gx, gy, gz);
-            const float f = li + lj + sc;
-            dpx += f * gx;
-            dpy += f * gy;
-            dpz += f * gz;
-        }
-    }
+    replay_neighbours(c, rec_in, nbr9, list_entries, list_count, i, pi, lds,
+                      [&](int32_t j) { return j < c.n ? lambda[j] : 0.0f; },
+                      [&](int32_t, const float4 &pj, float lj, bool listed) {
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const float sq = sq_len(dx, dy, dz);
+        if (!listed && sq > c.h2cut) return;
+        const float r = sqrt_rn(sq);
+        if ((!listed && r > c.h) || r == 0.0f) return;
+        float sc = c.scale * wpoly6<H1>(c, sq, r);
+        sc *= sc;
+        sc *= sc;
+        sc = -c.k * sc;
+        float gx, gy, gz;
+        grad_spiky<H1>(c, dx, dy, dz, r, gx, gy, gz);
+        const float f = li + lj + sc;
+        dpx += f * gx;
+        dpy += f * gy;
+        dpz += f * gz;
+    });
     float x = pi.x + c.inv_rho0 * dpx, y = pi.y + c.inv_rho0 * dpy, z = pi.z + c.inv_rho0 * dpz;
     x = clamp_glsl(x, c.lo[0], c.hi[0]);
     y = clamp_glsl(y, c.lo[1], c.hi[1]);
@@ -291,76 +434,81 @@ __global__ __launch_bounds__(BLOCK) void k_update(SolverConsts c, const float4 *
 }
 
 // vorticity.glsl:34-60 (first half): XSPH and omega from the velocities before
-// this pass; |omega| kept in omega.w.
-__global__ __launch_bounds__(BLOCK) void k_vorticity_a(SolverConsts c, const float4 *__restrict__ rec,
-                                                       const int32_t *__restrict__ nbr9,
-                                                       const float4 *__restrict__ vel_sorted,
-                                                       const float4 *__restrict__ vel_id,
-                                                       float4 *__restrict__ vxsph, float4 *__restrict__ omega) {
-    const int32_t i = blockIdx.x * BLOCK + threadIdx.x;
+// this pass; |omega| to the vorticity buffer (vorticities[], :60). Records the
+// neighbours within h for the second half.
+template <bool H1>
+__global__ __launch_bounds__(NB_BLOCK) void k_vorticity_a(SolverConsts c, const float4 *__restrict__ rec,
+                                                          const int32_t *__restrict__ nbr9,
+                                                          const float4 *__restrict__ vel_sorted,
+                                                          const float4 *__restrict__ vel_id,
+                                                          float4 *__restrict__ vxsph, float4 *__restrict__ omega,
+                                                          float *__restrict__ vort, uint16_t *__restrict__ list_entries,
+                                                          uint16_t *__restrict__ list_count, int32_t list_cap) {
+    __shared__ NbrLds lds;
+    const int32_t i = xcd_block() * NB_BLOCK + threadIdx.x;
     if (i >= c.n) return;
     const float4 pi = rec[i];
     const float4 vi = vel_sorted[i];
     float sx = 0.0f, sy = 0.0f, sz = 0.0f, wx = 0.0f, wy = 0.0f, wz = 0.0f;
-    for (int o = 0; o < 9; o++) {
-        const int32_t d = nbr9[(size_t)o * c.n + i];
-        const int32_t e = d >> 24, s = d & 0xFFFFFF;
-        for (int32_t j = s; j < s + e; j++) {
-            if (j == i) continue;
-            const float4 pj = rec_at(rec, j, c.n);
-            const float px = pi.x - pj.x, py = pi.y - pj.y, pz = pi.z - pj.z;
-            const float sq = sq_len(px, py, pz);
-            if (sq > c.h2cut) continue;
-            const float r = sqrtf(sq);
-            if (r > c.h) continue;
-            // a zero record past n has id 0: imageLoad (velocitytexture, 0)
-            const float4 vj = j < c.n ? vel_sorted[j] : vel_id[0];
-            const float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
-            const float w = wpoly6(c, sq, r);
-            sx += ux * w;
-            sy += uy * w;
-            sz += uz * w;
-            if (r == 0.0f) continue;
-            float gx, gy, gz;
-            grad_spiky(c, px, py, pz, r, gx, gy, gz);
-            wx = (wx + uy * gz) - uz * gy;
-            wy = (wy + uz * gx) - ux * gz;
-            wz = (wz + ux * gy) - uy * gx;
-        }
-    }
+    PairListOut out = {list_entries, list_count, list_cap, 0};
+    scan_neighbours(c, rec, nbr9, i, pi, lds, [&](int32_t j, uint32_t code) {
+        const float4 pj = rec_at(rec, j, c.n);
+        const float px = pi.x - pj.x, py = pi.y - pj.y, pz = pi.z - pj.z;
+        const float sq = sq_len(px, py, pz);
+        if (sq > c.h2cut) return;
+        const float r = sqrt_rn(sq);
+        if (r > c.h) return;
+        out.add(i, c.n, code);
+        // a zero record past n has id 0: imageLoad (velocitytexture, 0)
+        const float4 vj = j < c.n ? vel_sorted[j] : vel_id[0];
+        const float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
+        const float w = wpoly6<H1>(c, sq, r);
+        sx += ux * w;
+        sy += uy * w;
+        sz += uz * w;
+        if (r == 0.0f) return;
+        float gx, gy, gz;
+        grad_spiky<H1>(c, px, py, pz, r, gx, gy, gz);
+        wx = (wx + uy * gz) - uz * gy;
+        wy = (wy + uz * gx) - ux * gz;
+        wz = (wz + ux * gy) - uy * gx;
+    });
+    out.finish(i);
     vxsph[i] = make_float4(vi.x + c.xsph_c * sx, vi.y + c.xsph_c * sy, vi.z + c.xsph_c * sz, 0.0f);
-    omega[i] = make_float4(wx, wy, wz, sqrtf(sq_len(wx, wy, wz)));
+    omega[i] = make_float4(wx, wy, wz, 0.0f);
+    vort[i] = sqrtf(sq_len(wx, wy, wz));
 }
 
-// vorticity.glsl:65-85 (second half): confinement from this step's |omega|.
-__global__ __launch_bounds__(BLOCK) void k_vorticity_b(SolverConsts c, const float4 *__restrict__ rec,
-                                                       const int32_t *__restrict__ nbr9,
-                                                       const float4 *__restrict__ vxsph,
-                                                       const float4 *__restrict__ omega,
-                                                       float4 *__restrict__ vel_id) {
-    const int32_t i = blockIdx.x * BLOCK + threadIdx.x;
+// vorticity.glsl:65-85 (second half): confinement from this step's |omega|;
+// replays the first half's neighbour list.
+template <bool H1>
+__global__ __launch_bounds__(NB_BLOCK) void k_vorticity_b(SolverConsts c, const float4 *__restrict__ rec,
+                                                          const int32_t *__restrict__ nbr9,
+                                                          const uint16_t *__restrict__ list_entries,
+                                                          const uint16_t *__restrict__ list_count,
+                                                          const float4 *__restrict__ vxsph,
+                                                          const float4 *__restrict__ omega,
+                                                          const float *__restrict__ vort,
+                                                          float4 *__restrict__ vel_id) {
+    __shared__ NbrLds lds;
+    const int32_t i = xcd_block() * NB_BLOCK + threadIdx.x;
     if (i >= c.n) return;
     const float4 pi = rec[i];
     float nx = 0.0f, ny = 0.0f, nz = 0.0f;
-    for (int o = 0; o < 9; o++) {
-        const int32_t d = nbr9[(size_t)o * c.n + i];
-        const int32_t e = d >> 24, s = d & 0xFFFFFF;
-        for (int32_t j = s; j < s + e; j++) {
-            if (j == i) continue;
-            const float4 pj = rec_at(rec, j, c.n);
-            const float px = pi.x - pj.x, py = pi.y - pj.y, pz = pi.z - pj.z;
-            const float sq = sq_len(px, py, pz);
-            if (sq > c.h2cut) continue;
-            const float r = sqrtf(sq);
-            if (r > c.h || r == 0.0f) continue;
-            float gx, gy, gz;
-            grad_spiky(c, px, py, pz, r, gx, gy, gz);
-            const float om = j < c.n ? omega[j].w : 0.0f;
-            nx += om * gx;
-            ny += om * gy;
-            nz += om * gz;
-        }
-    }
+    replay_neighbours(c, rec, nbr9, list_entries, list_count, i, pi, lds,
+                      [&](int32_t j) { return j < c.n ? vort[j] : 0.0f; },
+                      [&](int32_t, const float4 &pj, float om, bool listed) {
+        const float px = pi.x - pj.x, py = pi.y - pj.y, pz = pi.z - pj.z;
+        const float sq = sq_len(px, py, pz);
+        if (!listed && sq > c.h2cut) return;
+        const float r = sqrt_rn(sq);
+        if ((!listed && r > c.h) || r == 0.0f) return;
+        float gx, gy, gz;
+        grad_spiky<H1>(c, px, py, pz, r, gx, gy, gz);
+        nx += om * gx;
+        ny += om * gy;
+        nz += om * gz;
+    });
     const float l = sqrtf(sq_len(nx, ny, nz));
     if (l > 0.0f) {
         nx = nx / l;
@@ -369,10 +517,10 @@ __global__ __launch_bounds__(BLOCK) void k_vorticity_b(SolverConsts c, const flo
     }
     const float4 w = omega[i];
     const float4 v = vxsph[i];
-    const float s = c.dt * c.vort_eps;
-    const float ox = v.x + s * (ny * w.z - nz * w.y);
-    const float oy = v.y + s * (nz * w.x - nx * w.z);
-    const float oz = v.z + s * (nx * w.y - ny * w.x);
+    const float sc = c.dt * c.vort_eps;
+    const float ox = v.x + sc * (ny * w.z - nz * w.y);
+    const float oy = v.y + sc * (nz * w.x - nx * w.z);
+    const float oz = v.z + sc * (nx * w.y - ny * w.x);
     vel_id[id_of(pi)] = make_float4(ox, oy, oz, 0.0f);
 }
 
@@ -412,14 +560,8 @@ __global__ __launch_bounds__(BLOCK) void k_i32_to_end64(size_t cells, const int3
         e64[k] = (unsigned long long)(uint32_t)e[k];
 }
 
-__global__ __launch_bounds__(BLOCK) void k_omega_norm(int32_t n, const float4 *__restrict__ omega,
-                                                      float *__restrict__ vort) {
-    const int32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
-    vort[i] = omega[i].w;
-}
-
 inline unsigned blocks_for(size_t n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
+inline unsigned nb_blocks_for(size_t n) { return (unsigned)((n + NB_BLOCK - 1) / NB_BLOCK); }
 inline unsigned grid_stride_blocks(size_t n) {
     size_t b = (n + BLOCK - 1) / BLOCK;
     return (unsigned)(b > 65536 ? 65536 : (b == 0 ? 1 : b));
@@ -452,24 +594,48 @@ void launch_neighbourcells(const SolverConsts &c, const float4 *sorted, const in
                            const unsigned long long *end64, int32_t *nbr9, hipStream_t st) {
     if (c.n > 0) k_neighbourcells<<<blocks_for(c.n), BLOCK, 0, st>>>(c, sorted, start, end64, nbr9);
 }
-void launch_lambda(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, float *lambda, hipStream_t st) {
-    if (c.n > 0) k_lambda<<<blocks_for(c.n), BLOCK, 0, st>>>(c, rec, nbr9, lambda);
+void launch_lambda(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, float *lambda, const PairList &pl,
+                   hipStream_t st) {
+    if (c.n <= 0) return;
+    if (c.h_is_one)
+        k_lambda<true><<<nb_blocks_for(c.n), NB_BLOCK, 0, st>>>(c, rec, nbr9, lambda, pl.entries, pl.count, pl.cap);
+    else
+        k_lambda<false><<<nb_blocks_for(c.n), NB_BLOCK, 0, st>>>(c, rec, nbr9, lambda, pl.entries, pl.count, pl.cap);
 }
 void launch_updatepos(const SolverConsts &c, const float4 *rec_in, const float *lambda, const int32_t *nbr9,
-                      float4 *rec_out, hipStream_t st) {
-    if (c.n > 0) k_updatepos<<<blocks_for(c.n), BLOCK, 0, st>>>(c, rec_in, lambda, nbr9, rec_out);
+                      const PairList &pl, float4 *rec_out, hipStream_t st) {
+    if (c.n <= 0) return;
+    if (c.h_is_one)
+        k_updatepos<true><<<nb_blocks_for(c.n), NB_BLOCK, 0, st>>>(c, rec_in, lambda, nbr9, pl.entries, pl.count,
+                                                                   rec_out);
+    else
+        k_updatepos<false><<<nb_blocks_for(c.n), NB_BLOCK, 0, st>>>(c, rec_in, lambda, nbr9, pl.entries, pl.count,
+                                                                    rec_out);
 }
 void launch_update(const SolverConsts &c, const float4 *rec, float4 *pos, float4 *vel, float4 *vel_sorted,
                    hipStream_t st) {
     if (c.n > 0) k_update<<<blocks_for(c.n), BLOCK, 0, st>>>(c, rec, pos, vel, vel_sorted);
 }
 void launch_vorticity_a(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, const float4 *vel_sorted,
-                        const float4 *vel_id, float4 *vxsph, float4 *omega, hipStream_t st) {
-    if (c.n > 0) k_vorticity_a<<<blocks_for(c.n), BLOCK, 0, st>>>(c, rec, nbr9, vel_sorted, vel_id, vxsph, omega);
+                        const float4 *vel_id, float4 *vxsph, float4 *omega, float *vort, const PairList &pl,
+                        hipStream_t st) {
+    if (c.n <= 0) return;
+    if (c.h_is_one)
+        k_vorticity_a<true><<<nb_blocks_for(c.n), NB_BLOCK, 0, st>>>(c, rec, nbr9, vel_sorted, vel_id, vxsph, omega,
+                                                                     vort, pl.entries, pl.count, pl.cap);
+    else
+        k_vorticity_a<false><<<nb_blocks_for(c.n), NB_BLOCK, 0, st>>>(c, rec, nbr9, vel_sorted, vel_id, vxsph, omega,
+                                                                      vort, pl.entries, pl.count, pl.cap);
 }
-void launch_vorticity_b(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, const float4 *vxsph,
-                        const float4 *omega, float4 *vel_id, hipStream_t st) {
-    if (c.n > 0) k_vorticity_b<<<blocks_for(c.n), BLOCK, 0, st>>>(c, rec, nbr9, vxsph, omega, vel_id);
+void launch_vorticity_b(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, const PairList &pl,
+                        const float4 *vxsph, const float4 *omega, const float *vort, float4 *vel_id, hipStream_t st) {
+    if (c.n <= 0) return;
+    if (c.h_is_one)
+        k_vorticity_b<true><<<nb_blocks_for(c.n), NB_BLOCK, 0, st>>>(c, rec, nbr9, pl.entries, pl.count, vxsph,
+                                                                     omega, vort, vel_id);
+    else
+        k_vorticity_b<false><<<nb_blocks_for(c.n), NB_BLOCK, 0, st>>>(c, rec, nbr9, pl.entries, pl.count, vxsph,
+                                                                      omega, vort, vel_id);
 }
 void launch_nbr9_to_12(int32_t n, const int32_t *nbr9, int32_t *nbr12, hipStream_t st) {
     if (n > 0) k_nbr9_to_12<<<blocks_for(n), BLOCK, 0, st>>>(n, nbr9, nbr12);
@@ -482,9 +648,6 @@ void launch_end64_to_i32(size_t cells, const unsigned long long *end64, int32_t 
 }
 void launch_i32_to_end64(size_t cells, const int32_t *end, unsigned long long *end64, hipStream_t st) {
     if (cells > 0) k_i32_to_end64<<<grid_stride_blocks(cells), BLOCK, 0, st>>>(cells, end, end64);
-}
-void launch_omega_norm(int32_t n, const float4 *omega, float *vort, hipStream_t st) {
-    if (n > 0) k_omega_norm<<<blocks_for(n), BLOCK, 0, st>>>(n, omega, vort);
 }
 
 }  // namespace pbf

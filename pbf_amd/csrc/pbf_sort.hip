@@ -8,7 +8,8 @@
 //   upsweep   per-tile digit counts (wave64 ballot match, one LDS add per
 //             distinct digit per wave), digit-major like the reference's
 //             blocksum layout (counting.glsl:96-101);
-//   scan      one exclusive scan of the 256 x tiles counts;
+//   scan      per digit, an exclusive scan of its tile counts (one workgroup
+//             per digit); the downsweep adds the scanned digit totals;
 //   downsweep stable in-tile ranks (ballot match + per-(item, wave) digit
 //             offsets scanned in LDS) and the scatter.
 // A tile is 256 threads x 8 items = 2048 keys, element order (item, wave, lane).
@@ -21,7 +22,6 @@ constexpr int RS_THREADS = 256;
 constexpr int RS_WAVES = RS_THREADS / 64;
 constexpr int RS_ITEMS = 8;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
-constexpr int SCAN_THREADS = 1024;
 
 __device__ __forceinline__ unsigned lane_id() { return threadIdx.x & 63u; }
 
@@ -56,16 +56,10 @@ __global__ __launch_bounds__(RS_THREADS) void k_upsweep(const uint32_t *__restri
     counts[(size_t)threadIdx.x * tiles + blockIdx.x] = hist[threadIdx.x];
 }
 
-// Exclusive scan of counts[0, total) in one workgroup (contiguous chunk per thread).
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan(uint32_t *__restrict__ counts, int32_t total) {
-    __shared__ uint32_t wsum[SCAN_THREADS / 64];
-    const int32_t chunk = (total + SCAN_THREADS - 1) / SCAN_THREADS;
-    const int32_t lo = threadIdx.x * chunk;
-    const int32_t hi = lo + chunk < total ? lo + chunk : total;
-    uint32_t sum = 0;
-    for (int32_t k = lo; k < hi; k++) sum += counts[k];
-    // inclusive scan of the thread sums: within the wave, then across waves
-    uint32_t inc = sum;
+// Inclusive scan of one value per thread over a 256-thread block; returns the
+// inclusive prefix, and the block total in *total (LDS scratch of 4 words).
+__device__ __forceinline__ uint32_t block_scan256(uint32_t v, uint32_t *wsum, uint32_t *total) {
+    uint32_t inc = v;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
         const uint32_t t = __shfl_up(inc, off, 64);
@@ -74,31 +68,51 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(uint32_t *__restrict__ co
     const int w = threadIdx.x >> 6;
     if (lane_id() == 63) wsum[w] = inc;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t run = 0;
-        for (int k = 0; k < SCAN_THREADS / 64; k++) {
-            const uint32_t t = wsum[k];
-            wsum[k] = run;
-            run += t;
-        }
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (int k = 0; k < RS_WAVES; k++) {
+        const uint32_t t = wsum[k];
+        before += k < w ? t : 0u;
+        all += t;
     }
     __syncthreads();
-    uint32_t run = wsum[w] + inc - sum;
-    for (int32_t k = lo; k < hi; k++) {
-        const uint32_t t = counts[k];
-        counts[k] = run;
-        run += t;
+    *total = all;
+    return before + inc;
+}
+
+// Per digit d (one workgroup each): exclusive scan of counts[d][0, tiles) in
+// place, row total to totals[d]. The digit offsets (scan of totals) are added
+// by the downsweep.
+__global__ __launch_bounds__(RS_THREADS) void k_scan_rows(uint32_t *__restrict__ counts, int32_t tiles,
+                                                          uint32_t *__restrict__ totals) {
+    __shared__ uint32_t wsum[RS_WAVES];
+    uint32_t *row = counts + (size_t)blockIdx.x * tiles;
+    uint32_t run = 0;
+    for (int32_t base = 0; base < tiles; base += RS_THREADS) {
+        const int32_t k = base + threadIdx.x;
+        const uint32_t v = k < tiles ? row[k] : 0u;
+        uint32_t tot;
+        const uint32_t inc = block_scan256(v, wsum, &tot);
+        if (k < tiles) row[k] = run + inc - v;
+        run += tot;
     }
+    if (threadIdx.x == 0) totals[blockIdx.x] = run;
 }
 
 __global__ __launch_bounds__(RS_THREADS) void k_downsweep(const uint32_t *__restrict__ keys_in,
                                                           const uint32_t *__restrict__ vals_in, int32_t n, int shift,
                                                           int32_t tiles, const uint32_t *__restrict__ offsets,
+                                                          const uint32_t *__restrict__ totals,
                                                           uint32_t *__restrict__ keys_out,
                                                           uint32_t *__restrict__ vals_out) {
     __shared__ uint32_t cnt[RS_ITEMS * RS_WAVES][256];
+    __shared__ uint32_t wsum[RS_WAVES];
     for (int s = 0; s < RS_ITEMS * RS_WAVES; s++) cnt[s][threadIdx.x] = 0;
-    const uint32_t tile_base = offsets[(size_t)threadIdx.x * tiles + blockIdx.x];
+    // global start of digit d (thread d): exclusive scan of the digit totals
+    const uint32_t dt = totals[threadIdx.x];
+    uint32_t all;
+    const uint32_t digit_base = block_scan256(dt, wsum, &all) - dt;
+    const uint32_t tile_base = digit_base + offsets[(size_t)threadIdx.x * tiles + blockIdx.x];
     __syncthreads();
     const int32_t base = blockIdx.x * RS_TILE;
     const int w = threadIdx.x >> 6;
@@ -146,6 +160,7 @@ bool radix_alloc(RadixScratch &s, int32_t n) {
     if (hipMalloc(&s.vals, nn * 4) != hipSuccess) return false;
     if (hipMalloc(&s.vals_alt, nn * 4) != hipSuccess) return false;
     if (hipMalloc(&s.counts, (size_t)256 * (tiles > 0 ? tiles : 1) * 4) != hipSuccess) return false;
+    if (hipMalloc(&s.totals, 256 * 4) != hipSuccess) return false;
     s.capacity = n;
     s.tiles = tiles;
     return true;
@@ -156,6 +171,7 @@ void radix_free(RadixScratch &s) {
     if (s.vals) (void)hipFree(s.vals);
     if (s.vals_alt) (void)hipFree(s.vals_alt);
     if (s.counts) (void)hipFree(s.counts);
+    if (s.totals) (void)hipFree(s.totals);
     s = RadixScratch();
 }
 
@@ -170,9 +186,9 @@ void radix_sort_perm(RadixScratch &s, uint32_t *keys, int32_t n, int key_bits, u
         const bool last = p == passes - 1;
         uint32_t *vout = last ? perm : vbuf[p & 1];
         hipLaunchKernelGGL(k_upsweep, dim3(tiles), dim3(RS_THREADS), 0, st, kin, n, 8 * p, tiles, s.counts);
-        hipLaunchKernelGGL(k_scan, dim3(1), dim3(SCAN_THREADS), 0, st, s.counts, 256 * tiles);
+        hipLaunchKernelGGL(k_scan_rows, dim3(256), dim3(RS_THREADS), 0, st, s.counts, tiles, s.totals);
         hipLaunchKernelGGL(k_downsweep, dim3(tiles), dim3(RS_THREADS), 0, st, kin, vin, n, 8 * p, tiles,
-                           s.counts, last ? nullptr : kout, vout);
+                           s.counts, s.totals, last ? nullptr : kout, vout);
         vin = vout;
         uint32_t *t = kin;
         kin = kout;

@@ -170,3 +170,18 @@ def test_error_paths(gpu_lib):
         P.RadixSort(511, 2, (64, 64, 64))
     with pytest.raises(P._lib.PbfError):
         P.SPH(512, (0, 64, 64))
+
+
+def test_violent_rollout_exercises_list_fallback(gpu_lib):
+    """Large random velocities move particles by more than the superset list's
+    skin within a step, so the displacement guard sends them to the full
+    candidate scan; the rollout must still equal the oracle bit for bit."""
+    sc = S.dam_break((16, 16, 16), (64, 32, 64), steps=12, name="violent")
+    n = sc.n
+    sc.vel = np.zeros_like(sc.pos)
+    sc.vel[:, :3] = (30.0 * S.normal(77, 3 * n).reshape(n, 3)).astype(np.float32)
+    say(f"[violent] n={n}")
+    _, pos, vel = run_sph(sc, 12)
+    wpos, wvel, _ = oracle_rollout(sc, 12)
+    assert_bitexact(pos, wpos, "violent positions")
+    assert_bitexact(vel, wvel, "violent velocities")
