@@ -42,6 +42,21 @@ ALG_BYTES = {
     "neighbourcells": 12.0,
 }
 
+# HBM bytes per particle of one launch, measured for the block_1m workload with
+# rocprofv3 FETCH_SIZE and WRITE_SIZE counters over steps 1-2 of the rollout,
+# scaled by the counters of a plain 256 MiB copy (profiles/hbm_traffic_block_1m.md).
+# lambda writes its pair lists (2 B per in-range neighbour) and updatepos reads
+# them back; both also read the 9-row neighbour table (36 B per particle).
+MEASURED_TRAFFIC_BLOCK_1M = {
+    "lambda": 129.3,
+    "updatepos": 150.0,
+    "vorticity_a": 212.7,
+    "vorticity_b": 194.8,
+    "update": 87.9,
+    "neighbourcells": 60.6,
+    "predict": 52.0,
+}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -63,25 +78,61 @@ def step_alg_bytes(scene) -> float:
 def cpu_baseline(scene, steps: int):
     import oracle as O
     cfg = O.cfg_for(scene)
-    pos, vel = scene.pos, scene.vel
+    pos, vel, end, vort = scene.pos, scene.vel, None, None
     t0 = time.perf_counter()
     for _ in range(steps):
-        pos, vel, _, _ = O.step(cfg, pos, vel, None, None, O.ORDER_JACOBI)
+        pos, vel, end, vort = O.step(cfg, pos, vel, end, vort, O.ORDER_JACOBI)
     dt = time.perf_counter() - t0
     return {"value": scene.n * steps / dt, "unit": "particle-steps/s", "cores": O.num_threads(), "kind": "port",
-            "sample": f"{steps} step(s) of the {scene.name} workload ({scene.n} particles, "
+            "sample": f"the first {steps} step(s) of the {scene.name} rollout ({scene.n} particles, "
                       f"{scene.params.iterations} iterations), oracle/pbf_oracle.c FP32 race-free order, "
                       f"{O.num_threads()} OpenMP threads, {dt:.2f} s"}
+
+
+def reset(sph, scene):
+    """Initial state of the workload: positions, velocities and the persistent
+    end table of the cell finder (never cleared by a step) back to a fresh handle's."""
+    import numpy as np
+    sph.upload(scene.pos, scene.vel)
+    gx, gy, gz = scene.grid
+    sph.set_end_table(np.zeros(gx * gy * gz, np.int32))
+
+
+def timed_run(sph, scene, steps, window):
+    """Runs `steps` steps from the initial state; returns (seconds, per-window ms/step)."""
+    from pbf_amd import _lib
+    lib = _lib.load()
+    reset(sph, scene)
+    _lib.check(lib.pbf_synchronize(sph.stream()))
+    import torch
+    torch.cuda.synchronize()
+    windows = []
+    t0 = time.perf_counter()
+    tw = t0
+    for k in range(steps):
+        sph.Run()
+        if window and (k + 1) % window == 0:
+            _lib.check(lib.pbf_synchronize(sph.stream()))
+            now = time.perf_counter()
+            windows.append(1e3 * (now - tw) / window)
+            tw = now
+    _lib.check(lib.pbf_synchronize(sph.stream()))
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, windows
 
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default: the workload's step count)")
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="block_1m", choices=["block_1m", "dam_32k", "stress_4m"])
-    ap.add_argument("--kernel-steps", type=int, default=10, help="steps timed per kernel after the main run")
-    ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps of the CPU baseline sample (0: skip)")
+    ap.add_argument("--window", type=int, default=100, help="steps per reported window of the timed run")
+    ap.add_argument("--no-kernel-timing", action="store_true", help="skip the per-kernel HIP-event run")
+    ap.add_argument("--cpu-steps", type=int, default=20, help="oracle steps of the CPU baseline sample (0: skip)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="after the run, launch a plain FP32 streaming copy (y = 1 * x) of MIB MiB three times: "
+                         "the known-size kernel that scales FETCH_SIZE / WRITE_SIZE counters to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         log("bench.py measures one GPU only: this build has no multi-GPU path (scenes would shard "
@@ -93,45 +144,45 @@ def main():
 
     scene = workload(args.workload)
     n = scene.n
+    steps = args.steps if args.steps is not None else scene.steps
     log(f"[bench] workload {scene.name}: n={n} grid={scene.grid} iterations={scene.params.iterations} "
-        f"dt={scene.params.timestep}")
+        f"dt={scene.params.timestep}, {steps} timed steps from the initial state")
     sph = P.SPH(n, scene.grid)
     sph.set_params(P.scene_params(scene))
-    sph.upload(scene.pos, scene.vel)
     torch.cuda.init()
+    reset(sph, scene)
     for _ in range(args.warmup):
         sph.Run()
-    P._lib.check(P._lib.load().pbf_synchronize(sph.stream()))
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        sph.Run()
-    P._lib.check(P._lib.load().pbf_synchronize(sph.stream()))
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-    value = n * args.steps / elapsed
-    ms_per_step = 1e3 * elapsed / args.steps
-    log(f"[bench] {args.steps} steps in {elapsed:.4f} s: {ms_per_step:.4f} ms/step, {value:.4e} particle-steps/s")
-    phases = sph.timing()
-    log("[bench] last step phases (ms): " + ", ".join(f"{k} {v:.4f}" for k, v in phases.items()))
+    elapsed, windows = timed_run(sph, scene, steps, args.window)
+    value = n * steps / elapsed
+    ms_per_step = 1e3 * elapsed / steps
+    log(f"[bench] {steps} steps in {elapsed:.4f} s: {ms_per_step:.4f} ms/step, {value:.4e} particle-steps/s")
+    if windows:
+        log("[bench] ms/step per window of " + str(args.window) + ": " + " ".join(f"{w:.3f}" for w in windows))
 
-    # per-kernel durations, HIP events on the library's stream
-    sph.set_kernel_timing(True)
-    for _ in range(args.kernel_steps):
-        sph.Run()
-    kt = sph.kernel_times()
-    sph.set_kernel_timing(False)
-    for name, (ms, cnt) in sorted(kt.items(), key=lambda kv: -kv[1][0]):
-        if cnt:
-            log(f"[bench] kernel {name:16s} {ms / cnt * 1e3:10.2f} us/launch x {cnt // args.kernel_steps} per step")
-    dom = max((k for k in kt if k in ALG_BYTES and kt[k][1]), key=lambda k: kt[k][0])
-    dom_ms = kt[dom][0] / kt[dom][1]
-    alg = ALG_BYTES[dom] * n
-    achieved = alg / (dom_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dom,
-                "launch_us": round(dom_ms * 1e3, 2), "alg_bytes_per_launch": int(alg)}
+    roofline = None
+    if not args.no_kernel_timing:
+        # the same run again with every kernel launch bracketed by HIP events
+        sph.set_kernel_timing(True)
+        elapsed_k, _ = timed_run(sph, scene, steps, 0)
+        kt = sph.kernel_times()
+        sph.set_kernel_timing(False)
+        log(f"[bench] with per-kernel events: {1e3 * elapsed_k / steps:.4f} ms/step")
+        for name, (ms, cnt) in sorted(kt.items(), key=lambda kv: -kv[1][0]):
+            if cnt:
+                log(f"[bench] kernel {name:16s} {ms / cnt * 1e3:10.2f} us/launch x {cnt / steps:.0f} per step, "
+                    f"{ms / steps:.4f} ms/step")
+        dom = max((k for k in kt if k in ALG_BYTES and kt[k][1]), key=lambda k: kt[k][0])
+        dom_ms = kt[dom][0] / kt[dom][1]
+        alg = ALG_BYTES[dom] * n
+        achieved = alg / (dom_ms * 1e-3) / 1e9
+        traffic = None
+        if scene.name == "block_1m" and dom in MEASURED_TRAFFIC_BLOCK_1M:
+            traffic = int(MEASURED_TRAFFIC_BLOCK_1M[dom] * n)
+        roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": dom,
+                    "launch_us": round(dom_ms * 1e3, 2), "alg_bytes_per_launch": int(alg),
+                    "kernel_share_of_step": round(kt[dom][0] / (1e3 * elapsed_k), 4)}
     b_alg = step_alg_bytes(scene)
     step_frac = b_alg * value / (HBM_PEAK_GBS * 1e9)
 
@@ -146,7 +197,7 @@ def main():
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
         "ms_per_step": ms_per_step,
         "higher_is_better": True,
@@ -156,16 +207,24 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{scene.name}: {n} particles, 128x64x128 lattice block at 0.5h, grid "
                                f"{scene.grid[0]}x{scene.grid[1]}x{scene.grid[2]}, {scene.params.iterations} "
-                               f"iterations, dt {scene.params.timestep}, XSPH + vorticity on"
+                               f"iterations, dt {scene.params.timestep}, XSPH + vorticity on, {steps}-step "
+                               f"rollout from the initial state"
                    if scene.name == "block_1m" else scene.name,
                    "particles": n, "grid": list(scene.grid), "iterations": scene.params.iterations,
-                   "parallelism": "single GPU"},
+                   "parallelism": "single GPU", "ms_per_step_windows": [round(w, 3) for w in windows]},
         "step_roofline": {"alg_bytes_per_particle_step": b_alg, "frac": round(step_frac, 5)},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
     sph.close()
+    if args.copy_calibration > 0:
+        src = torch.ones(args.copy_calibration << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        log(f"[bench] copy calibration: 3 x {args.copy_calibration} MiB read + {args.copy_calibration} MiB written")
 
 
 if __name__ == "__main__":
