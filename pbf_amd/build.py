@@ -1,4 +1,5 @@
-"""Builds libpbf_amd.so in-tree with hipcc for gfx950 (no GPU needed).
+"""Builds libpbf_amd.so in-tree with hipcc for gfx950 (no GPU needed), and the
+C++ program of tests/cpp that drives the classes of include/pbf.hpp over it.
 
     python -m pbf_amd.build
 """
@@ -11,6 +12,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libpbf_amd.so")
+CPP_DEMO_SRC = os.path.join(ROOT, "tests", "cpp", "sph_cpp.cpp")
+CPP_DEMO = os.path.join(ROOT, "tests", "cpp", "sph_cpp")
 SOURCES = ["pbf_kernels.hip", "pbf_sort.hip", "pbf_selftest.hip", "pbf_batched.hip", "pbf_adjoint.hip",
            "pbf_capi.hip"]
 
@@ -50,7 +53,20 @@ def build(verbose: bool = False) -> str:
     if out.strip() and verbose:
         print(out)
     os.replace(tmp, LIB)
+    if os.path.exists(CPP_DEMO_SRC):
+        build_cpp_demo()
     return LIB
+
+
+def build_cpp_demo() -> str:
+    """g++ build of the C++ class demo, linked to the in-tree library by rpath."""
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), "-o", CPP_DEMO + ".tmp",
+           CPP_DEMO_SRC, "-L", HERE, "-l:libpbf_amd.so", "-Wl,-rpath,$ORIGIN/../../pbf_amd"]
+    res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"g++ failed ({res.returncode}):\n{res.stdout}")
+    os.replace(CPP_DEMO + ".tmp", CPP_DEMO)
+    return CPP_DEMO
 
 
 if __name__ == "__main__":

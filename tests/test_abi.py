@@ -40,3 +40,21 @@ def test_host_only_entry_points():
     assert (p.one_over_rho_0, p.epsilon, p.gravity, p.iterations, p.h) == (1.0, 5.0, 10.0, 5, 2.0)
     assert list(p.wall_lo) == [16.0, 0.0, 16.0] and list(p.wall_hi) == [112.0, 64.0, 112.0]
     assert list(p.gravity_dir) == [0.0, -1.0, 0.0]
+
+
+def test_cpp_classes_compile_and_link(tmp_path):
+    """include/pbf.hpp compiles with g++ alone (no HIP headers) and the demo
+    program built over the library starts without a GPU."""
+    import subprocess
+    from pbf_amd.build import CPP_DEMO, build_cpp_demo
+    src = tmp_path / "use.cpp"
+    src.write_text('#include "pbf.hpp"\nint main() { return pbf::SPH::Wpoly6(0.5f, 1.0f) > 0.0f ? 0 : 1; }\n')
+    exe = str(tmp_path / "use")
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src), "-o", exe,
+                    "-L", os.path.dirname(_lib.LIB_PATH), "-l:libpbf_amd.so",
+                    "-Wl,-rpath," + os.path.dirname(_lib.LIB_PATH)], check=True)
+    assert subprocess.run([exe]).returncode == 0
+    if not os.path.exists(CPP_DEMO):
+        build_cpp_demo()
+    res = subprocess.run([CPP_DEMO], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert res.returncode == 2 and "usage" in res.stdout
