@@ -89,6 +89,117 @@ def cpu_baseline(scene, steps: int):
                       f"{O.num_threads()} OpenMP threads, {dt:.2f} s"}
 
 
+def bench_batched(steps: int, tune_steps: int):
+    """The 256-scene batched mix (pbf_amd.scene.batched_scenes) from its initial
+    state: one autotuned, graph-captured batch against the plain solver looped
+    over the scenes (one handle and stream per scene, each step issued scene by
+    scene), same GPU, same steps."""
+    import numpy as np
+    import torch
+    from pbf_amd import _lib
+    from pbf_amd import batch as Bt
+    from pbf_amd import scene as S
+    from pbf_amd import sph as P
+    lib = _lib.load()
+    scenes = S.batched_scenes()
+    total = sum(s.n for s in scenes)
+    cells = sum(int(np.prod(s.grid)) for s in scenes)
+    b = Bt.Batch(scenes)
+    zero_end = np.zeros(cells, np.int32)
+
+    def reset_batch():
+        b.upload()
+        b.set_end_tables(zero_end)
+
+    reset_batch()
+    b.step(2)
+    best, trials = b.autotune(tune_steps)
+    log(f"[bench] batched autotune: best {best} of {len(trials)} shapes")
+    b.set_graph(True)
+    reset_batch()
+    b.step(1)  # capture
+    reset_batch()
+    _lib.check(lib.pbf_synchronize(b.stream()))
+    t0 = time.perf_counter()
+    b.step(steps)
+    _lib.check(lib.pbf_synchronize(b.stream()))
+    tb = time.perf_counter() - t0
+    b.close()
+    handles = []
+    for sc in scenes:
+        h = P.SPH(sc.n, sc.grid)
+        h.set_params(P.scene_params(sc))
+        handles.append(h)
+
+    def reset_loop():
+        for h, sc in zip(handles, scenes):
+            h.upload(sc.pos, sc.vel)
+            h.set_end_table(np.zeros(int(np.prod(sc.grid)), np.int32))
+
+    reset_loop()
+    for h in handles:
+        h.Run()
+    reset_loop()
+    torch.cuda.synchronize()
+    for h in handles:
+        _lib.check(lib.pbf_synchronize(h.stream()))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for h in handles:
+            h.Run()
+    for h in handles:
+        _lib.check(lib.pbf_synchronize(h.stream()))
+    tl = time.perf_counter() - t0
+    for h in handles:
+        h.close()
+    vb, vl = total * steps / tb, total * steps / tl
+    p8 = (max(S.key_bits(s.grid) for s in scenes) + 8 + 7) // 8
+    b_alg = 296.0 + 20.0 * p8 + 56.0 * scenes[0].params.iterations + 4.0 * cells / total
+    log(f"[bench] batched: {1e3 * tb / steps:.3f} ms/step, {vb:.4e} particle-steps/s; looped plain: "
+        f"{1e3 * tl / steps:.3f} ms/step, {vl:.4e}; ratio {vb / vl:.2f}")
+    return {"workload": f"{len(scenes)} scenes, n in [{min(s.n for s in scenes)}, {max(s.n for s in scenes)}], "
+                        f"total {total}, 3 iterations, dt 0.016, {steps} steps from the initial state",
+            "value": vb, "unit": "particle-steps/s", "ms_per_step": 1e3 * tb / steps,
+            "looped_plain_value": vl, "looped_plain_ms_per_step": 1e3 * tl / steps, "speedup_vs_looped": vb / vl,
+            "tuning": {k: best[k] for k in ("neighbour_block", "stream_block", "stream_ppt")},
+            "step_roofline": {"alg_bytes_per_particle_step": b_alg, "frac": vb * b_alg / (HBM_PEAK_GBS * 1e9)}}
+
+
+def bench_adjoint(reps: int):
+    """The differentiable workload: 16,384 particles (32x16x32 lattice, grid
+    64^3), v0 ~ N(0, 0.3), 32 steps x 3 iterations, checkpoints every 8 steps,
+    relaxation epsilon 5. Times the FP32 forward rollout and the reverse pass
+    (segment recomputation included) and reports per-step times."""
+    import numpy as np
+    from pbf_amd import adjoint as A
+    from pbf_amd import scene as S
+    sc = S.cfg_adjoint_16k()
+    sc.params.epsilon = 5.0
+    tgt = S.cfg_adjoint_16k(seed=40)
+    tgt.params.epsilon = 5.0
+    r = A.Rollout.for_scene(sc, steps=32, checkpoint_every=8)
+    t = r.forward(sc.pos, tgt.vel)
+    x = r.forward(sc.pos, sc.vel)
+    g = np.zeros_like(x)
+    g[:, :3] = 2.0 * (x[:, :3] - t[:, :3]) / (3 * sc.n)
+    r.backward(g)
+    tf = tb = 0.0
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        r.forward(sc.pos, sc.vel)
+        t1 = time.perf_counter()
+        r.backward(g)
+        t2 = time.perf_counter()
+        tf += t1 - t0
+        tb += t2 - t1
+    r.close()
+    fwd = 1e3 * tf / (reps * 32)
+    bwd = 1e3 * tb / (reps * 32)
+    log(f"[bench] adjoint workload: forward {fwd:.3f} ms/step, reverse {bwd:.3f} ms/step, ratio {bwd / fwd:.2f}")
+    return {"workload": "16384 particles, 32 steps x 3 iterations, checkpoint every 8, FP32, epsilon 5",
+            "forward_ms_per_step": fwd, "adjoint_ms_per_step": bwd, "adjoint_over_forward": bwd / fwd}
+
+
 def reset(sph, scene):
     """Initial state of the workload: positions, velocities and the persistent
     end table of the cell finder (never cleared by a step) back to a fresh handle's."""
@@ -130,6 +241,9 @@ def main():
     ap.add_argument("--window", type=int, default=100, help="steps per reported window of the timed run")
     ap.add_argument("--no-kernel-timing", action="store_true", help="skip the per-kernel HIP-event run")
     ap.add_argument("--cpu-steps", type=int, default=20, help="oracle steps of the CPU baseline sample (0: skip)")
+    ap.add_argument("--batched-steps", type=int, default=100, help="steps of the batched mix (0: skip)")
+    ap.add_argument("--tune-steps", type=int, default=2, help="steps per autotuner trial of the batched mix")
+    ap.add_argument("--adjoint-reps", type=int, default=3, help="timed forward+reverse rollouts of the adjoint workload (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the run, launch a plain FP32 streaming copy (y = 1 * x) of MIB MiB three times: "
                          "the known-size kernel that scales FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -186,6 +300,15 @@ def main():
     b_alg = step_alg_bytes(scene)
     step_frac = b_alg * value / (HBM_PEAK_GBS * 1e9)
 
+    batched = None
+    if args.batched_steps > 0:
+        sph.close()
+        batched = bench_batched(args.batched_steps, args.tune_steps)
+
+    adjoint = None
+    if args.adjoint_reps > 0:
+        adjoint = bench_adjoint(args.adjoint_reps)
+
     cpu = None
     if args.cpu_steps > 0:
         log(f"[bench] CPU baseline: {args.cpu_steps} oracle step(s) ...")
@@ -215,6 +338,8 @@ def main():
         "step_roofline": {"alg_bytes_per_particle_step": b_alg, "frac": round(step_frac, 5)},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batched,
+        "adjoint": adjoint,
     }
     print(json.dumps(line), flush=True)
     sph.close()

@@ -194,6 +194,98 @@ int pbf_ncf_set_end_table(pbf_ncf *f, const int32_t *end, int host);
 int pbf_ncf_set_stream(pbf_ncf *f, void *stream);
 void *pbf_ncf_get_stream(const pbf_ncf *f);
 
+/* ---------------------------------------------------------------- batched scenes
+ * No reference counterpart: many independent scenes of uneven particle counts
+ * advanced together, one launch per stage for all of them. Each scene has its
+ * own grid, walls, gravity direction and XSPH coefficient; the other constants
+ * are shared. Per scene, every result equals a pbf_sph handle created with that
+ * scene's grid and parameters, bit for bit (same arithmetic, same neighbour
+ * order): records carry scene-local ids and the tables scene-local indices.
+ * State arrays hold the scenes one after another, each in id order. */
+typedef struct pbf_scene_desc {
+    uint32_t n;             /* particles of the scene, a positive multiple of 512 as for pbf_sph_create */
+    int32_t grid[3];
+    float wall_lo[3];
+    float wall_hi[3];
+    float gravity_dir[3];
+    float xsph_viscosity_c;
+} pbf_scene_desc;
+
+/* Launch shapes the autotuner picks from: workgroup size of the neighbour
+ * passes (128, 256 or 512), and workgroup size (64 .. 1024) and particles per
+ * thread (1 .. 4) of the streaming passes (predict, cell table, update). */
+typedef struct pbf_batch_tuning {
+    int32_t neighbour_block;
+    int32_t stream_block;
+    int32_t stream_ppt;
+    double step_ms;         /* measured mean step time of this choice (autotune output) */
+} pbf_batch_tuning;
+
+typedef struct pbf_batch pbf_batch;
+
+/* shared: h, the eight uniform-block floats except the XSPH coefficient,
+ * iterations, vorticity and extforce flags; its walls, gravity direction and
+ * XSPH coefficient are ignored (each scene has its own). At most 65536 scenes. */
+int pbf_batch_create(pbf_batch **out, uint32_t nscenes, const pbf_scene_desc *scenes, const pbf_params *shared);
+int pbf_batch_destroy(pbf_batch *b);
+/* Total particle count and, when offsets is not null, the first index of each
+ * scene in the state arrays (nscenes + 1 entries, the last one the total). */
+int pbf_batch_layout(const pbf_batch *b, uint32_t *total, uint32_t *offsets);
+int pbf_batch_upload_state(pbf_batch *b, const float *pos4, const float *vel4, int host);
+int pbf_batch_download_state(pbf_batch *b, float *pos4, float *vel4, int host);
+/* nsteps steps of every scene, asynchronous on the batch's stream. */
+int pbf_batch_step(pbf_batch *b, int32_t nsteps);
+/* enable != 0: the first step after this call captures the step's launches in
+ * a hipGraph, and every step replays it (one launch call per step). */
+int pbf_batch_set_graph(pbf_batch *b, int enable);
+int pbf_batch_set_tuning(pbf_batch *b, const pbf_batch_tuning *t);
+int pbf_batch_get_tuning(const pbf_batch *b, pbf_batch_tuning *out);
+/* Times every launch shape on the current state (steps_per_trial steps each,
+ * state restored afterwards) and keeps the fastest. results, when not null,
+ * receives up to max_results trials; *count their number. */
+int pbf_batch_autotune(pbf_batch *b, int32_t steps_per_trial, pbf_batch_tuning *best, pbf_batch_tuning *results,
+                       int32_t max_results, int32_t *count);
+/* The scenes' persistent end tables, one after another (int32 per cell). */
+int pbf_batch_get_end_tables(pbf_batch *b, int32_t *dst, int host);
+int pbf_batch_set_end_tables(pbf_batch *b, const int32_t *src, int host);
+int pbf_batch_set_stream(pbf_batch *b, void *stream);
+void *pbf_batch_get_stream(const pbf_batch *b);
+
+/* ---------------------------------------------------------------- adjoint
+ * No reference counterpart: reverse mode through whole steps (predict, the
+ * solver iterations, update, XSPH and vorticity confinement) for gradients of
+ * a loss on the final positions with respect to the initial velocities, the
+ * XSPH coefficient and the vorticity epsilon. Each step's sort order and
+ * neighbour table are held fixed: they are computed by the plain step's
+ * kernels from the state and not differentiated. The clamp to the walls has a
+ * zero derivative where it clamps; N = eta/|eta| a zero derivative at eta = 0.
+ * States are checkpointed every checkpoint_every steps and each segment is
+ * recomputed from its checkpoint during the reverse pass. fp64 != 0: every
+ * state, tape and adjoint in double (for finite-difference checks), and the
+ * element type of every state array below is double instead of float. Arrays
+ * hold 4 elements per particle (x, y, z, unused) in id order. */
+typedef struct pbf_adjoint pbf_adjoint;
+int pbf_adjoint_create(pbf_adjoint **out, uint32_t numparticles, int32_t gx, int32_t gy, int32_t gz,
+                       const pbf_params *p, int32_t steps, int32_t checkpoint_every, int32_t fp64);
+int pbf_adjoint_destroy(pbf_adjoint *a);
+/* Values of the two differentiated coefficients for the next forward (defaults
+ * from the parameters given at creation). */
+int pbf_adjoint_set_coefficients(pbf_adjoint *a, double xsph_c, double vorticity_epsilon);
+/* steps steps from (pos4, vel4), keeping the checkpoints; keep_structure != 0
+ * also keeps every step's sort order and neighbour table for replays. */
+int pbf_adjoint_forward(pbf_adjoint *a, const void *pos4, const void *vel4, int host, int keep_structure);
+int pbf_adjoint_positions(pbf_adjoint *a, void *pos4, int host);
+int pbf_adjoint_velocities(pbf_adjoint *a, void *vel4, int host);
+/* From dL/d(final positions) to dL/d(initial velocities) and the two
+ * coefficient derivatives (synchronizes the handle's stream). */
+int pbf_adjoint_backward(pbf_adjoint *a, const void *grad_pos4, int host, void *grad_vel4, double *grad_xsph_c,
+                         double *grad_vorticity_epsilon);
+/* steps steps from (pos4, vel4) with the structure kept by the last forward
+ * (the map whose derivative backward computes); final positions to pos_out4. */
+int pbf_adjoint_replay(pbf_adjoint *a, const void *pos4, const void *vel4, int host, void *pos_out4);
+int pbf_adjoint_set_stream(pbf_adjoint *a, void *stream);
+void *pbf_adjoint_get_stream(const pbf_adjoint *a);
+
 /* ---------------------------------------------------------------- stages
  * Single shader dispatches of SPH::Run on caller memory (device pointers,
  * reference layouts), so each stage can be checked alone. n is a multiple of

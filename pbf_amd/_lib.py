@@ -34,6 +34,17 @@ class PbfTiming(C.Structure):
                 ("solver_ms", C.c_double), ("vorticity_ms", C.c_double)]
 
 
+class PbfSceneDesc(C.Structure):
+    """``pbf_scene_desc`` of include/pbf.h: one scene of the batched form."""
+    _fields_ = [("n", C.c_uint32), ("grid", C.c_int32 * 3), ("wall_lo", C.c_float * 3), ("wall_hi", C.c_float * 3),
+                ("gravity_dir", C.c_float * 3), ("xsph_viscosity_c", C.c_float)]
+
+
+class PbfBatchTuning(C.Structure):
+    _fields_ = [("neighbour_block", C.c_int32), ("stream_block", C.c_int32), ("stream_ppt", C.c_int32),
+                ("step_ms", C.c_double)]
+
+
 vp = C.c_void_p
 i32 = C.c_int32
 u32 = C.c_uint32
@@ -85,6 +96,31 @@ SIGNATURES = {
     "pbf_ncf_set_end_table": ([vp, vp, C.c_int], C.c_int),
     "pbf_ncf_set_stream": ([vp, vp], C.c_int),
     "pbf_ncf_get_stream": ([vp], vp),
+    "pbf_batch_create": ([C.POINTER(vp), u32, C.POINTER(PbfSceneDesc), pp], C.c_int),
+    "pbf_batch_destroy": ([vp], C.c_int),
+    "pbf_batch_layout": ([vp, C.POINTER(u32), C.POINTER(u32)], C.c_int),
+    "pbf_batch_upload_state": ([vp, vp, vp, C.c_int], C.c_int),
+    "pbf_batch_download_state": ([vp, vp, vp, C.c_int], C.c_int),
+    "pbf_batch_step": ([vp, i32], C.c_int),
+    "pbf_batch_set_graph": ([vp, C.c_int], C.c_int),
+    "pbf_batch_set_tuning": ([vp, C.POINTER(PbfBatchTuning)], C.c_int),
+    "pbf_batch_get_tuning": ([vp, C.POINTER(PbfBatchTuning)], C.c_int),
+    "pbf_batch_autotune": ([vp, i32, C.POINTER(PbfBatchTuning), C.POINTER(PbfBatchTuning), i32, C.POINTER(i32)],
+                           C.c_int),
+    "pbf_batch_get_end_tables": ([vp, vp, C.c_int], C.c_int),
+    "pbf_batch_set_end_tables": ([vp, vp, C.c_int], C.c_int),
+    "pbf_batch_set_stream": ([vp, vp], C.c_int),
+    "pbf_batch_get_stream": ([vp], vp),
+    "pbf_adjoint_create": ([C.POINTER(vp), u32, i32, i32, i32, pp, i32, i32, i32], C.c_int),
+    "pbf_adjoint_destroy": ([vp], C.c_int),
+    "pbf_adjoint_set_coefficients": ([vp, C.c_double, C.c_double], C.c_int),
+    "pbf_adjoint_forward": ([vp, vp, vp, C.c_int, C.c_int], C.c_int),
+    "pbf_adjoint_positions": ([vp, vp, C.c_int], C.c_int),
+    "pbf_adjoint_velocities": ([vp, vp, C.c_int], C.c_int),
+    "pbf_adjoint_backward": ([vp, vp, C.c_int, vp, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
+    "pbf_adjoint_replay": ([vp, vp, vp, C.c_int, vp], C.c_int),
+    "pbf_adjoint_set_stream": ([vp, vp], C.c_int),
+    "pbf_adjoint_get_stream": ([vp], vp),
     "pbf_stage_predict": ([u32, grid3, pp, vp, vp, vp, vp], C.c_int),
     "pbf_stage_lambda": ([u32, grid3, pp, vp, vp, vp, vp], C.c_int),
     "pbf_stage_updatepos": ([u32, grid3, pp, vp, vp, vp, vp, vp], C.c_int),

@@ -29,6 +29,8 @@ HIPCC_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall",
     "-Wno-unused-function",
+    # Range tests combine booleans with & on purpose (no short-circuit branches).
+    "-Wno-bitwise-instead-of-logical",
 ]
 
 

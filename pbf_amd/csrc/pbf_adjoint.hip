@@ -1,0 +1,1106 @@
+// Differentiable form of the step (include/pbf.h, pbf_adjoint_*): reverse
+// mode through predict, the I x (lambda, position update) iterations, update,
+// XSPH and vorticity confinement, with each step's sort order and neighbour
+// table held fixed (computed by the plain step's kernels from the state).
+//
+// The step is differentiated as the race-free solver computes it:
+//   x*   = x + dt (v + dt g gdir)                      predictpos.glsl:18-38
+//   p0   = x*[perm]                                    sorted by cell key
+//   lam  = -(rho/rho0 - 1) / (sum |g_j|^2 + |sum g_j|^2 + eps), g_j = gradW/rho0
+//   p'   = clamp(p + 1/rho0 sum (lam_i + lam_j + s_ij) gradW_ij)   (I times)
+//   v    = (p_I - x) / dt ;  x' = p_I                  update.glsl:16-28
+//   vx   = v + c sum (v_j - v_i) W_ij ; w = sum (v_j - v_i) x gradW_ij
+//   eta  = sum |w_j| gradW_ij ; N = eta / |eta| ; v' = vx + dt eps_v (N x w)
+// with W = poly6, gradW the spiky gradient, s_ij = -k (scale W)^4. The
+// derivative of the clamp is 1 inside the walls and 0 where it clamps; N has a
+// zero derivative where eta = 0; neighbours past n (a zero record, reached
+// only through a wrapped count) are constants.
+//
+// Reverse mode visits each particle's neighbours and scatters the neighbour
+// side of every pair term with atomic adds, so it needs no symmetry of the
+// neighbour sets (the reference's are not always symmetric). Element type T is
+// float (the production gradient) or double (the finite-difference check).
+// Forward states are checkpointed every K steps; the backward pass recomputes
+// each segment's steps from its checkpoint into a tape, then runs the segment
+// in reverse.
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "pbf_internal.h"
+
+namespace pbf {
+namespace {
+
+constexpr int AB = 256;  // workgroup size of the adjoint kernels
+
+template <class T>
+struct alignas(sizeof(T) * 4) T4 {
+    T x, y, z, w;
+};
+
+template <class T>
+__device__ __forceinline__ T sqlen(T x, T y, T z) {
+    return (z * z + y * y) + x * x;
+}
+
+// Per-step constants in T.
+template <class T>
+struct AConsts {
+    int32_t n, gx, gy, gz;
+    T h, hh, h6, h9, inv_rho0, eps, gravity, dt, k, scale, xsph_c, vort_eps;
+    T lo[3], hi[3], gdir[3];
+    int32_t extforce;
+    int32_t vorticity;
+};
+
+// 315/(64 pi) and -3 * 15/pi (calclambda.glsl:46, 63): the shaders' FP32
+// constants for float, the same decimal constants in double for the FP64 form.
+template <class T>
+__device__ __forceinline__ T poly6c();
+template <>
+__device__ __forceinline__ float poly6c<float>() { return 1.56668147106f; }
+template <>
+__device__ __forceinline__ double poly6c<double>() { return 1.56668147106; }
+template <class T>
+__device__ __forceinline__ T spikyc();
+template <>
+__device__ __forceinline__ float spikyc<float>() { return -3.0f * 4.774648292756860f; }
+template <>
+__device__ __forceinline__ double spikyc<double>() { return -3.0 * 4.774648292756860; }
+
+// W_poly6 of sq = r^2 (r <= h).
+template <class T>
+__device__ __forceinline__ T wpoly(const AConsts<T> &c, T sq) {
+    const T t = c.hh - sq;
+    return (t * t) * (t * poly6c<T>()) / c.h9;
+}
+// dW/dd = -6 poly6 (hh - sq)^2 / h^9 * d  (scalar factor)
+template <class T>
+__device__ __forceinline__ T dwpoly(const AConsts<T> &c, T sq) {
+    const T t = c.hh - sq;
+    return (T)-6.0 * poly6c<T>() * t * t / c.h9;
+}
+// spiky gradient factor: gradW = sg * d, sg = spiky (h - r)^2 / (r h^6)
+template <class T>
+__device__ __forceinline__ T gspiky(const AConsts<T> &c, T r) {
+    const T t = c.h - r;
+    return ((spikyc<T>() * t) * t) / (r * c.h6);
+}
+// gradW (calclambda.glsl:57-64) rounded as the plain step computes it:
+// ((spiky t) t) d / (r h h h h h h), so the float forward equals the plain step.
+template <class T>
+__device__ __forceinline__ void grad3(const AConsts<T> &c, T r, T ex, T ey, T ez, T &gx, T &gy, T &gz) {
+    const T t = c.h - r;
+    const T s = (spikyc<T>() * t) * t;
+    const T den = (((((r * c.h) * c.h) * c.h) * c.h) * c.h) * c.h;
+    gx = (s * ex) / den;
+    gy = (s * ey) / den;
+    gz = (s * ez) / den;
+}
+// Jacobian of gradW: J a = sp/h^6 [f a + (f'/r) (d.a) d], f = (h-r)^2/r,
+// f'/r = -(h-r)(h+r)/r^3.
+template <class T>
+__device__ __forceinline__ void jspiky(const AConsts<T> &c, T r, T dx, T dy, T dz, T ax, T ay, T az, T &ox, T &oy,
+                                       T &oz) {
+    const T t = c.h - r;
+    const T f = t * t / r;
+    const T fr = -t * (c.h + r) / (r * r * r);
+    const T s = spikyc<T>() / c.h6;
+    const T da = dx * ax + dy * ay + dz * az;
+    ox = s * (f * ax + fr * da * dx);
+    oy = s * (f * ay + fr * da * dy);
+    oz = s * (f * az + fr * da * dz);
+}
+
+template <class T>
+__device__ __forceinline__ T4<T> load_nb(const T4<T> *p, int32_t j, int32_t n) {
+    return j < n ? p[j] : T4<T>{0, 0, 0, 0};
+}
+
+template <class T>
+__device__ __forceinline__ void atomic_add3(T4<T> *p, int32_t j, T x, T y, T z) {
+    atomicAdd(&p[j].x, x);
+    atomicAdd(&p[j].y, y);
+    atomicAdd(&p[j].z, z);
+}
+
+// FOR_EACH_NEIGHBOUR (foreachneighbour.glsl:1-10) over the fixed table:
+// body(j) for every candidate j != i of the 9 rows, in the reference's order.
+template <class F>
+__device__ __forceinline__ void each_candidate(const int32_t *__restrict__ nbr9, int32_t n, int32_t i, F &&body) {
+    for (int o = 0; o < 9; o++) {
+        const int32_t d = nbr9[(size_t)o * n + i];
+        const int32_t s0 = d & 0xFFFFFF, e = d >> 24;
+        for (int32_t j = s0; j < s0 + e; j++)
+            if (j != i) body(j);
+    }
+}
+
+__device__ __forceinline__ void block_add2(double a, double b, double *acc) {
+    __shared__ double s[2][AB];
+    s[0][threadIdx.x] = a;
+    s[1][threadIdx.x] = b;
+    __syncthreads();
+    for (int k = AB / 2; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) {
+            s[0][threadIdx.x] += s[0][threadIdx.x + k];
+            s[1][threadIdx.x] += s[1][threadIdx.x + k];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (s[0][0] != 0.0) atomicAdd(&acc[0], s[0][0]);
+        if (s[1][0] != 0.0) atomicAdd(&acc[1], s[1][0]);
+    }
+}
+
+// ------------------------------------------------------------------ forward
+// predictpos.glsl:18-38 in T; also the float record {x*, id} and its key for
+// the structure (sort and cell table) of this step.
+template <class T>
+__global__ __launch_bounds__(AB) void ka_predict(AConsts<T> c, const T4<T> *__restrict__ x, const T4<T> *__restrict__ v,
+                                                 T4<T> *__restrict__ xs, float4 *__restrict__ xs_f) {
+    const int32_t i = blockIdx.x * AB + threadIdx.x;
+    if (i >= c.n) return;
+    const T4<T> p = x[i], u = v[i];
+    T vx = u.x, vy = u.y, vz = u.z;
+    if (c.extforce && p.z > (T)c.gz / (T)2) {
+        vz = vz + ((T)2 * c.gravity * (T)-1) * c.dt;
+    }
+    vx = vx + (c.gravity * c.gdir[0]) * c.dt;
+    vy = vy + (c.gravity * c.gdir[1]) * c.dt;
+    vz = vz + (c.gravity * c.gdir[2]) * c.dt;
+    const T a = p.x + c.dt * vx, b = p.y + c.dt * vy, d = p.z + c.dt * vz;
+    xs[i] = T4<T>{a, b, d, 0};
+    xs_f[i] = make_float4((float)a, (float)b, (float)d, __int_as_float(i));
+}
+
+template <class T>
+__global__ __launch_bounds__(AB) void ka_gather(int32_t n, const uint32_t *__restrict__ perm,
+                                                const T4<T> *__restrict__ xs, T4<T> *__restrict__ p) {
+    const int32_t g = blockIdx.x * AB + threadIdx.x;
+    if (g >= n) return;
+    p[g] = xs[perm[g]];
+}
+
+// calclambda.glsl:66-103; keeps S + eps and G = sum g_j for the reverse pass.
+template <class T>
+__global__ __launch_bounds__(AB) void ka_lambda(AConsts<T> c, const T4<T> *__restrict__ p,
+                                                const int32_t *__restrict__ nbr9, T *__restrict__ lam,
+                                                T *__restrict__ sden_out, T4<T> *__restrict__ G_out) {
+    const int32_t i = blockIdx.x * AB + threadIdx.x;
+    if (i >= c.n) return;
+    const T4<T> pi = p[i];
+    T rho = 0, sk = 0, gx = 0, gy = 0, gz = 0;
+    each_candidate(nbr9, c.n, i, [&](int32_t j) {
+        const T4<T> pj = load_nb(p, j, c.n);
+        const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const T sq = sqlen(dx, dy, dz);
+        const T r = sqrt(sq);
+        if (r > c.h) return;
+        rho += wpoly(c, sq);
+        if (r == (T)0) return;
+        T ax, ay, az;
+        grad3(c, r, dx, dy, dz, ax, ay, az);
+        ax = ax * c.inv_rho0;
+        ay = ay * c.inv_rho0;
+        az = az * c.inv_rho0;
+        sk += sqlen(ax, ay, az);
+        gx += ax;
+        gy += ay;
+        gz += az;
+    });
+    sk += sqlen(gx, gy, gz);
+    const T ci = rho * c.inv_rho0 - (T)1;
+    lam[i] = -ci / (sk + c.eps);
+    if (sden_out) {
+        sden_out[i] = sk + c.eps;
+        G_out[i] = T4<T>{gx, gy, gz, 0};
+    }
+}
+
+// updatepos.glsl:42-105, race-free; mask bit a set where axis a was not clamped.
+template <class T>
+__global__ __launch_bounds__(AB) void ka_updatepos(AConsts<T> c, const T4<T> *__restrict__ p,
+                                                   const T *__restrict__ lam, const int32_t *__restrict__ nbr9,
+                                                   T4<T> *__restrict__ out, uint8_t *__restrict__ mask) {
+    const int32_t i = blockIdx.x * AB + threadIdx.x;
+    if (i >= c.n) return;
+    const T4<T> pi = p[i];
+    const T li = lam[i];
+    T dx = 0, dy = 0, dz = 0;
+    each_candidate(nbr9, c.n, i, [&](int32_t j) {
+        const T4<T> pj = load_nb(p, j, c.n);
+        const T ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
+        const T sq = sqlen(ex, ey, ez);
+        const T r = sqrt(sq);
+        if (r > c.h || r == (T)0) return;
+        T sc = c.scale * wpoly(c, sq);
+        sc *= sc;
+        sc *= sc;
+        sc = -c.k * sc;
+        const T lj = j < c.n ? lam[j] : (T)0;
+        T gx, gy, gz;
+        grad3(c, r, ex, ey, ez, gx, gy, gz);
+        const T f = li + lj + sc;
+        dx += f * gx;
+        dy += f * gy;
+        dz += f * gz;
+    });
+    const T q[3] = {pi.x + c.inv_rho0 * dx, pi.y + c.inv_rho0 * dy, pi.z + c.inv_rho0 * dz};
+    T o[3];
+    uint8_t m = 0;
+    for (int a = 0; a < 3; a++) {
+        o[a] = q[a] < c.lo[a] ? c.lo[a] : (q[a] > c.hi[a] ? c.hi[a] : q[a]);
+        if (q[a] >= c.lo[a] && q[a] <= c.hi[a]) m |= (uint8_t)(1u << a);
+    }
+    out[i] = T4<T>{o[0], o[1], o[2], pi.w};
+    if (mask) mask[i] = m;
+}
+
+// update.glsl:16-28: v = (p_I - x)/dt by id; x = p_I; vs keeps v in sorted order.
+template <class T>
+__global__ __launch_bounds__(AB) void ka_update(AConsts<T> c, const uint32_t *__restrict__ perm,
+                                                const T4<T> *__restrict__ p, T4<T> *__restrict__ x,
+                                                T4<T> *__restrict__ v, T4<T> *__restrict__ vs) {
+    const int32_t g = blockIdx.x * AB + threadIdx.x;
+    if (g >= c.n) return;
+    const uint32_t id = perm[g];
+    const T4<T> r = p[g], old = x[id];
+    const T4<T> u = T4<T>{(r.x - old.x) / c.dt, (r.y - old.y) / c.dt, (r.z - old.z) / c.dt, 0};
+    x[id] = T4<T>{r.x, r.y, r.z, 0};
+    v[id] = u;
+    vs[g] = u;
+}
+
+// vorticity.glsl:34-60: XSPH velocity and omega; |omega| per particle.
+template <class T>
+__global__ __launch_bounds__(AB) void ka_vort_a(AConsts<T> c, const T4<T> *__restrict__ p,
+                                                const int32_t *__restrict__ nbr9, const T4<T> *__restrict__ vs,
+                                                const T4<T> *__restrict__ v_id0, T4<T> *__restrict__ vx,
+                                                T4<T> *__restrict__ om, T *__restrict__ o) {
+    const int32_t i = blockIdx.x * AB + threadIdx.x;
+    if (i >= c.n) return;
+    const T4<T> pi = p[i], vi = vs[i];
+    T sx = 0, sy = 0, sz = 0, wx = 0, wy = 0, wz = 0;
+    each_candidate(nbr9, c.n, i, [&](int32_t j) {
+        const T4<T> pj = load_nb(p, j, c.n);
+        const T ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
+        const T sq = sqlen(ex, ey, ez);
+        const T r = sqrt(sq);
+        if (r > c.h) return;
+        const T4<T> vj = j < c.n ? vs[j] : v_id0[0];
+        const T ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
+        const T w = wpoly(c, sq);
+        sx += ux * w;
+        sy += uy * w;
+        sz += uz * w;
+        if (r == (T)0) return;
+        T gx, gy, gz;
+        grad3(c, r, ex, ey, ez, gx, gy, gz);
+        wx = (wx + uy * gz) - uz * gy;
+        wy = (wy + uz * gx) - ux * gz;
+        wz = (wz + ux * gy) - uy * gx;
+    });
+    vx[i] = T4<T>{vi.x + c.xsph_c * sx, vi.y + c.xsph_c * sy, vi.z + c.xsph_c * sz, 0};
+    om[i] = T4<T>{wx, wy, wz, 0};
+    o[i] = sqrt(sqlen(wx, wy, wz));
+}
+
+// vorticity.glsl:65-85: eta, N and the confinement; v by id.
+template <class T>
+__global__ __launch_bounds__(AB) void ka_vort_b(AConsts<T> c, const uint32_t *__restrict__ perm,
+                                                const T4<T> *__restrict__ p, const int32_t *__restrict__ nbr9,
+                                                const T4<T> *__restrict__ vx, const T4<T> *__restrict__ om,
+                                                const T *__restrict__ o, T4<T> *__restrict__ v) {
+    const int32_t i = blockIdx.x * AB + threadIdx.x;
+    if (i >= c.n) return;
+    const T4<T> pi = p[i];
+    T nx = 0, ny = 0, nz = 0;
+    each_candidate(nbr9, c.n, i, [&](int32_t j) {
+        const T4<T> pj = load_nb(p, j, c.n);
+        const T ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
+        const T sq = sqlen(ex, ey, ez);
+        const T r = sqrt(sq);
+        if (r > c.h || r == (T)0) return;
+        const T oj = j < c.n ? o[j] : (T)0;
+        T gx, gy, gz;
+        grad3(c, r, ex, ey, ez, gx, gy, gz);
+        nx += oj * gx;
+        ny += oj * gy;
+        nz += oj * gz;
+    });
+    const T l = sqrt(sqlen(nx, ny, nz));
+    if (l > (T)0) {
+        nx = nx / l;
+        ny = ny / l;
+        nz = nz / l;
+    }
+    const T4<T> w = om[i], u = vx[i];
+    const T sc = c.dt * c.vort_eps;
+    v[perm[i]] = T4<T>{u.x + sc * (ny * w.z - nz * w.y), u.y + sc * (nz * w.x - nx * w.z),
+                       u.z + sc * (nx * w.y - ny * w.x), 0};
+}
+
+// ------------------------------------------------------------------ reverse
+// vorticity pass B reversed: from vbar (by id) to vxbar, ombar (own part),
+// obar and pbar (scattered), and d/d eps_v.
+template <class T>
+__global__ __launch_bounds__(AB) void kr_vort_b(AConsts<T> c, const uint32_t *__restrict__ perm,
+                                                const T4<T> *__restrict__ p, const int32_t *__restrict__ nbr9,
+                                                const T4<T> *__restrict__ om, const T *__restrict__ o,
+                                                const T4<T> *__restrict__ vbar, T4<T> *__restrict__ vxbar,
+                                                T4<T> *__restrict__ ombar, T *__restrict__ obar,
+                                                T4<T> *__restrict__ pbar, double *__restrict__ acc) {
+    const int32_t i = blockIdx.x * AB + threadIdx.x;
+    double deps = 0.0;
+    if (i < c.n) {
+        const T4<T> pi = p[i];
+        // recompute eta
+        T nx = 0, ny = 0, nz = 0;
+        each_candidate(nbr9, c.n, i, [&](int32_t j) {
+            const T4<T> pj = load_nb(p, j, c.n);
+            const T ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
+            const T sq = sqlen(ex, ey, ez);
+            const T r = sqrt(sq);
+            if (r > c.h || r == (T)0) return;
+            const T oj = j < c.n ? o[j] : (T)0;
+            const T s = gspiky(c, r);
+            nx += oj * (s * ex);
+            ny += oj * (s * ey);
+            nz += oj * (s * ez);
+        });
+        const T l = sqrt(sqlen(nx, ny, nz));
+        T Nx = 0, Ny = 0, Nz = 0;
+        if (l > (T)0) {
+            Nx = nx / l;
+            Ny = ny / l;
+            Nz = nz / l;
+        }
+        const T4<T> w = om[i];
+        const T4<T> a = vbar[perm[i]];
+        vxbar[i] = T4<T>{a.x, a.y, a.z, 0};
+        const T sc = c.dt * c.vort_eps;
+        // d/d eps_v of a . (dt eps (N x w))
+        const T cx = Ny * w.z - Nz * w.y, cy = Nz * w.x - Nx * w.z, cz = Nx * w.y - Ny * w.x;
+        deps = (double)(c.dt * (a.x * cx + a.y * cy + a.z * cz));
+        // a . (N x w) = N . (w x a) = w . (a x N)
+        const T Nbx = sc * (w.y * a.z - w.z * a.y), Nby = sc * (w.z * a.x - w.x * a.z),
+                Nbz = sc * (w.x * a.y - w.y * a.x);
+        ombar[i] = T4<T>{sc * (a.y * Nz - a.z * Ny), sc * (a.z * Nx - a.x * Nz), sc * (a.x * Ny - a.y * Nx), 0};
+        // N = eta / |eta|: etabar = (Nbar - N (N . Nbar)) / |eta|
+        T ebx = 0, eby = 0, ebz = 0;
+        if (l > (T)0) {
+            const T nn = Nx * Nbx + Ny * Nby + Nz * Nbz;
+            ebx = (Nbx - Nx * nn) / l;
+            eby = (Nby - Ny * nn) / l;
+            ebz = (Nbz - Nz * nn) / l;
+        }
+        if (l > (T)0) {
+            T pbx = 0, pby = 0, pbz = 0;
+            each_candidate(nbr9, c.n, i, [&](int32_t j) {
+                const T4<T> pj = load_nb(p, j, c.n);
+                const T ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
+                const T sq = sqlen(ex, ey, ez);
+                const T r = sqrt(sq);
+                if (r > c.h || r == (T)0) return;
+                const T s = gspiky(c, r);
+                if (j < c.n) atomicAdd(&obar[j], s * (ex * ebx + ey * eby + ez * ebz));
+                const T oj = j < c.n ? o[j] : (T)0;
+                T jx, jy, jz;
+                jspiky(c, r, ex, ey, ez, ebx, eby, ebz, jx, jy, jz);
+                pbx += oj * jx;
+                pby += oj * jy;
+                pbz += oj * jz;
+                if (j < c.n) atomic_add3(pbar, j, -oj * jx, -oj * jy, -oj * jz);
+            });
+            atomic_add3(pbar, i, pbx, pby, pbz);
+        }
+    }
+    block_add2(0.0, deps, acc);
+}
+
+// |omega| reversed: ombar += obar omega / |omega|.
+template <class T>
+__global__ __launch_bounds__(AB) void kr_omega_norm(int32_t n, const T4<T> *__restrict__ om, const T *__restrict__ o,
+                                                    const T *__restrict__ obar, T4<T> *__restrict__ ombar) {
+    const int32_t i = blockIdx.x * AB + threadIdx.x;
+    if (i >= n) return;
+    const T oi = o[i];
+    if (oi > (T)0) {
+        const T f = obar[i] / oi;
+        const T4<T> w = om[i];
+        T4<T> b = ombar[i];
+        b.x += f * w.x;
+        b.y += f * w.y;
+        b.z += f * w.z;
+        ombar[i] = b;
+    }
+}
+
+// vorticity pass A reversed: from vxbar and ombar to vsbar and pbar (both
+// scattered), and d/d xsph_c.
+template <class T>
+__global__ __launch_bounds__(AB) void kr_vort_a(AConsts<T> c, const T4<T> *__restrict__ p,
+                                                const int32_t *__restrict__ nbr9, const T4<T> *__restrict__ vs,
+                                                const T4<T> *__restrict__ vxbar, const T4<T> *__restrict__ ombar,
+                                                T4<T> *__restrict__ vsbar, T4<T> *__restrict__ pbar,
+                                                double *__restrict__ acc) {
+    const int32_t i = blockIdx.x * AB + threadIdx.x;
+    double dc = 0.0;
+    if (i < c.n) {
+        const T4<T> pi = p[i], vi = vs[i], a = vxbar[i], wb = ombar[i];
+        T sx = 0, sy = 0, sz = 0;
+        T vbx = a.x, vby = a.y, vbz = a.z;  // vx = v + ...
+        T pbx = 0, pby = 0, pbz = 0;
+        each_candidate(nbr9, c.n, i, [&](int32_t j) {
+            const T4<T> pj = load_nb(p, j, c.n);
+            const T ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
+            const T sq = sqlen(ex, ey, ez);
+            const T r = sqrt(sq);
+            if (r > c.h) return;
+            if (j >= c.n) return;  // zero record: its velocity is a constant here
+            const T4<T> vj = vs[j];
+            const T ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
+            const T W = wpoly(c, sq);
+            sx += ux * W;
+            sy += uy * W;
+            sz += uz * W;
+            // ubar = c W a + gradW x ombar
+            T ubx = c.xsph_c * W * a.x, uby = c.xsph_c * W * a.y, ubz = c.xsph_c * W * a.z;
+            // d(c W (u . a))/dd
+            const T dw = dwpoly(c, sq) * c.xsph_c * (ux * a.x + uy * a.y + uz * a.z);
+            T dx = dw * ex, dy = dw * ey, dz = dw * ez;
+            if (r > (T)0) {
+                const T s = gspiky(c, r);
+                const T gx = s * ex, gy = s * ey, gz = s * ez;
+                ubx += gy * wb.z - gz * wb.y;
+                uby += gz * wb.x - gx * wb.z;
+                ubz += gx * wb.y - gy * wb.x;
+                // ombar . (u x gradW) = gradW . (ombar x u)
+                const T qx = wb.y * uz - wb.z * uy, qy = wb.z * ux - wb.x * uz, qz = wb.x * uy - wb.y * ux;
+                T jx, jy, jz;
+                jspiky(c, r, ex, ey, ez, qx, qy, qz, jx, jy, jz);
+                dx += jx;
+                dy += jy;
+                dz += jz;
+            }
+            atomic_add3(vsbar, j, ubx, uby, ubz);
+            vbx -= ubx;
+            vby -= uby;
+            vbz -= ubz;
+            pbx += dx;
+            pby += dy;
+            pbz += dz;
+            atomic_add3(pbar, j, -dx, -dy, -dz);
+        });
+        atomic_add3(vsbar, i, vbx, vby, vbz);
+        atomic_add3(pbar, i, pbx, pby, pbz);
+        dc = (double)(a.x * sx + a.y * sy + a.z * sz);
+    }
+    block_add2(dc, 0.0, acc);
+}
+
+// update reversed: pbar_I += xbar[id] + vsbar/dt ; xbar_old[id] = -vsbar/dt.
+// Without vorticity vsbar is the incoming vbar[id].
+template <class T>
+__global__ __launch_bounds__(AB) void kr_update(AConsts<T> c, const uint32_t *__restrict__ perm,
+                                                const T4<T> *__restrict__ xbar_new, const T4<T> *__restrict__ vbar_new,
+                                                const T4<T> *__restrict__ vsbar, T4<T> *__restrict__ pbar,
+                                                T4<T> *__restrict__ xbar_old) {
+    const int32_t g = blockIdx.x * AB + threadIdx.x;
+    if (g >= c.n) return;
+    const uint32_t id = perm[g];
+    const T4<T> vb = vsbar ? vsbar[g] : vbar_new[id];
+    const T4<T> xb = xbar_new[id];
+    T4<T> pb = pbar[g];
+    pb.x += xb.x + vb.x / c.dt;
+    pb.y += xb.y + vb.y / c.dt;
+    pb.z += xb.z + vb.z / c.dt;
+    pbar[g] = pb;
+    xbar_old[id] = T4<T>{-vb.x / c.dt, -vb.y / c.dt, -vb.z / c.dt, 0};
+}
+
+// position update reversed: pbar_prev = mask pbar_next + pair terms;
+// lambar scattered.
+template <class T>
+__global__ __launch_bounds__(AB) void kr_updatepos(AConsts<T> c, const T4<T> *__restrict__ p,
+                                                   const T *__restrict__ lam, const int32_t *__restrict__ nbr9,
+                                                   const uint8_t *__restrict__ mask, const T4<T> *__restrict__ pbn,
+                                                   T4<T> *__restrict__ pbp, T *__restrict__ lambar) {
+    const int32_t i = blockIdx.x * AB + threadIdx.x;
+    if (i >= c.n) return;
+    const T4<T> pi = p[i];
+    const uint8_t m = mask[i];
+    const T4<T> b = pbn[i];
+    const T ax = (m & 1) ? b.x : (T)0, ay = (m & 2) ? b.y : (T)0, az = (m & 4) ? b.z : (T)0;
+    T pbx = ax, pby = ay, pbz = az;
+    const T li = lam[i];
+    T lb = 0;
+    const T k4 = (T)-4 * c.k * c.scale * c.scale * c.scale * c.scale;
+    each_candidate(nbr9, c.n, i, [&](int32_t j) {
+        const T4<T> pj = load_nb(p, j, c.n);
+        const T ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
+        const T sq = sqlen(ex, ey, ez);
+        const T r = sqrt(sq);
+        if (r > c.h || r == (T)0) return;
+        const T W = wpoly(c, sq);
+        T sc = c.scale * W;
+        sc *= sc;
+        sc *= sc;
+        sc = -c.k * sc;
+        const T lj = j < c.n ? lam[j] : (T)0;
+        const T f = li + lj + sc;
+        const T s = gspiky(c, r);
+        const T gdot = c.inv_rho0 * s * (ex * ax + ey * ay + ez * az);  // (1/rho0) gradW . a
+        lb += gdot;
+        if (j < c.n) atomicAdd(&lambar[j], gdot);
+        // d/dd of (1/rho0) f gradW . a: f J a + (gradW . a) ds/dd
+        T jx, jy, jz;
+        jspiky(c, r, ex, ey, ez, ax, ay, az, jx, jy, jz);
+        const T ds = k4 * W * W * W * dwpoly(c, sq);  // ds/dd = -4k scale^4 W^3 dW/dd (factor of d)
+        const T dx = c.inv_rho0 * f * jx + gdot * ds * ex;
+        const T dy = c.inv_rho0 * f * jy + gdot * ds * ey;
+        const T dz = c.inv_rho0 * f * jz + gdot * ds * ez;
+        pbx += dx;
+        pby += dy;
+        pbz += dz;
+        if (j < c.n) atomic_add3(pbp, j, -dx, -dy, -dz);
+    });
+    atomicAdd(&lambar[i], lb);
+    atomic_add3(pbp, i, pbx, pby, pbz);
+}
+
+// lambda reversed: from lambar to pbar (scattered).
+template <class T>
+__global__ __launch_bounds__(AB) void kr_lambda(AConsts<T> c, const T4<T> *__restrict__ p,
+                                                const int32_t *__restrict__ nbr9, const T *__restrict__ lam,
+                                                const T *__restrict__ sden, const T4<T> *__restrict__ G,
+                                                const T *__restrict__ lambar, T4<T> *__restrict__ pbar) {
+    const int32_t i = blockIdx.x * AB + threadIdx.x;
+    if (i >= c.n) return;
+    const T lb = lambar[i];
+    if (lb == (T)0) return;
+    const T4<T> pi = p[i];
+    const T S = sden[i];
+    // lam = -C/S: rhobar = -lb/(rho0 S), Sbar = -lam lb / S (C = -lam S)
+    const T rb = -lb * c.inv_rho0 / S;
+    const T sb = -lam[i] * lb / S;
+    const T4<T> Gi = G[i];
+    T pbx = 0, pby = 0, pbz = 0;
+    each_candidate(nbr9, c.n, i, [&](int32_t j) {
+        const T4<T> pj = load_nb(p, j, c.n);
+        const T ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
+        const T sq = sqlen(ex, ey, ez);
+        const T r = sqrt(sq);
+        if (r > c.h) return;
+        const T dw = rb * dwpoly(c, sq);
+        T dx = dw * ex, dy = dw * ey, dz = dw * ez;
+        if (r > (T)0) {
+            const T s = gspiky(c, r);
+            // gbar = 2 Sbar (g + G), g = gradW / rho0; d/dd = J gbar / rho0
+            const T gx = 2 * sb * (s * ex * c.inv_rho0 + Gi.x), gy = 2 * sb * (s * ey * c.inv_rho0 + Gi.y),
+                    gz = 2 * sb * (s * ez * c.inv_rho0 + Gi.z);
+            T jx, jy, jz;
+            jspiky(c, r, ex, ey, ez, gx, gy, gz, jx, jy, jz);
+            dx += c.inv_rho0 * jx;
+            dy += c.inv_rho0 * jy;
+            dz += c.inv_rho0 * jz;
+        }
+        pbx += dx;
+        pby += dy;
+        pbz += dz;
+        if (j < c.n) atomic_add3(pbar, j, -dx, -dy, -dz);
+    });
+    atomic_add3(pbar, i, pbx, pby, pbz);
+}
+
+// gather and predict reversed: xsbar[perm[g]] = pbar0[g];
+// xbar_old += xsbar ; vbar_old = dt xsbar (v enters the step only through x*).
+template <class T>
+__global__ __launch_bounds__(AB) void kr_predict(AConsts<T> c, const uint32_t *__restrict__ perm,
+                                                 const T4<T> *__restrict__ pbar0, T4<T> *__restrict__ xbar_old,
+                                                 T4<T> *__restrict__ vbar_old) {
+    const int32_t g = blockIdx.x * AB + threadIdx.x;
+    if (g >= c.n) return;
+    const uint32_t id = perm[g];
+    const T4<T> b = pbar0[g];
+    T4<T> x = xbar_old[id];
+    x.x += b.x;
+    x.y += b.y;
+    x.z += b.z;
+    xbar_old[id] = x;
+    vbar_old[id] = T4<T>{c.dt * b.x, c.dt * b.y, c.dt * b.z, 0};
+}
+
+template <class T>
+__global__ __launch_bounds__(AB) void ka_zero(T *__restrict__ p, size_t count) {
+    for (size_t k = (size_t)blockIdx.x * AB + threadIdx.x; k < count; k += (size_t)gridDim.x * AB) p[k] = (T)0;
+}
+
+unsigned ablocks(size_t n) { return (unsigned)((n + AB - 1) / AB); }
+unsigned zblocks(size_t n) {
+    const size_t b = (n + AB - 1) / AB;
+    return (unsigned)(b > 4096 ? 4096 : (b ? b : 1));
+}
+
+// ------------------------------------------------------------------ host
+struct AdjBase {
+    virtual ~AdjBase() {}
+    virtual int forward(const void *pos4, const void *vel4, int host, int keep) = 0;
+    virtual int positions(void *pos4, int host) = 0;
+    virtual int velocities(void *vel4, int host) = 0;
+    virtual int backward(const void *gpos4, int host, void *gvel4, double *gc, double *geps) = 0;
+    virtual int replay(const void *pos4, const void *vel4, int host, void *pos_out4) = 0;
+    virtual void set_coefficients(double c, double eps) = 0;
+    virtual void set_stream(hipStream_t st) = 0;
+    virtual hipStream_t stream() const = 0;
+};
+
+template <class T>
+struct StepTape {
+    uint32_t *perm = nullptr;
+    int32_t *nbr9 = nullptr;
+    T4<T> *p = nullptr;      // (I + 1) x n positions of the iterations (sorted)
+    T *lam = nullptr, *sden = nullptr;  // I x n
+    T4<T> *G = nullptr;      // I x n
+    uint8_t *mask = nullptr; // I x n
+    T4<T> *vs = nullptr, *vx = nullptr, *om = nullptr;
+    T *o = nullptr;
+};
+
+template <class T>
+struct Adj : AdjBase {
+    uint32_t n;
+    int32_t grid[3];
+    size_t cells;
+    int key_bits;
+    int32_t steps, every, iters;
+    pbf_params params;
+    SolverConsts fc;         // float constants of the structure kernels
+    AConsts<T> c;
+    hipStream_t own = nullptr, st = nullptr;
+    // structure machinery (float)
+    float4 *xs_f = nullptr, *rec_f = nullptr;
+    uint32_t *keys = nullptr;
+    RadixScratch rs;
+    int32_t *start = nullptr;
+    unsigned long long *end64 = nullptr;
+    uint32_t *epoch = nullptr;
+    // state and checkpoints
+    T4<T> *x = nullptr, *v = nullptr, *xf = nullptr, *vf = nullptr;  // working and final state
+    std::vector<T4<T> *> ck_x, ck_v;
+    std::vector<unsigned long long *> ck_end;
+    std::vector<uint32_t *> ck_epoch;
+    // structure of every step (kept on request, for replays)
+    uint32_t *all_perm = nullptr;
+    int32_t *all_nbr = nullptr;
+    bool have_all = false;
+    // tape of one segment
+    std::vector<StepTape<T>> tape;
+    T4<T> *xs = nullptr;
+    T *lam_tmp = nullptr;
+    // reverse buffers
+    T4<T> *xb = nullptr, *vb = nullptr, *xb2 = nullptr, *vb2 = nullptr, *pb_a = nullptr, *pb_b = nullptr;
+    T4<T> *vxb = nullptr, *omb = nullptr, *vsb = nullptr;
+    T *ob = nullptr, *lb = nullptr;
+    double *acc = nullptr;
+    bool forwarded = false;
+
+    ~Adj() override { release(); }
+
+    void release() {
+        dfree(xs_f);
+        dfree(rec_f);
+        dfree(keys);
+        radix_free(rs);
+        dfree(start);
+        dfree(end64);
+        dfree(epoch);
+        dfree(x);
+        dfree(v);
+        dfree(xf);
+        dfree(vf);
+        for (auto &p : ck_x) dfree(p);
+        for (auto &p : ck_v) dfree(p);
+        for (auto &p : ck_end) dfree(p);
+        for (auto &p : ck_epoch) dfree(p);
+        dfree(all_perm);
+        dfree(all_nbr);
+        for (auto &t : tape) {
+            dfree(t.perm);
+            dfree(t.nbr9);
+            dfree(t.p);
+            dfree(t.lam);
+            dfree(t.sden);
+            dfree(t.G);
+            dfree(t.mask);
+            dfree(t.vs);
+            dfree(t.vx);
+            dfree(t.om);
+            dfree(t.o);
+        }
+        dfree(xs);
+        dfree(lam_tmp);
+        dfree(xb);
+        dfree(vb);
+        dfree(xb2);
+        dfree(vb2);
+        dfree(pb_a);
+        dfree(pb_b);
+        dfree(vxb);
+        dfree(omb);
+        dfree(vsb);
+        dfree(ob);
+        dfree(lb);
+        dfree(acc);
+        if (own) (void)hipStreamDestroy(own);
+        own = nullptr;
+    }
+
+    hipError_t init(uint32_t n_, const int32_t g[3], const pbf_params &p, int32_t steps_, int32_t every_) {
+        n = n_;
+        for (int a = 0; a < 3; a++) grid[a] = g[a];
+        cells = (size_t)g[0] * g[1] * g[2];
+        key_bits = key_bits_of(grid);
+        steps = steps_;
+        every = every_;
+        params = p;
+        iters = (int32_t)p.iterations;
+        fc = make_consts(n, grid, p, key_bits);
+        set_consts(p.xsph_viscosity_c, p.vorticity_epsilon);
+        hipError_t e;
+        if ((e = hipStreamCreateWithFlags(&own, hipStreamNonBlocking))) return e;
+        st = own;
+        if ((e = dalloc(&xs_f, n)) || (e = dalloc(&rec_f, n)) || (e = dalloc(&keys, n))) return e;
+        if (!radix_alloc(rs, (int32_t)n)) return hipErrorOutOfMemory;
+        if ((e = dalloc(&start, cells)) || (e = dalloc(&end64, cells)) || (e = dalloc(&epoch, 1))) return e;
+        if ((e = dalloc(&x, n)) || (e = dalloc(&v, n)) || (e = dalloc(&xf, n)) || (e = dalloc(&vf, n))) return e;
+        const int32_t nck = (steps + every - 1) / every;
+        ck_x.assign(nck, nullptr);
+        ck_v.assign(nck, nullptr);
+        ck_end.assign(nck, nullptr);
+        ck_epoch.assign(nck, nullptr);
+        for (int32_t k = 0; k < nck; k++) {
+            if ((e = dalloc(&ck_x[k], n)) || (e = dalloc(&ck_v[k], n)) || (e = dalloc(&ck_end[k], cells)) ||
+                (e = dalloc(&ck_epoch[k], 1)))
+                return e;
+        }
+        if ((e = dalloc(&all_perm, (size_t)steps * n)) || (e = dalloc(&all_nbr, (size_t)steps * 9 * n))) return e;
+        tape.assign(every, StepTape<T>());
+        for (auto &t : tape) {
+            if ((e = dalloc(&t.perm, n)) || (e = dalloc(&t.nbr9, (size_t)9 * n)) ||
+                (e = dalloc(&t.p, (size_t)(iters + 1) * n)) || (e = dalloc(&t.lam, (size_t)iters * n)) ||
+                (e = dalloc(&t.sden, (size_t)iters * n)) || (e = dalloc(&t.G, (size_t)iters * n)) ||
+                (e = dalloc(&t.mask, (size_t)iters * n)) || (e = dalloc(&t.vs, n)) || (e = dalloc(&t.vx, n)) ||
+                (e = dalloc(&t.om, n)) || (e = dalloc(&t.o, n)))
+                return e;
+        }
+        if ((e = dalloc(&xs, n)) || (e = dalloc(&lam_tmp, n))) return e;
+        if ((e = dalloc(&xb, n)) || (e = dalloc(&vb, n)) || (e = dalloc(&xb2, n)) || (e = dalloc(&vb2, n)) ||
+            (e = dalloc(&pb_a, n)) || (e = dalloc(&pb_b, n)) || (e = dalloc(&vxb, n)) || (e = dalloc(&omb, n)) ||
+            (e = dalloc(&vsb, n)) || (e = dalloc(&ob, n)) || (e = dalloc(&lb, n)) || (e = dalloc(&acc, 2)))
+            return e;
+        return hipSuccess;
+    }
+
+    void set_consts(double xsph, double veps) {
+        const pbf_params &p = params;
+        c.n = (int32_t)n;
+        c.gx = grid[0];
+        c.gy = grid[1];
+        c.gz = grid[2];
+        const T h = (T)p.h;
+        c.h = h;
+        c.hh = h * h;
+        c.h6 = h * h * h * h * h * h;
+        c.h9 = h * h * h * h * h * h * h * h * h;
+        c.inv_rho0 = (T)p.one_over_rho_0;
+        c.eps = (T)p.epsilon;
+        c.gravity = (T)p.gravity;
+        c.dt = (T)p.timestep;
+        c.k = (T)p.tensile_instability_k;
+        c.scale = (T)p.tensile_instability_scale;
+        c.xsph_c = (T)xsph;
+        c.vort_eps = (T)veps;
+        for (int a = 0; a < 3; a++) {
+            c.lo[a] = (T)p.wall_lo[a];
+            c.hi[a] = (T)p.wall_hi[a];
+            c.gdir[a] = (T)p.gravity_dir[a];
+        }
+        c.extforce = p.extforce;
+        c.vorticity = p.vorticity;
+    }
+    void set_coefficients(double cc, double eps) override { set_consts(cc, eps); }
+    void set_stream(hipStream_t s) override { st = s ? s : own; }
+    hipStream_t stream() const override { return st; }
+
+    template <class U>
+    void zero(U *p, size_t count) {
+        ka_zero<U><<<zblocks(count), AB, 0, st>>>(p, count);
+    }
+
+    // Structure of the step from state (x, v): predicted records, sort order
+    // and neighbour table, by the plain step's kernels (float), into perm/nbr9.
+    void structure(uint32_t *perm, int32_t *nbr9) {
+        launch_keys_of_records(fc, xs_f, keys, st);
+        radix_sort_perm(rs, keys, (int32_t)n, key_bits, perm, st);
+        launch_gather_records((int32_t)n, perm, xs_f, rec_f, st);
+        launch_begin_cells(start, cells, epoch, st);
+        launch_findcells(fc, rec_f, start, end64, epoch, st);
+        launch_neighbourcells(fc, rec_f, start, end64, nbr9, st);
+    }
+
+    // One step on (x, v) in place. With a tape, everything the reverse pass
+    // reads is kept; perm/nbr9 given: fixed structure, else computed into them.
+    void step(StepTape<T> *tp, uint32_t *perm, int32_t *nbr9, bool compute_structure) {
+        const unsigned nb = ablocks(n);
+        ka_predict<T><<<nb, AB, 0, st>>>(c, x, v, xs, xs_f);
+        if (compute_structure) structure(perm, nbr9);
+        T4<T> *p0 = tp ? tp->p : pb_a;  // pb_a/pb_b double as iteration buffers without a tape
+        ka_gather<T><<<nb, AB, 0, st>>>((int32_t)n, perm, xs, p0);
+        T4<T> *cur = p0;
+        for (int32_t it = 0; it < iters; it++) {
+            T4<T> *nxt = tp ? tp->p + (size_t)(it + 1) * n : (cur == pb_a ? pb_b : pb_a);
+            T *lam = tp ? tp->lam + (size_t)it * n : lam_tmp;
+            ka_lambda<T><<<nb, AB, 0, st>>>(c, cur, nbr9, lam, tp ? tp->sden + (size_t)it * n : nullptr,
+                                            tp ? tp->G + (size_t)it * n : nullptr);
+            ka_updatepos<T><<<nb, AB, 0, st>>>(c, cur, lam, nbr9, nxt, tp ? tp->mask + (size_t)it * n : nullptr);
+            cur = nxt;
+        }
+        T4<T> *vs = tp ? tp->vs : vsb;
+        ka_update<T><<<nb, AB, 0, st>>>(c, perm, cur, x, v, vs);
+        if (c.vorticity) {
+            T4<T> *vx = tp ? tp->vx : vxb;
+            T4<T> *om = tp ? tp->om : omb;
+            T *o = tp ? tp->o : ob;
+            // a zero record's velocity is that of id 0 (imageLoad (velocitytexture, 0))
+            ka_vort_a<T><<<nb, AB, 0, st>>>(c, cur, nbr9, vs, v, vx, om, o);
+            ka_vort_b<T><<<nb, AB, 0, st>>>(c, perm, cur, nbr9, vx, om, o, v);
+        }
+    }
+
+    hipError_t save_ck(int32_t k) {
+        hipError_t e;
+        if ((e = hipMemcpyAsync(ck_x[k], x, n * sizeof(T4<T>), hipMemcpyDeviceToDevice, st))) return e;
+        if ((e = hipMemcpyAsync(ck_v[k], v, n * sizeof(T4<T>), hipMemcpyDeviceToDevice, st))) return e;
+        if ((e = hipMemcpyAsync(ck_end[k], end64, cells * 8, hipMemcpyDeviceToDevice, st))) return e;
+        return hipMemcpyAsync(ck_epoch[k], epoch, 4, hipMemcpyDeviceToDevice, st);
+    }
+    hipError_t load_ck(int32_t k) {
+        hipError_t e;
+        if ((e = hipMemcpyAsync(x, ck_x[k], n * sizeof(T4<T>), hipMemcpyDeviceToDevice, st))) return e;
+        if ((e = hipMemcpyAsync(v, ck_v[k], n * sizeof(T4<T>), hipMemcpyDeviceToDevice, st))) return e;
+        if ((e = hipMemcpyAsync(end64, ck_end[k], cells * 8, hipMemcpyDeviceToDevice, st))) return e;
+        return hipMemcpyAsync(epoch, ck_epoch[k], 4, hipMemcpyDeviceToDevice, st);
+    }
+
+    hipError_t upload(T4<T> *dx, T4<T> *dv, const void *pos4, const void *vel4, int host) {
+        const hipMemcpyKind kd = host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+        hipError_t e;
+        if ((e = hipMemcpyAsync(dx, pos4, n * sizeof(T4<T>), kd, st))) return e;
+        if ((e = hipMemcpyAsync(dv, vel4, n * sizeof(T4<T>), kd, st))) return e;
+        return host ? hipStreamSynchronize(st) : hipSuccess;
+    }
+
+    int forward(const void *pos4, const void *vel4, int host, int keep) override {
+        PBF_HIP(upload(x, v, pos4, vel4, host));
+        // a fresh end table, as a new SPH handle's
+        PBF_HIP(hipMemsetAsync(end64, 0, cells * 8, st));
+        PBF_HIP(hipMemsetAsync(epoch, 0, 4, st));
+        for (int32_t s = 0; s < steps; s++) {
+            if (s % every == 0) PBF_HIP(save_ck(s / every));
+            uint32_t *perm = all_perm + (size_t)(keep ? s : 0) * n;
+            int32_t *nbr = all_nbr + (size_t)(keep ? s : 0) * 9 * n;
+            step(nullptr, perm, nbr, true);
+        }
+        PBF_HIP(hipMemcpyAsync(xf, x, n * sizeof(T4<T>), hipMemcpyDeviceToDevice, st));
+        PBF_HIP(hipMemcpyAsync(vf, v, n * sizeof(T4<T>), hipMemcpyDeviceToDevice, st));
+        have_all = keep != 0;
+        forwarded = true;
+        PBF_HIP(hipGetLastError());
+        return PBF_OK;
+    }
+
+    int positions(void *pos4, int host) override {
+        PBF_HIP(hipMemcpyAsync(pos4, xf, n * sizeof(T4<T>), host ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice,
+                               st));
+        if (host) PBF_HIP(hipStreamSynchronize(st));
+        return PBF_OK;
+    }
+    int velocities(void *vel4, int host) override {
+        PBF_HIP(hipMemcpyAsync(vel4, vf, n * sizeof(T4<T>), host ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice,
+                               st));
+        if (host) PBF_HIP(hipStreamSynchronize(st));
+        return PBF_OK;
+    }
+
+    // Reverse pass of one step from its tape: (xb, vb) after the step -> (xb, vb) before it.
+    void reverse_step(const StepTape<T> &t) {
+        const unsigned nb = ablocks(n);
+        const T4<T> *pI = t.p + (size_t)iters * n;
+        zero(reinterpret_cast<T *>(pb_a), 4 * (size_t)n);
+        const T4<T> *vsbar = nullptr;
+        if (c.vorticity) {
+            zero(ob, n);
+            zero(reinterpret_cast<T *>(vsb), 4 * (size_t)n);
+            kr_vort_b<T><<<nb, AB, 0, st>>>(c, t.perm, pI, t.nbr9, t.om, t.o, vb, vxb, omb, ob, pb_a, acc);
+            kr_omega_norm<T><<<nb, AB, 0, st>>>((int32_t)n, t.om, t.o, ob, omb);
+            kr_vort_a<T><<<nb, AB, 0, st>>>(c, pI, t.nbr9, t.vs, vxb, omb, vsb, pb_a, acc);
+            vsbar = vsb;
+        }
+        kr_update<T><<<nb, AB, 0, st>>>(c, t.perm, xb, vb, vsbar, pb_a, xb2);
+        T4<T> *next = pb_a, *prev = pb_b;
+        for (int32_t it = iters - 1; it >= 0; it--) {
+            const T4<T> *p = t.p + (size_t)it * n;
+            const T *lam = t.lam + (size_t)it * n;
+            zero(reinterpret_cast<T *>(prev), 4 * (size_t)n);
+            zero(lb, n);
+            kr_updatepos<T><<<nb, AB, 0, st>>>(c, p, lam, t.nbr9, t.mask + (size_t)it * n, next, prev, lb);
+            kr_lambda<T><<<nb, AB, 0, st>>>(c, p, t.nbr9, lam, t.sden + (size_t)it * n, t.G + (size_t)it * n, lb,
+                                            prev);
+            std::swap(next, prev);
+        }
+        kr_predict<T><<<nb, AB, 0, st>>>(c, t.perm, next, xb2, vb2);
+        std::swap(xb, xb2);
+        std::swap(vb, vb2);
+    }
+
+    int backward(const void *gpos4, int host, void *gvel4, double *gc, double *geps) override {
+        if (!forwarded) return invalid("pbf_adjoint_backward: no forward rollout");
+        PBF_HIP(hipMemcpyAsync(xb, gpos4, n * sizeof(T4<T>), host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice,
+                               st));
+        zero(reinterpret_cast<T *>(vb), 4 * (size_t)n);
+        zero(acc, 2);
+        const int32_t nck = (steps + every - 1) / every;
+        for (int32_t k = nck - 1; k >= 0; k--) {
+            const int32_t s0 = k * every, s1 = std::min(steps, s0 + every);
+            PBF_HIP(load_ck(k));
+            for (int32_t s = s0; s < s1; s++) {
+                StepTape<T> &t = tape[s - s0];
+                step(&t, t.perm, t.nbr9, true);
+            }
+            for (int32_t s = s1 - 1; s >= s0; s--) reverse_step(tape[s - s0]);
+        }
+        PBF_HIP(hipGetLastError());
+        if (gvel4)
+            PBF_HIP(hipMemcpyAsync(gvel4, vb, n * sizeof(T4<T>), host ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice,
+                                   st));
+        double h_acc[2] = {0.0, 0.0};
+        PBF_HIP(hipMemcpyAsync(h_acc, acc, sizeof h_acc, hipMemcpyDeviceToHost, st));
+        PBF_HIP(hipStreamSynchronize(st));
+        if (gc) *gc = h_acc[0];
+        if (geps) *geps = h_acc[1];
+        return PBF_OK;
+    }
+
+    int replay(const void *pos4, const void *vel4, int host, void *pos_out4) override {
+        if (!have_all) return invalid("pbf_adjoint_replay: the last forward kept no structure");
+        PBF_HIP(upload(x, v, pos4, vel4, host));
+        for (int32_t s = 0; s < steps; s++) step(nullptr, all_perm + (size_t)s * n, all_nbr + (size_t)s * 9 * n, false);
+        PBF_HIP(hipGetLastError());
+        PBF_HIP(hipMemcpyAsync(pos_out4, x, n * sizeof(T4<T>), host ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice,
+                               st));
+        if (host) PBF_HIP(hipStreamSynchronize(st));
+        return PBF_OK;
+    }
+};
+
+}  // namespace
+}  // namespace pbf
+
+using namespace pbf;
+
+struct pbf_adjoint {
+    AdjBase *impl = nullptr;
+    int fp64 = 0;
+};
+
+extern "C" {
+
+int pbf_adjoint_create(pbf_adjoint **out, uint32_t n, int32_t gx, int32_t gy, int32_t gz, const pbf_params *p,
+                       int32_t steps, int32_t checkpoint_every, int32_t fp64) {
+    if (!out) return invalid("null output handle");
+    *out = nullptr;
+    if (n == 0 || n % 256 != 0 || n >= (1u << 24)) return invalid("n must be a positive multiple of 256 below 2^24");
+    if (!valid_grid(gx, gy, gz)) return invalid("bad grid");
+    int rc = check_params(p);
+    if (rc) return rc;
+    if (steps < 1 || checkpoint_every < 1) return invalid("steps and checkpoint_every must be positive");
+    if (pbf_device_count() < 1) {
+        set_error("no HIP device");
+        return PBF_ERR_NODEVICE;
+    }
+    const int32_t g[3] = {gx, gy, gz};
+    pbf_adjoint *a = new (std::nothrow) pbf_adjoint();
+    if (!a) return invalid("host allocation failed");
+    a->fp64 = fp64 != 0;
+    hipError_t e;
+    if (a->fp64) {
+        auto *impl = new (std::nothrow) Adj<double>();
+        a->impl = impl;
+        e = impl ? impl->init(n, g, *p, steps, std::min(checkpoint_every, steps)) : hipErrorOutOfMemory;
+    } else {
+        auto *impl = new (std::nothrow) Adj<float>();
+        a->impl = impl;
+        e = impl ? impl->init(n, g, *p, steps, std::min(checkpoint_every, steps)) : hipErrorOutOfMemory;
+    }
+    if (e != hipSuccess) {
+        int r = hip_fail(e, "pbf_adjoint_create");
+        delete a->impl;
+        delete a;
+        return r;
+    }
+    *out = a;
+    return PBF_OK;
+}
+
+int pbf_adjoint_destroy(pbf_adjoint *a) {
+    if (!a) return PBF_OK;
+    delete a->impl;
+    delete a;
+    return PBF_OK;
+}
+
+int pbf_adjoint_set_coefficients(pbf_adjoint *a, double xsph_c, double vorticity_epsilon) {
+    if (!a) return invalid("null adjoint handle");
+    a->impl->set_coefficients(xsph_c, vorticity_epsilon);
+    return PBF_OK;
+}
+
+int pbf_adjoint_forward(pbf_adjoint *a, const void *pos4, const void *vel4, int host, int keep_structure) {
+    if (!a || !pos4 || !vel4) return invalid("null argument");
+    return a->impl->forward(pos4, vel4, host, keep_structure);
+}
+
+int pbf_adjoint_positions(pbf_adjoint *a, void *pos4, int host) {
+    if (!a || !pos4) return invalid("null argument");
+    return a->impl->positions(pos4, host);
+}
+
+int pbf_adjoint_velocities(pbf_adjoint *a, void *vel4, int host) {
+    if (!a || !vel4) return invalid("null argument");
+    return a->impl->velocities(vel4, host);
+}
+
+int pbf_adjoint_backward(pbf_adjoint *a, const void *grad_pos4, int host, void *grad_vel4, double *grad_xsph_c,
+                         double *grad_vorticity_epsilon) {
+    if (!a || !grad_pos4) return invalid("null argument");
+    return a->impl->backward(grad_pos4, host, grad_vel4, grad_xsph_c, grad_vorticity_epsilon);
+}
+
+int pbf_adjoint_replay(pbf_adjoint *a, const void *pos4, const void *vel4, int host, void *pos_out4) {
+    if (!a || !pos4 || !vel4 || !pos_out4) return invalid("null argument");
+    return a->impl->replay(pos4, vel4, host, pos_out4);
+}
+
+int pbf_adjoint_set_stream(pbf_adjoint *a, void *stream) {
+    if (!a) return invalid("null adjoint handle");
+    a->impl->set_stream((hipStream_t)stream);
+    return PBF_OK;
+}
+
+void *pbf_adjoint_get_stream(const pbf_adjoint *a) { return a ? (void *)a->impl->stream() : nullptr; }
+
+}  // extern "C"

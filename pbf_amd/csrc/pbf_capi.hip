@@ -34,9 +34,14 @@ int key_bits_of(const int32_t g[3]) {
     return (int)(2 * ((nb + 1) >> 1));
 }
 
-// Capacity of the per-particle neighbour list that the lambda pass records for
-// the position update (particles with more neighbours within h scan instead).
-constexpr int32_t kListCap = 64;
+
+int check_params(const pbf_params *p) {
+    if (!p) return invalid("null parameters");
+    if (!(p->h > 0.0f) || !std::isfinite(p->h)) return invalid("h must be positive and finite");
+    if (!(p->timestep != 0.0f) || !std::isfinite(p->timestep)) return invalid("timestep must be nonzero and finite");
+    if (p->iterations > 1000) return invalid("iterations above 1000");
+    return PBF_OK;
+}
 
 SolverConsts make_consts(uint32_t n, const int32_t grid[3], const pbf_params &p, int key_bits) {
     SolverConsts c;
@@ -68,7 +73,6 @@ SolverConsts make_consts(uint32_t n, const int32_t grid[3], const pbf_params &p,
     }
     c.extforce = p.extforce;
     c.key_mask = key_bits >= 32 ? 0xFFFFFFFFu : ((1u << key_bits) - 1u);
-    c.epoch = 0;
     return c;
 }
 
@@ -77,26 +81,6 @@ SolverConsts make_consts(uint32_t n, const int32_t grid[3], const pbf_params &p,
 using namespace pbf;
 
 namespace {
-
-int invalid(const std::string &msg) {
-    set_error(msg);
-    return PBF_ERR_INVALID;
-}
-
-bool valid_grid(int32_t gx, int32_t gy, int32_t gz) {
-    return gx > 0 && gy > 0 && gz > 0 && (uint64_t)gx * gy * gz < (1ull << 31);
-}
-
-template <class T>
-hipError_t dalloc(T **p, size_t count) {
-    return hipMalloc((void **)p, (count ? count : 1) * sizeof(T));
-}
-
-template <class T>
-void dfree(T *&p) {
-    if (p) (void)hipFree((void *)p);
-    p = nullptr;
-}
 
 hipMemcpyKind to_dev(int host) { return host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice; }
 hipMemcpyKind from_dev(int host) { return host ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice; }
@@ -107,7 +91,7 @@ struct CellTables {
     int32_t *start = nullptr;
     unsigned long long *end64 = nullptr;
     int32_t *nbr9 = nullptr;
-    uint32_t epoch = 0;
+    uint32_t *epoch = nullptr;  // device word: tag of the current step's end writes
 
     hipError_t alloc(uint32_t n, size_t c) {
         cells = c;
@@ -115,6 +99,8 @@ struct CellTables {
         if ((e = dalloc(&start, cells)) != hipSuccess) return e;
         if ((e = dalloc(&end64, cells)) != hipSuccess) return e;
         if ((e = dalloc(&nbr9, (size_t)9 * n)) != hipSuccess) return e;
+        if ((e = dalloc(&epoch, 1)) != hipSuccess) return e;
+        if ((e = hipMemset(epoch, 0, sizeof(uint32_t))) != hipSuccess) return e;
         // The reference's end texture starts as zeros (glTexImage3D with NULL
         // data reads 0 on llvmpipe) and is never cleared afterwards.
         return hipMemset(end64, 0, cells * sizeof(unsigned long long));
@@ -123,12 +109,12 @@ struct CellTables {
         dfree(start);
         dfree(end64);
         dfree(nbr9);
+        dfree(epoch);
     }
     // FindNeighbourCells (src/NeighbourCellFinder.cpp:113-148).
-    void find(SolverConsts c, const float4 *sorted, hipStream_t st) {
-        c.epoch = ++epoch;
-        launch_fill_i32(start, -1, cells, st);
-        launch_findcells(c, sorted, start, end64, st);
+    void find(const SolverConsts &c, const float4 *sorted, hipStream_t st) {
+        launch_begin_cells(start, cells, epoch, st);
+        launch_findcells(c, sorted, start, end64, epoch, st);
         launch_neighbourcells(c, sorted, start, end64, nbr9, st);
     }
 };
@@ -407,14 +393,6 @@ int pbf_sph_destroy(pbf_sph *s) {
     return PBF_OK;
 }
 
-static int check_params(const pbf_params *p) {
-    if (!p) return invalid("null parameters");
-    if (!(p->h > 0.0f) || !std::isfinite(p->h)) return invalid("h must be positive and finite");
-    if (!(p->timestep != 0.0f) || !std::isfinite(p->timestep)) return invalid("timestep must be nonzero and finite");
-    if (p->iterations > 1000) return invalid("iterations above 1000");
-    return PBF_OK;
-}
-
 int pbf_sph_set_params(pbf_sph *s, const pbf_params *p) {
     if (!s) return invalid("null SPH handle");
     int rc = check_params(p);
@@ -470,13 +448,9 @@ int pbf_sph_run(pbf_sph *s) {
     PBF_HIP(hipEventRecord(s->ev[2], st));
     // neighbour cells (src/SPH.cpp:270-275; src/NeighbourCellFinder.cpp:113-148)
     CellTables &t = s->cells_t;
-    {
-        SolverConsts cc = c;
-        cc.epoch = ++t.epoch;
-        TIMED(KID_CLEAR_START, launch_fill_i32(t.start, -1, t.cells, st));
-        TIMED(KID_FINDCELLS, launch_findcells(cc, s->rec0, t.start, t.end64, st));
-        TIMED(KID_NEIGHBOURCELLS, launch_neighbourcells(cc, s->rec0, t.start, t.end64, t.nbr9, st));
-    }
+    TIMED(KID_CLEAR_START, launch_begin_cells(t.start, t.cells, t.epoch, st));
+    TIMED(KID_FINDCELLS, launch_findcells(c, s->rec0, t.start, t.end64, t.epoch, st));
+    TIMED(KID_NEIGHBOURCELLS, launch_neighbourcells(c, s->rec0, t.start, t.end64, t.nbr9, st));
     PBF_HIP(hipEventRecord(s->ev[3], st));
     // solver iterations (src/SPH.cpp:303-311), race-free: x* -> a -> b -> a ...
     PairList pl;

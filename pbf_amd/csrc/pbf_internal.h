@@ -25,10 +25,37 @@ struct SolverConsts {
     float lo[3], hi[3], gdir[3];
     int32_t extforce;
     uint32_t key_mask;
-    uint32_t epoch;       // tag of this step's end-table writes
 };
 
 SolverConsts make_consts(uint32_t n, const int32_t grid[3], const pbf_params &p, int key_bits);
+
+// One scene's slice of the particle arrays and its own constants. Sorted and
+// id-order arrays hold each scene's particles in the same contiguous range
+// [off, off + n); records carry scene-local ids, the neighbour table and the
+// cell tables scene-local particle indices, so every array restricted to a
+// scene equals the plain solver's array for that scene.
+struct Seg {
+    int32_t off, n;
+    int32_t gx, gy, gz;
+    int32_t cell_base;          // first entry of the scene's cells in the cell tables
+    uint32_t key_mask, key_hi;  // key = key_hi | (cell hash & key_mask)
+    float lo[3], hi[3], gdir[3];
+    float xsph_c;
+};
+
+Seg seg_of(const SolverConsts &c);
+
+// Where a kernel finds the scene of global slot i: one scene for the whole
+// launch (kernel argument), or a table indexed by a per-slot scene number.
+struct DomOne {
+    Seg s;
+    __device__ __forceinline__ Seg at(int32_t) const { return s; }
+};
+struct DomMany {
+    const Seg *segs;
+    const uint16_t *scene_of;
+    __device__ __forceinline__ Seg at(int32_t i) const { return segs[scene_of[i]]; }
+};
 
 // ------------------------------------------------------------ launchers
 void launch_predict(const SolverConsts &c, const float4 *pos, const float4 *vel, float4 *rec, uint32_t *keys,
@@ -39,14 +66,21 @@ void launch_gather_records(int32_t n, const uint32_t *perm, const float4 *rec_in
 void launch_gather_by_id(int32_t n, const float4 *rec, const float4 *src, float4 *out, hipStream_t st);
 void launch_keys_of_records(const SolverConsts &c, const float4 *rec, uint32_t *keys, hipStream_t st);
 void launch_fill_i32(int32_t *p, int32_t value, size_t count, hipStream_t st);
+// FindNeighbourCells' clear of the start table (src/NeighbourCellFinder.cpp:
+// 115-126), which also advances the device-side tag of the step's end writes.
+void launch_begin_cells(int32_t *start, size_t cells, uint32_t *epoch, hipStream_t st);
 void launch_findcells(const SolverConsts &c, const float4 *sorted, int32_t *start, unsigned long long *end64,
-                      hipStream_t st);
+                      const uint32_t *epoch, hipStream_t st);
 void launch_neighbourcells(const SolverConsts &c, const float4 *sorted, const int32_t *start,
                            const unsigned long long *end64, int32_t *nbr9, hipStream_t st);
 // Per-particle list of the neighbours within h at one set of positions:
 // written by the lambda pass (vorticity pass A), replayed by the position
 // update (vorticity pass B). entries [cap][n] of 16-bit (row, offset) codes.
 // All pointers null: the replaying pass scans its candidates.
+// Capacity of the per-particle neighbour list that the lambda pass records for
+// the position update (particles with more neighbours within h scan instead).
+constexpr int32_t kListCap = 64;
+
 struct PairList {
     uint16_t *entries = nullptr;
     uint16_t *count = nullptr;
@@ -63,6 +97,37 @@ void launch_vorticity_a(const SolverConsts &c, const float4 *rec, const int32_t 
                         hipStream_t st);
 void launch_vorticity_b(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, const PairList &pl,
                         const float4 *vxsph, const float4 *omega, const float *vort, float4 *vel_id, hipStream_t st);
+// Launch shape of the batched form: workgroup size of the neighbour passes
+// (128, 256, 512), workgroup size (64 .. 1024, a power of two) and particles
+// per thread (1 .. 4) of the streaming passes. The one-scene form uses the
+// default.
+struct LaunchShape {
+    int32_t nb = 256;
+    int32_t sb = 256;
+    int32_t ppt = 1;
+};
+bool valid_shape(const LaunchShape &ls);
+
+// Batched (many-scene) forms of the step's launches; n of c is the total.
+void launch_predict(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *pos,
+                    const float4 *vel, float4 *rec, uint32_t *keys, hipStream_t st);
+void launch_findcells(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *sorted,
+                      int32_t *start, unsigned long long *end64, const uint32_t *epoch, hipStream_t st);
+void launch_neighbourcells(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *sorted,
+                           const int32_t *start, const unsigned long long *end64, int32_t *nbr9, hipStream_t st);
+void launch_lambda(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec,
+                   const int32_t *nbr9, float *lambda, const PairList &pl, hipStream_t st);
+void launch_updatepos(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec_in,
+                      const float *lambda, const int32_t *nbr9, const PairList &pl, float4 *rec_out,
+                      hipStream_t st);
+void launch_update(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec, float4 *pos,
+                   float4 *vel, float4 *vel_sorted, hipStream_t st);
+void launch_vorticity_a(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec,
+                        const int32_t *nbr9, const float4 *vel_sorted, const float4 *vel_id, float4 *vxsph,
+                        float4 *omega, float *vort, const PairList &pl, hipStream_t st);
+void launch_vorticity_b(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec,
+                        const int32_t *nbr9, const PairList &pl, const float4 *vxsph, const float4 *omega,
+                        const float *vort, float4 *vel_id, hipStream_t st);
 void launch_nbr9_to_12(int32_t n, const int32_t *nbr9, int32_t *nbr12, hipStream_t st);
 void launch_nbr12_to_9(int32_t n, const int32_t *nbr12, int32_t *nbr9, hipStream_t st);
 void launch_end64_to_i32(size_t cells, const unsigned long long *end64, int32_t *end, hipStream_t st);
@@ -86,9 +151,31 @@ void radix_free(RadixScratch &s);
 // the scratch buffers.
 void radix_sort_perm(RadixScratch &s, uint32_t *keys, int32_t n, int key_bits, uint32_t *perm, hipStream_t st);
 
-// ------------------------------------------------------------ errors
+// ------------------------------------------------------------ errors and helpers
 void set_error(const std::string &msg);
 int hip_fail(hipError_t e, const char *what);
+int check_params(const pbf_params *p);
+int key_bits_of(const int32_t g[3]);
+
+inline int invalid(const std::string &msg) {
+    set_error(msg);
+    return PBF_ERR_INVALID;
+}
+
+inline bool valid_grid(int32_t gx, int32_t gy, int32_t gz) {
+    return gx > 0 && gy > 0 && gz > 0 && (uint64_t)gx * gy * gz < (1ull << 31);
+}
+
+template <class T>
+hipError_t dalloc(T **p, size_t count) {
+    return hipMalloc((void **)p, (count ? count : 1) * sizeof(T));
+}
+
+template <class T>
+void dfree(T *&p) {
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
 
 }  // namespace pbf
 

@@ -14,6 +14,8 @@
 // (src/SPH.cpp:119-133); the neighbour table is int32[9][n] (row-major over
 // the reference's 9 (dy,dz) rows, each start + (count << 24)); the end table
 // is uint64 {step tag, gid} so the newest step's highest gid wins.
+#include <type_traits>
+
 #include "pbf_internal.h"
 #include "pbf_math.h"
 
@@ -54,11 +56,11 @@ __device__ __forceinline__ float clamp_glsl(float x, float lo, float hi) {
 }
 
 // GetHash, counting.glsl:53-57: FP32 dot of the clamped integer cell.
-__device__ __forceinline__ uint32_t cell_hash(const SolverConsts &c, float x, float y, float z) {
-    int32_t ix = (int32_t)clamp_glsl(x, 0.0f, (float)c.gx);
-    int32_t iy = (int32_t)clamp_glsl(y, 0.0f, (float)c.gy);
-    int32_t iz = (int32_t)clamp_glsl(z, 0.0f, (float)c.gz);
-    const float w1 = (float)(c.gx * c.gz), w2 = (float)c.gx;
+__device__ __forceinline__ uint32_t cell_hash(const Seg &s, float x, float y, float z) {
+    int32_t ix = (int32_t)clamp_glsl(x, 0.0f, (float)s.gx);
+    int32_t iy = (int32_t)clamp_glsl(y, 0.0f, (float)s.gy);
+    int32_t iz = (int32_t)clamp_glsl(z, 0.0f, (float)s.gz);
+    const float w1 = (float)(s.gx * s.gz), w2 = (float)s.gx;
     float d = (float)ix * 1.0f + (float)iy * w1 + (float)iz * w2;
     return (uint32_t)d;
 }
@@ -76,35 +78,52 @@ __device__ __forceinline__ uint32_t xcd_block() {
 }
 
 // ----------------------------------------------------------------- predict
-// predictpos.glsl:18-38 fused with the key of counting.glsl:70.
-__global__ __launch_bounds__(BLOCK) void k_predict(SolverConsts c, const float4 *__restrict__ pos,
-                                                   const float4 *__restrict__ vel, float4 *__restrict__ rec,
-                                                   uint32_t *__restrict__ keys) {
-    const int32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= c.n) return;
+// predictpos.glsl:18-38 fused with the key of counting.glsl:70. Records carry
+// the scene-local id; keys put the scene number above the cell key, so one
+// stable sort orders every scene by its own key and keeps scenes apart.
+template <class Dom>
+__device__ __forceinline__ void predict_one(const SolverConsts &c, const Dom &dom, int32_t i,
+                                            const float4 *__restrict__ pos, const float4 *__restrict__ vel,
+                                            float4 *__restrict__ rec, uint32_t *__restrict__ keys) {
+    const Seg sg = dom.at(i);
     const float4 p = pos[i];
     const float4 v = vel[i];
     float vx = v.x, vy = v.y, vz = v.z;
-    if (c.extforce && p.z > (float)c.gz / 2.0f) {
+    if (c.extforce && p.z > (float)sg.gz / 2.0f) {
         const float g2 = 2.0f * c.gravity;
         vx = vx + (g2 * 0.0f) * c.dt;
         vy = vy + (g2 * 0.0f) * c.dt;
         vz = vz + (g2 * -1.0f) * c.dt;
     }
-    vx = vx + (c.gravity * c.gdir[0]) * c.dt;
-    vy = vy + (c.gravity * c.gdir[1]) * c.dt;
-    vz = vz + (c.gravity * c.gdir[2]) * c.dt;
+    vx = vx + (c.gravity * sg.gdir[0]) * c.dt;
+    vy = vy + (c.gravity * sg.gdir[1]) * c.dt;
+    vz = vz + (c.gravity * sg.gdir[2]) * c.dt;
     const float x = p.x + c.dt * vx, y = p.y + c.dt * vy, z = p.z + c.dt * vz;
-    rec[i] = make_float4(x, y, z, __int_as_float(i));
-    if (keys) keys[i] = cell_hash(c, x, y, z) & c.key_mask;
+    rec[i] = make_float4(x, y, z, __int_as_float(i - sg.off));
+    if (keys) keys[i] = sg.key_hi | (cell_hash(sg, x, y, z) & sg.key_mask);
 }
 
-__global__ __launch_bounds__(BLOCK) void k_keys(SolverConsts c, const float4 *__restrict__ rec,
+// Streaming kernels take any workgroup size and ppt particles per thread
+// (particle i = workgroup base + k * blockDim + thread, k < ppt).
+#define PBF_STREAM_FOR(i)                                                                     \
+    for (int32_t k_ = 0, i = (int32_t)(blockIdx.x * blockDim.x) * ppt + (int32_t)threadIdx.x; \
+         k_ < ppt && i < c.n; k_++, i += (int32_t)blockDim.x)
+
+template <class Dom>
+__global__ __launch_bounds__(1024) void k_predict(SolverConsts c, Dom dom, int32_t ppt, const float4 *__restrict__ pos,
+                                                  const float4 *__restrict__ vel, float4 *__restrict__ rec,
+                                                  uint32_t *__restrict__ keys) {
+    PBF_STREAM_FOR(i) predict_one(c, dom, i, pos, vel, rec, keys);
+}
+
+template <class Dom>
+__global__ __launch_bounds__(BLOCK) void k_keys(SolverConsts c, Dom dom, const float4 *__restrict__ rec,
                                                 uint32_t *__restrict__ keys) {
     const int32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= c.n) return;
+    const Seg sg = dom.at(i);
     const float4 r = rec[i];
-    keys[i] = cell_hash(c, r.x, r.y, r.z) & c.key_mask;
+    keys[i] = sg.key_hi | (cell_hash(sg, r.x, r.y, r.z) & sg.key_mask);
 }
 
 // globalsort.glsl:64 as a gather: out[g] = in[perm[g]] (bit-preserving).
@@ -133,52 +152,79 @@ __global__ __launch_bounds__(BLOCK) void k_fill_i32(int32_t *__restrict__ p, int
     }
 }
 
-// ----------------------------------------------------------------- cells
-__device__ __forceinline__ int64_t clamped_cell_index(const SolverConsts &c, const float4 &r) {
-    const int32_t x = (int32_t)clamp_glsl(r.x, 0.0f, (float)c.gx);
-    const int32_t y = (int32_t)clamp_glsl(r.y, 0.0f, (float)c.gy);
-    const int32_t z = (int32_t)clamp_glsl(r.z, 0.0f, (float)c.gz);
-    if (x >= c.gx || y >= c.gy || z >= c.gz) return -1;  // imageStore out of range: dropped
-    return (int64_t)x + (int64_t)c.gx * ((int64_t)y + (int64_t)c.gy * z);
+// The start table cleared to -1, and the tag of this step's end writes
+// advanced by one (read by the cell finder that follows on the stream).
+__global__ __launch_bounds__(BLOCK) void k_begin_cells(int32_t *__restrict__ start, size_t count,
+                                                       uint32_t *__restrict__ epoch) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) epoch[0] = epoch[0] + 1u;
+    const size_t stride = (size_t)gridDim.x * BLOCK * 4;
+    for (size_t i = ((size_t)blockIdx.x * BLOCK + threadIdx.x) * 4; i < count; i += stride) {
+        if (i + 4 <= count) {
+            *reinterpret_cast<int4 *>(start + i) = make_int4(-1, -1, -1, -1);
+        } else {
+            for (size_t k = i; k < count; k++) start[k] = -1;
+        }
+    }
 }
 
-__device__ __forceinline__ bool same_cell(const SolverConsts &c, const float4 &a, const float4 &b) {
-    return (int32_t)clamp_glsl(a.x, 0.0f, (float)c.gx) == (int32_t)clamp_glsl(b.x, 0.0f, (float)c.gx) &&
-           (int32_t)clamp_glsl(a.y, 0.0f, (float)c.gy) == (int32_t)clamp_glsl(b.y, 0.0f, (float)c.gy) &&
-           (int32_t)clamp_glsl(a.z, 0.0f, (float)c.gz) == (int32_t)clamp_glsl(b.z, 0.0f, (float)c.gz);
+// ----------------------------------------------------------------- cells
+__device__ __forceinline__ int64_t clamped_cell_index(const Seg &s, const float4 &r) {
+    const int32_t x = (int32_t)clamp_glsl(r.x, 0.0f, (float)s.gx);
+    const int32_t y = (int32_t)clamp_glsl(r.y, 0.0f, (float)s.gy);
+    const int32_t z = (int32_t)clamp_glsl(r.z, 0.0f, (float)s.gz);
+    if (x >= s.gx || y >= s.gy || z >= s.gz) return -1;  // imageStore out of range: dropped
+    return (int64_t)s.cell_base + (int64_t)x + (int64_t)s.gx * ((int64_t)y + (int64_t)s.gy * z);
+}
+
+__device__ __forceinline__ bool same_cell(const Seg &s, const float4 &a, const float4 &b) {
+    return (int32_t)clamp_glsl(a.x, 0.0f, (float)s.gx) == (int32_t)clamp_glsl(b.x, 0.0f, (float)s.gx) &&
+           (int32_t)clamp_glsl(a.y, 0.0f, (float)s.gy) == (int32_t)clamp_glsl(b.y, 0.0f, (float)s.gy) &&
+           (int32_t)clamp_glsl(a.z, 0.0f, (float)s.gz) == (int32_t)clamp_glsl(b.z, 0.0f, (float)s.gz);
 }
 
 // findcells.glsl:34-53. The reference writes in dispatch order, so with
 // colliding keys the highest gid is the last writer: atomicMax reproduces it
 // (start is cleared to -1 first; end carries the step tag in its high word).
-__global__ __launch_bounds__(BLOCK) void k_findcells(SolverConsts c, const float4 *__restrict__ sorted,
-                                                     int32_t *__restrict__ start,
-                                                     unsigned long long *__restrict__ end64) {
-    const int32_t g = blockIdx.x * BLOCK + threadIdx.x;
-    if (g >= c.n) return;
-    if (g == 0) {
-        atomicMax(&start[0], 0);  // imageStore (gridtexture, ivec3 (0, 0, 0), 0)
+// gid is the scene-local sorted index.
+template <class Dom>
+__device__ __forceinline__ void findcells_one(const SolverConsts &c, const Dom &dom, int32_t g,
+                                              const float4 *__restrict__ sorted, int32_t *__restrict__ start,
+                                              unsigned long long *__restrict__ end64, uint32_t ep) {
+    const Seg sg = dom.at(g);
+    const int32_t gl = g - sg.off;
+    if (gl == 0) {
+        atomicMax(&start[sg.cell_base], 0);  // imageStore (gridtexture, ivec3 (0, 0, 0), 0)
         return;
     }
     const float4 a = sorted[g], b = sorted[g - 1];
-    if (same_cell(c, a, b)) return;
-    const int64_t ca = clamped_cell_index(c, a), cb = clamped_cell_index(c, b);
-    if (ca >= 0) atomicMax(&start[ca], g);
-    if (cb >= 0) atomicMax(&end64[cb], ((unsigned long long)c.epoch << 32) | (unsigned long long)(uint32_t)g);
+    if (same_cell(sg, a, b)) return;
+    const int64_t ca = clamped_cell_index(sg, a), cb = clamped_cell_index(sg, b);
+    if (ca >= 0) atomicMax(&start[ca], gl);
+    if (cb >= 0) atomicMax(&end64[cb], ((unsigned long long)ep << 32) | (unsigned long long)(uint32_t)gl);
 }
 
-__device__ __forceinline__ bool in_grid(const SolverConsts &c, int32_t x, int32_t y, int32_t z) {
-    return x >= 0 && y >= 0 && z >= 0 && x < c.gx && y < c.gy && z < c.gz;
+template <class Dom>
+__global__ __launch_bounds__(1024) void k_findcells(SolverConsts c, Dom dom, int32_t ppt,
+                                                    const float4 *__restrict__ sorted, int32_t *__restrict__ start,
+                                                    unsigned long long *__restrict__ end64,
+                                                    const uint32_t *__restrict__ epoch) {
+    const uint32_t ep = epoch[0];
+    PBF_STREAM_FOR(g) findcells_one(c, dom, g, sorted, start, end64, ep);
+}
+
+__device__ __forceinline__ bool in_grid(const Seg &s, int32_t x, int32_t y, int32_t z) {
+    return x >= 0 && y >= 0 && z >= 0 && x < s.gx && y < s.gy && z < s.gz;
 }
 
 // neighbourcells.glsl:52-92: per particle, 9 (dy,dz) rows merging x-1..x+1;
 // texel fetches outside the grid read 0 (as llvmpipe does).
-__global__ __launch_bounds__(BLOCK) void k_neighbourcells(SolverConsts c, const float4 *__restrict__ sorted,
-                                                          const int32_t *__restrict__ start,
-                                                          const unsigned long long *__restrict__ end64,
-                                                          int32_t *__restrict__ nbr9) {
-    const int32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= c.n) return;
+template <class Dom>
+__device__ __forceinline__ void neighbourcells_one(const SolverConsts &c, const Dom &dom, int32_t i,
+                                                   const float4 *__restrict__ sorted,
+                                                   const int32_t *__restrict__ start,
+                                                   const unsigned long long *__restrict__ end64,
+                                                   int32_t *__restrict__ nbr9) {
+    const Seg sg = dom.at(i);
     const float4 p = sorted[i];
     const int32_t px = (int32_t)p.x, py = (int32_t)p.y, pz = (int32_t)p.z;  // ivec3 (pos): truncation
 #pragma unroll
@@ -188,8 +234,9 @@ __global__ __launch_bounds__(BLOCK) void k_neighbourcells(SolverConsts c, const 
 #pragma unroll
         for (int dx = -1; dx <= 1; dx++) {
             const int32_t x = px + dx;
-            const bool inb = in_grid(c, x, y, z);
-            const size_t idx = inb ? (size_t)x + (size_t)c.gx * ((size_t)y + (size_t)c.gy * (size_t)z) : 0;
+            const bool inb = in_grid(sg, x, y, z);
+            const size_t idx =
+                inb ? (size_t)sg.cell_base + (size_t)x + (size_t)sg.gx * ((size_t)y + (size_t)sg.gy * (size_t)z) : 0;
             const int32_t s = inb ? start[idx] : 0;
             if (cell == -1) cell = s;
             if (s != -1) {
@@ -201,11 +248,21 @@ __global__ __launch_bounds__(BLOCK) void k_neighbourcells(SolverConsts c, const 
     }
 }
 
+template <class Dom>
+__global__ __launch_bounds__(1024) void k_neighbourcells(SolverConsts c, Dom dom, int32_t ppt,
+                                                         const float4 *__restrict__ sorted,
+                                                         const int32_t *__restrict__ start,
+                                                         const unsigned long long *__restrict__ end64,
+                                                         int32_t *__restrict__ nbr9) {
+    PBF_STREAM_FOR(i) neighbourcells_one(c, dom, i, sorted, start, end64, nbr9);
+}
+
 // ----------------------------------------------------------------- solver
-// Neighbour records j of the packed ranges; indices past n (a wrapped count)
-// read a zero record, as the reference's robust buffer access does.
-__device__ __forceinline__ float4 rec_at(const float4 *__restrict__ rec, int32_t j, int32_t n) {
-    return j < n ? rec[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+// Neighbour records j (scene-local) of the packed ranges; indices past the
+// scene's n (a wrapped count) read a zero record, as the reference's robust
+// buffer access does.
+__device__ __forceinline__ float4 rec_at(const float4 *__restrict__ rec, const Seg &sg, int32_t j) {
+    return j < sg.n ? rec[sg.off + j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 // Neighbour iteration in two phases. FOR_EACH_NEIGHBOUR (foreachneighbour.glsl:
@@ -219,12 +276,12 @@ __device__ __forceinline__ float4 rec_at(const float4 *__restrict__ rec, int32_t
 // the list. A column that fills up is evaluated early and reused. Sums keep the
 // reference's order, and candidates with fl(sqrt(sq)) > h add exact zeros
 // there, so skipping them changes no bit of the result.
-constexpr int NB_BLOCK = 256;
 constexpr int LCAP = 48;
 
+template <int NB>
 struct NbrLds {
-    uint16_t list[(LCAP + 1) * NB_BLOCK];  // row LCAP: sink of the branch-free appends
-    int32_t rowstart[9 * NB_BLOCK];
+    uint16_t list[(LCAP + 1) * NB];  // row LCAP: sink of the branch-free appends
+    int32_t rowstart[9 * NB];
 };
 
 // True when the candidate may satisfy sq <= h2cut: the fused multiply-add
@@ -234,39 +291,42 @@ __device__ __forceinline__ bool maybe_near(const SolverConsts &c, const float4 &
     return fmaf(dx, dx, fmaf(dy, dy, dz * dz)) <= c.h2loose;
 }
 
-// Full candidate scan; body(j, code) for every candidate that may lie within h.
-template <class Body>
-__device__ __forceinline__ void scan_neighbours(const SolverConsts &c, const float4 *__restrict__ rec,
+// Full candidate scan of global slot i (scene-local index li); body(j, code)
+// for every candidate j (scene-local) that may lie within h.
+template <int NB, class Body>
+__device__ __forceinline__ void scan_neighbours(const SolverConsts &c, const Seg &sg, const float4 *__restrict__ rec,
                                                 const int32_t *__restrict__ nbr9, int32_t i, const float4 &pi,
-                                                NbrLds &lds, Body &&body) {
+                                                NbrLds<NB> &lds, Body &&body) {
     const int t = threadIdx.x;
+    const int32_t li = i - sg.off;
+    const float4 *__restrict__ srec = rec + sg.off;
     int cnt = 0;
     auto flush = [&]() {
         for (int k = 0; k < cnt; k++) {
-            const uint32_t e = lds.list[k * NB_BLOCK + t];
-            body(lds.rowstart[(e >> 8) * NB_BLOCK + t] + (int32_t)(e & 255u), e);
+            const uint32_t e = lds.list[k * NB + t];
+            body(lds.rowstart[(e >> 8) * NB + t] + (int32_t)(e & 255u), e);
         }
         cnt = 0;
     };
     for (int o = 0; o < 9; o++) {
         const int32_t d = nbr9[(size_t)o * c.n + i];
         const int32_t s0 = d & 0xFFFFFF, e = d >> 24;
-        lds.rowstart[o * NB_BLOCK + t] = s0;
-        // Loads stay in bounds: indices past the row or past n repeat the last
-        // valid one, and candidates past n are the zero record.
-        const int32_t last = max(min(s0 + e, c.n) - 1, 0);
+        lds.rowstart[o * NB + t] = s0;
+        // Loads stay in bounds: indices past the row or past the scene repeat
+        // the last valid one, and candidates past the scene are the zero record.
+        const int32_t last = max(min(s0 + e, sg.n) - 1, 0);
         for (int32_t q = 0; q < e; q += 4) {
             if (cnt > LCAP - 4) flush();
             const int32_t j0 = s0 + q;
             float4 p[4];
 #pragma unroll
-            for (int u = 0; u < 4; u++) p[u] = rec[min(j0 + u, last)];
+            for (int u = 0; u < 4; u++) p[u] = srec[min(j0 + u, last)];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 const int32_t j = j0 + u;
-                const float4 pj = j < c.n ? p[u] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                const bool f = (q + u < e) & (j != i) & maybe_near(c, pi, pj);
-                lds.list[(f ? cnt : LCAP) * NB_BLOCK + t] = (uint16_t)((o << 8) | (q + u));
+                const float4 pj = j < sg.n ? p[u] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                const bool f = (q + u < e) & (j != li) & maybe_near(c, pi, pj);
+                lds.list[(f ? cnt : LCAP) * NB + t] = (uint16_t)((o << 8) | (q + u));
                 cnt += f;
             }
         }
@@ -300,22 +360,22 @@ struct PairListOut {
 
 // body(j, rec[j], extra(j), listed) for i's neighbours within h (listed: the
 // recording pass saw these exact positions, so the range tests are known to
-// pass) or for candidates that may be within h (scanned); extra(j) gathers the
-// kernel's per-neighbour operand so that, on the list path, the loads of four
-// entries are in flight together.
-template <class Extra, class Body>
-__device__ __forceinline__ void replay_neighbours(const SolverConsts &c, const float4 *__restrict__ rec,
+// pass) or for candidates that may be within h (scanned); j is scene-local;
+// extra(j) gathers the kernel's per-neighbour operand so that, on the list
+// path, the loads of four entries are in flight together.
+template <int NB, class Extra, class Body>
+__device__ __forceinline__ void replay_neighbours(const SolverConsts &c, const Seg &sg, const float4 *__restrict__ rec,
                                                   const int32_t *__restrict__ nbr9, const uint16_t *__restrict__ entries,
                                                   const uint16_t *__restrict__ count, int32_t i, const float4 &pi,
-                                                  NbrLds &lds, Extra &&extra, Body &&body) {
+                                                  NbrLds<NB> &lds, Extra &&extra, Body &&body) {
     const int t = threadIdx.x;
     const uint32_t cnt = entries ? count[i] : LIST_NONE;
     if (cnt == LIST_NONE) {
-        scan_neighbours(c, rec, nbr9, i, pi, lds,
-                        [&](int32_t j, uint32_t) { body(j, rec_at(rec, j, c.n), extra(j), false); });
+        scan_neighbours<NB>(c, sg, rec, nbr9, i, pi, lds,
+                        [&](int32_t j, uint32_t) { body(j, rec_at(rec, sg, j), extra(j), false); });
         return;
     }
-    for (int o = 0; o < 9; o++) lds.rowstart[o * NB_BLOCK + t] = nbr9[(size_t)o * c.n + i] & 0xFFFFFF;
+    for (int o = 0; o < 9; o++) lds.rowstart[o * NB + t] = nbr9[(size_t)o * c.n + i] & 0xFFFFFF;
     using X = decltype(extra(0));
     for (uint32_t k = 0; k < cnt; k += 4) {
         uint32_t e[4];
@@ -325,11 +385,11 @@ __device__ __forceinline__ void replay_neighbours(const SolverConsts &c, const f
 #pragma unroll
         for (int u = 0; u < 4; u++) e[u] = k + u < cnt ? entries[(size_t)(k + u) * c.n + i] : 0u;
 #pragma unroll
-        for (int u = 0; u < 4; u++) j[u] = lds.rowstart[(e[u] >> 8) * NB_BLOCK + t] + (int32_t)(e[u] & 255u);
+        for (int u = 0; u < 4; u++) j[u] = lds.rowstart[(e[u] >> 8) * NB + t] + (int32_t)(e[u] & 255u);
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             const bool v = k + u < cnt;
-            pj[u] = v ? rec_at(rec, j[u], c.n) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            pj[u] = v ? rec_at(rec, sg, j[u]) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             x[u] = v ? extra(j[u]) : X();
         }
 #pragma unroll
@@ -339,19 +399,20 @@ __device__ __forceinline__ void replay_neighbours(const SolverConsts &c, const f
 }
 
 // calclambda.glsl:66-103; records the neighbours within h for the position update.
-template <bool H1>
-__global__ __launch_bounds__(NB_BLOCK) void k_lambda(SolverConsts c, const float4 *__restrict__ rec,
+template <bool H1, int NB, class Dom>
+__global__ __launch_bounds__(NB) void k_lambda(SolverConsts c, Dom dom, const float4 *__restrict__ rec,
                                                      const int32_t *__restrict__ nbr9, float *__restrict__ lambda,
                                                      uint16_t *__restrict__ list_entries,
                                                      uint16_t *__restrict__ list_count, int32_t list_cap) {
-    __shared__ NbrLds lds;
-    const int32_t i = xcd_block() * NB_BLOCK + threadIdx.x;
+    __shared__ NbrLds<NB> lds;
+    const int32_t i = xcd_block() * NB + threadIdx.x;
     if (i >= c.n) return;
+    const Seg sg = dom.at(i);
     const float4 pi = rec[i];
     float rho = 0.0f, sk = 0.0f, gix = 0.0f, giy = 0.0f, giz = 0.0f;
     PairListOut out = {list_entries, list_count, list_cap, 0};
-    scan_neighbours(c, rec, nbr9, i, pi, lds, [&](int32_t j, uint32_t code) {
-        const float4 pj = rec_at(rec, j, c.n);
+    scan_neighbours<NB>(c, sg, rec, nbr9, i, pi, lds, [&](int32_t j, uint32_t code) {
+        const float4 pj = rec_at(rec, sg, j);
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const float sq = sq_len(dx, dy, dz);
         if (sq > c.h2cut) return;
@@ -378,21 +439,23 @@ __global__ __launch_bounds__(NB_BLOCK) void k_lambda(SolverConsts c, const float
 
 // updatepos.glsl:42-105, race-free: neighbours read from rec_in, result to
 // rec_out; replays the lambda pass's neighbour list (same positions).
-template <bool H1>
-__global__ __launch_bounds__(NB_BLOCK) void k_updatepos(SolverConsts c, const float4 *__restrict__ rec_in,
+template <bool H1, int NB, class Dom>
+__global__ __launch_bounds__(NB) void k_updatepos(SolverConsts c, Dom dom, const float4 *__restrict__ rec_in,
                                                         const float *__restrict__ lambda,
                                                         const int32_t *__restrict__ nbr9,
                                                         const uint16_t *__restrict__ list_entries,
                                                         const uint16_t *__restrict__ list_count,
                                                         float4 *__restrict__ rec_out) {
-    __shared__ NbrLds lds;
-    const int32_t i = xcd_block() * NB_BLOCK + threadIdx.x;
+    __shared__ NbrLds<NB> lds;
+    const int32_t i = xcd_block() * NB + threadIdx.x;
     if (i >= c.n) return;
+    const Seg sg = dom.at(i);
     const float4 pi = rec_in[i];
     const float li = lambda[i];
+    const float *__restrict__ slam = lambda + sg.off;
     float dpx = 0.0f, dpy = 0.0f, dpz = 0.0f;
-    replay_neighbours(c, rec_in, nbr9, list_entries, list_count, i, pi, lds,
-                      [&](int32_t j) { return j < c.n ? lambda[j] : 0.0f; },
+    replay_neighbours<NB>(c, sg, rec_in, nbr9, list_entries, list_count, i, pi, lds,
+                      [&](int32_t j) { return j < sg.n ? slam[j] : 0.0f; },
                       [&](int32_t, const float4 &pj, float lj, bool listed) {
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const float sq = sq_len(dx, dy, dz);
@@ -411,56 +474,59 @@ __global__ __launch_bounds__(NB_BLOCK) void k_updatepos(SolverConsts c, const fl
         dpz += f * gz;
     });
     float x = pi.x + c.inv_rho0 * dpx, y = pi.y + c.inv_rho0 * dpy, z = pi.z + c.inv_rho0 * dpz;
-    x = clamp_glsl(x, c.lo[0], c.hi[0]);
-    y = clamp_glsl(y, c.lo[1], c.hi[1]);
-    z = clamp_glsl(z, c.lo[2], c.hi[2]);
+    x = clamp_glsl(x, sg.lo[0], sg.hi[0]);
+    y = clamp_glsl(y, sg.lo[1], sg.hi[1]);
+    z = clamp_glsl(z, sg.lo[2], sg.hi[2]);
     rec_out[i] = make_float4(x, y, z, pi.w);
 }
 
 // update.glsl:16-28; also keeps the velocities in sorted order for the
 // vorticity passes.
-__global__ __launch_bounds__(BLOCK) void k_update(SolverConsts c, const float4 *__restrict__ rec,
-                                                  float4 *__restrict__ pos, float4 *__restrict__ vel,
-                                                  float4 *__restrict__ vel_sorted) {
-    const int32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= c.n) return;
-    const float4 r = rec[i];
-    const int32_t id = id_of(r);
-    const float4 old = pos[id];
-    const float4 v = make_float4((r.x - old.x) / c.dt, (r.y - old.y) / c.dt, (r.z - old.z) / c.dt, 0.0f);
-    pos[id] = make_float4(r.x, r.y, r.z, 0.0f);
-    vel[id] = v;
-    if (vel_sorted) vel_sorted[i] = v;
+template <class Dom>
+__global__ __launch_bounds__(1024) void k_update(SolverConsts c, Dom dom, int32_t ppt, const float4 *__restrict__ rec,
+                                                 float4 *__restrict__ pos, float4 *__restrict__ vel,
+                                                 float4 *__restrict__ vel_sorted) {
+    PBF_STREAM_FOR(i) {
+        const Seg sg = dom.at(i);
+        const float4 r = rec[i];
+        const int32_t id = sg.off + id_of(r);
+        const float4 old = pos[id];
+        const float4 v = make_float4((r.x - old.x) / c.dt, (r.y - old.y) / c.dt, (r.z - old.z) / c.dt, 0.0f);
+        pos[id] = make_float4(r.x, r.y, r.z, 0.0f);
+        vel[id] = v;
+        if (vel_sorted) vel_sorted[i] = v;
+    }
 }
 
 // vorticity.glsl:34-60 (first half): XSPH and omega from the velocities before
 // this pass; |omega| to the vorticity buffer (vorticities[], :60). Records the
 // neighbours within h for the second half.
-template <bool H1>
-__global__ __launch_bounds__(NB_BLOCK) void k_vorticity_a(SolverConsts c, const float4 *__restrict__ rec,
+template <bool H1, int NB, class Dom>
+__global__ __launch_bounds__(NB) void k_vorticity_a(SolverConsts c, Dom dom, const float4 *__restrict__ rec,
                                                           const int32_t *__restrict__ nbr9,
                                                           const float4 *__restrict__ vel_sorted,
                                                           const float4 *__restrict__ vel_id,
                                                           float4 *__restrict__ vxsph, float4 *__restrict__ omega,
                                                           float *__restrict__ vort, uint16_t *__restrict__ list_entries,
                                                           uint16_t *__restrict__ list_count, int32_t list_cap) {
-    __shared__ NbrLds lds;
-    const int32_t i = xcd_block() * NB_BLOCK + threadIdx.x;
+    __shared__ NbrLds<NB> lds;
+    const int32_t i = xcd_block() * NB + threadIdx.x;
     if (i >= c.n) return;
+    const Seg sg = dom.at(i);
     const float4 pi = rec[i];
     const float4 vi = vel_sorted[i];
     float sx = 0.0f, sy = 0.0f, sz = 0.0f, wx = 0.0f, wy = 0.0f, wz = 0.0f;
     PairListOut out = {list_entries, list_count, list_cap, 0};
-    scan_neighbours(c, rec, nbr9, i, pi, lds, [&](int32_t j, uint32_t code) {
-        const float4 pj = rec_at(rec, j, c.n);
+    scan_neighbours<NB>(c, sg, rec, nbr9, i, pi, lds, [&](int32_t j, uint32_t code) {
+        const float4 pj = rec_at(rec, sg, j);
         const float px = pi.x - pj.x, py = pi.y - pj.y, pz = pi.z - pj.z;
         const float sq = sq_len(px, py, pz);
         if (sq > c.h2cut) return;
         const float r = sqrt_rn(sq);
         if (r > c.h) return;
         out.add(i, c.n, code);
-        // a zero record past n has id 0: imageLoad (velocitytexture, 0)
-        const float4 vj = j < c.n ? vel_sorted[j] : vel_id[0];
+        // a zero record past the scene has id 0: imageLoad (velocitytexture, 0)
+        const float4 vj = j < sg.n ? vel_sorted[sg.off + j] : vel_id[sg.off];
         const float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
         const float w = wpoly6<H1>(c, sq, r);
         sx += ux * w;
@@ -474,15 +540,15 @@ __global__ __launch_bounds__(NB_BLOCK) void k_vorticity_a(SolverConsts c, const 
         wz = (wz + ux * gy) - uy * gx;
     });
     out.finish(i);
-    vxsph[i] = make_float4(vi.x + c.xsph_c * sx, vi.y + c.xsph_c * sy, vi.z + c.xsph_c * sz, 0.0f);
+    vxsph[i] = make_float4(vi.x + sg.xsph_c * sx, vi.y + sg.xsph_c * sy, vi.z + sg.xsph_c * sz, 0.0f);
     omega[i] = make_float4(wx, wy, wz, 0.0f);
     vort[i] = sqrtf(sq_len(wx, wy, wz));
 }
 
 // vorticity.glsl:65-85 (second half): confinement from this step's |omega|;
 // replays the first half's neighbour list.
-template <bool H1>
-__global__ __launch_bounds__(NB_BLOCK) void k_vorticity_b(SolverConsts c, const float4 *__restrict__ rec,
+template <bool H1, int NB, class Dom>
+__global__ __launch_bounds__(NB) void k_vorticity_b(SolverConsts c, Dom dom, const float4 *__restrict__ rec,
                                                           const int32_t *__restrict__ nbr9,
                                                           const uint16_t *__restrict__ list_entries,
                                                           const uint16_t *__restrict__ list_count,
@@ -490,13 +556,15 @@ __global__ __launch_bounds__(NB_BLOCK) void k_vorticity_b(SolverConsts c, const 
                                                           const float4 *__restrict__ omega,
                                                           const float *__restrict__ vort,
                                                           float4 *__restrict__ vel_id) {
-    __shared__ NbrLds lds;
-    const int32_t i = xcd_block() * NB_BLOCK + threadIdx.x;
+    __shared__ NbrLds<NB> lds;
+    const int32_t i = xcd_block() * NB + threadIdx.x;
     if (i >= c.n) return;
+    const Seg sg = dom.at(i);
     const float4 pi = rec[i];
+    const float *__restrict__ svort = vort + sg.off;
     float nx = 0.0f, ny = 0.0f, nz = 0.0f;
-    replay_neighbours(c, rec, nbr9, list_entries, list_count, i, pi, lds,
-                      [&](int32_t j) { return j < c.n ? vort[j] : 0.0f; },
+    replay_neighbours<NB>(c, sg, rec, nbr9, list_entries, list_count, i, pi, lds,
+                      [&](int32_t j) { return j < sg.n ? svort[j] : 0.0f; },
                       [&](int32_t, const float4 &pj, float om, bool listed) {
         const float px = pi.x - pj.x, py = pi.y - pj.y, pz = pi.z - pj.z;
         const float sq = sq_len(px, py, pz);
@@ -521,7 +589,7 @@ __global__ __launch_bounds__(NB_BLOCK) void k_vorticity_b(SolverConsts c, const 
     const float ox = v.x + sc * (ny * w.z - nz * w.y);
     const float oy = v.y + sc * (nz * w.x - nx * w.z);
     const float oz = v.z + sc * (nx * w.y - ny * w.x);
-    vel_id[id_of(pi)] = make_float4(ox, oy, oz, 0.0f);
+    vel_id[sg.off + id_of(pi)] = make_float4(ox, oy, oz, 0.0f);
 }
 
 // ----------------------------------------------------------------- layout conversions
@@ -561,7 +629,6 @@ __global__ __launch_bounds__(BLOCK) void k_i32_to_end64(size_t cells, const int3
 }
 
 inline unsigned blocks_for(size_t n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
-inline unsigned nb_blocks_for(size_t n) { return (unsigned)((n + NB_BLOCK - 1) / NB_BLOCK); }
 inline unsigned grid_stride_blocks(size_t n) {
     size_t b = (n + BLOCK - 1) / BLOCK;
     return (unsigned)(b > 65536 ? 65536 : (b == 0 ? 1 : b));
@@ -570,12 +637,142 @@ inline unsigned grid_stride_blocks(size_t n) {
 }  // namespace
 
 // ----------------------------------------------------------------- launchers
+Seg seg_of(const SolverConsts &c) {
+    Seg s;
+    s.off = 0;
+    s.n = c.n;
+    s.gx = c.gx;
+    s.gy = c.gy;
+    s.gz = c.gz;
+    s.cell_base = 0;
+    s.key_mask = c.key_mask;
+    s.key_hi = 0;
+    for (int a = 0; a < 3; a++) {
+        s.lo[a] = c.lo[a];
+        s.hi[a] = c.hi[a];
+        s.gdir[a] = c.gdir[a];
+    }
+    s.xsph_c = c.xsph_c;
+    return s;
+}
+
+bool valid_shape(const LaunchShape &ls) {
+    const bool nb = ls.nb == 128 || ls.nb == 256 || ls.nb == 512;
+    const bool sb = ls.sb >= 64 && ls.sb <= 1024 && (ls.sb & (ls.sb - 1)) == 0;
+    return nb && sb && ls.ppt >= 1 && ls.ppt <= 4;
+}
+
+namespace {
+DomOne one(const SolverConsts &c) { return DomOne{seg_of(c)}; }
+
+const LaunchShape kDefaultShape;
+
+unsigned stream_blocks(int32_t n, const LaunchShape &ls) {
+    const int64_t per = (int64_t)ls.sb * ls.ppt;
+    return (unsigned)((n + per - 1) / per);
+}
+unsigned nb_blocks(int32_t n, int nb) { return (unsigned)((n + nb - 1) / nb); }
+
+// Calls f with the neighbour workgroup size as a compile-time constant.
+template <class F>
+void with_nb(int nb, F &&f) {
+    switch (nb) {
+        case 128: f(std::integral_constant<int, 128>()); break;
+        case 512: f(std::integral_constant<int, 512>()); break;
+        default: f(std::integral_constant<int, 256>()); break;
+    }
+}
+
+template <class Dom>
+void predict_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, const float4 *pos, const float4 *vel,
+                  float4 *rec, uint32_t *keys, hipStream_t st) {
+    if (c.n > 0) k_predict<Dom><<<stream_blocks(c.n, ls), ls.sb, 0, st>>>(c, d, ls.ppt, pos, vel, rec, keys);
+}
+template <class Dom>
+void findcells_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, const float4 *sorted, int32_t *start,
+                    unsigned long long *end64, const uint32_t *epoch, hipStream_t st) {
+    if (c.n > 0)
+        k_findcells<Dom><<<stream_blocks(c.n, ls), ls.sb, 0, st>>>(c, d, ls.ppt, sorted, start, end64, epoch);
+}
+template <class Dom>
+void neighbourcells_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, const float4 *sorted,
+                         const int32_t *start, const unsigned long long *end64, int32_t *nbr9, hipStream_t st) {
+    if (c.n > 0)
+        k_neighbourcells<Dom><<<stream_blocks(c.n, ls), ls.sb, 0, st>>>(c, d, ls.ppt, sorted, start, end64, nbr9);
+}
+template <class Dom>
+void update_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, const float4 *rec, float4 *pos,
+                 float4 *vel, float4 *vel_sorted, hipStream_t st) {
+    if (c.n > 0) k_update<Dom><<<stream_blocks(c.n, ls), ls.sb, 0, st>>>(c, d, ls.ppt, rec, pos, vel, vel_sorted);
+}
+template <class Dom>
+void lambda_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, const float4 *rec, const int32_t *nbr9,
+                 float *lambda, const PairList &pl, hipStream_t st) {
+    if (c.n <= 0) return;
+    with_nb(ls.nb, [&](auto nbc) {
+        constexpr int NB = decltype(nbc)::value;
+        if (c.h_is_one)
+            k_lambda<true, NB, Dom><<<nb_blocks(c.n, NB), NB, 0, st>>>(c, d, rec, nbr9, lambda, pl.entries, pl.count,
+                                                                       pl.cap);
+        else
+            k_lambda<false, NB, Dom><<<nb_blocks(c.n, NB), NB, 0, st>>>(c, d, rec, nbr9, lambda, pl.entries,
+                                                                        pl.count, pl.cap);
+    });
+}
+template <class Dom>
+void updatepos_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, const float4 *rec_in,
+                    const float *lambda, const int32_t *nbr9, const PairList &pl, float4 *rec_out, hipStream_t st) {
+    if (c.n <= 0) return;
+    with_nb(ls.nb, [&](auto nbc) {
+        constexpr int NB = decltype(nbc)::value;
+        if (c.h_is_one)
+            k_updatepos<true, NB, Dom><<<nb_blocks(c.n, NB), NB, 0, st>>>(c, d, rec_in, lambda, nbr9, pl.entries,
+                                                                          pl.count, rec_out);
+        else
+            k_updatepos<false, NB, Dom><<<nb_blocks(c.n, NB), NB, 0, st>>>(c, d, rec_in, lambda, nbr9, pl.entries,
+                                                                           pl.count, rec_out);
+    });
+}
+template <class Dom>
+void vorticity_a_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, const float4 *rec,
+                      const int32_t *nbr9, const float4 *vel_sorted, const float4 *vel_id, float4 *vxsph,
+                      float4 *omega, float *vort, const PairList &pl, hipStream_t st) {
+    if (c.n <= 0) return;
+    with_nb(ls.nb, [&](auto nbc) {
+        constexpr int NB = decltype(nbc)::value;
+        if (c.h_is_one)
+            k_vorticity_a<true, NB, Dom><<<nb_blocks(c.n, NB), NB, 0, st>>>(c, d, rec, nbr9, vel_sorted, vel_id, vxsph,
+                                                                            omega, vort, pl.entries, pl.count, pl.cap);
+        else
+            k_vorticity_a<false, NB, Dom><<<nb_blocks(c.n, NB), NB, 0, st>>>(c, d, rec, nbr9, vel_sorted, vel_id,
+                                                                             vxsph, omega, vort, pl.entries, pl.count,
+                                                                             pl.cap);
+    });
+}
+template <class Dom>
+void vorticity_b_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, const float4 *rec,
+                      const int32_t *nbr9, const PairList &pl, const float4 *vxsph, const float4 *omega,
+                      const float *vort, float4 *vel_id, hipStream_t st) {
+    if (c.n <= 0) return;
+    with_nb(ls.nb, [&](auto nbc) {
+        constexpr int NB = decltype(nbc)::value;
+        if (c.h_is_one)
+            k_vorticity_b<true, NB, Dom><<<nb_blocks(c.n, NB), NB, 0, st>>>(c, d, rec, nbr9, pl.entries, pl.count,
+                                                                            vxsph, omega, vort, vel_id);
+        else
+            k_vorticity_b<false, NB, Dom><<<nb_blocks(c.n, NB), NB, 0, st>>>(c, d, rec, nbr9, pl.entries, pl.count,
+                                                                             vxsph, omega, vort, vel_id);
+    });
+}
+}  // namespace
+
+// one scene
 void launch_predict(const SolverConsts &c, const float4 *pos, const float4 *vel, float4 *rec, uint32_t *keys,
                     hipStream_t st) {
-    if (c.n > 0) k_predict<<<blocks_for(c.n), BLOCK, 0, st>>>(c, pos, vel, rec, keys);
+    predict_impl(c, one(c), kDefaultShape, pos, vel, rec, keys, st);
 }
 void launch_keys_of_records(const SolverConsts &c, const float4 *rec, uint32_t *keys, hipStream_t st) {
-    if (c.n > 0) k_keys<<<blocks_for(c.n), BLOCK, 0, st>>>(c, rec, keys);
+    if (c.n > 0) k_keys<DomOne><<<blocks_for(c.n), BLOCK, 0, st>>>(c, one(c), rec, keys);
 }
 void launch_gather_records(int32_t n, const uint32_t *perm, const float4 *in, float4 *out, hipStream_t st) {
     if (n > 0) k_gather<<<blocks_for(n), BLOCK, 0, st>>>(n, perm, in, out);
@@ -586,56 +783,74 @@ void launch_gather_by_id(int32_t n, const float4 *rec, const float4 *src, float4
 void launch_fill_i32(int32_t *p, int32_t value, size_t count, hipStream_t st) {
     if (count > 0) k_fill_i32<<<grid_stride_blocks((count + 3) / 4), BLOCK, 0, st>>>(p, value, count);
 }
+void launch_begin_cells(int32_t *start, size_t cells, uint32_t *epoch, hipStream_t st) {
+    k_begin_cells<<<grid_stride_blocks((cells + 3) / 4), BLOCK, 0, st>>>(start, cells, epoch);
+}
 void launch_findcells(const SolverConsts &c, const float4 *sorted, int32_t *start, unsigned long long *end64,
-                      hipStream_t st) {
-    if (c.n > 0) k_findcells<<<blocks_for(c.n), BLOCK, 0, st>>>(c, sorted, start, end64);
+                      const uint32_t *epoch, hipStream_t st) {
+    findcells_impl(c, one(c), kDefaultShape, sorted, start, end64, epoch, st);
 }
 void launch_neighbourcells(const SolverConsts &c, const float4 *sorted, const int32_t *start,
                            const unsigned long long *end64, int32_t *nbr9, hipStream_t st) {
-    if (c.n > 0) k_neighbourcells<<<blocks_for(c.n), BLOCK, 0, st>>>(c, sorted, start, end64, nbr9);
+    neighbourcells_impl(c, one(c), kDefaultShape, sorted, start, end64, nbr9, st);
 }
 void launch_lambda(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, float *lambda, const PairList &pl,
                    hipStream_t st) {
-    if (c.n <= 0) return;
-    if (c.h_is_one)
-        k_lambda<true><<<nb_blocks_for(c.n), NB_BLOCK, 0, st>>>(c, rec, nbr9, lambda, pl.entries, pl.count, pl.cap);
-    else
-        k_lambda<false><<<nb_blocks_for(c.n), NB_BLOCK, 0, st>>>(c, rec, nbr9, lambda, pl.entries, pl.count, pl.cap);
+    lambda_impl(c, one(c), kDefaultShape, rec, nbr9, lambda, pl, st);
 }
 void launch_updatepos(const SolverConsts &c, const float4 *rec_in, const float *lambda, const int32_t *nbr9,
                       const PairList &pl, float4 *rec_out, hipStream_t st) {
-    if (c.n <= 0) return;
-    if (c.h_is_one)
-        k_updatepos<true><<<nb_blocks_for(c.n), NB_BLOCK, 0, st>>>(c, rec_in, lambda, nbr9, pl.entries, pl.count,
-                                                                   rec_out);
-    else
-        k_updatepos<false><<<nb_blocks_for(c.n), NB_BLOCK, 0, st>>>(c, rec_in, lambda, nbr9, pl.entries, pl.count,
-                                                                    rec_out);
+    updatepos_impl(c, one(c), kDefaultShape, rec_in, lambda, nbr9, pl, rec_out, st);
 }
 void launch_update(const SolverConsts &c, const float4 *rec, float4 *pos, float4 *vel, float4 *vel_sorted,
                    hipStream_t st) {
-    if (c.n > 0) k_update<<<blocks_for(c.n), BLOCK, 0, st>>>(c, rec, pos, vel, vel_sorted);
+    update_impl(c, one(c), kDefaultShape, rec, pos, vel, vel_sorted, st);
 }
 void launch_vorticity_a(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, const float4 *vel_sorted,
                         const float4 *vel_id, float4 *vxsph, float4 *omega, float *vort, const PairList &pl,
                         hipStream_t st) {
-    if (c.n <= 0) return;
-    if (c.h_is_one)
-        k_vorticity_a<true><<<nb_blocks_for(c.n), NB_BLOCK, 0, st>>>(c, rec, nbr9, vel_sorted, vel_id, vxsph, omega,
-                                                                     vort, pl.entries, pl.count, pl.cap);
-    else
-        k_vorticity_a<false><<<nb_blocks_for(c.n), NB_BLOCK, 0, st>>>(c, rec, nbr9, vel_sorted, vel_id, vxsph, omega,
-                                                                      vort, pl.entries, pl.count, pl.cap);
+    vorticity_a_impl(c, one(c), kDefaultShape, rec, nbr9, vel_sorted, vel_id, vxsph, omega, vort, pl, st);
 }
 void launch_vorticity_b(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, const PairList &pl,
                         const float4 *vxsph, const float4 *omega, const float *vort, float4 *vel_id, hipStream_t st) {
-    if (c.n <= 0) return;
-    if (c.h_is_one)
-        k_vorticity_b<true><<<nb_blocks_for(c.n), NB_BLOCK, 0, st>>>(c, rec, nbr9, pl.entries, pl.count, vxsph,
-                                                                     omega, vort, vel_id);
-    else
-        k_vorticity_b<false><<<nb_blocks_for(c.n), NB_BLOCK, 0, st>>>(c, rec, nbr9, pl.entries, pl.count, vxsph,
-                                                                      omega, vort, vel_id);
+    vorticity_b_impl(c, one(c), kDefaultShape, rec, nbr9, pl, vxsph, omega, vort, vel_id, st);
+}
+
+// many scenes
+void launch_predict(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *pos,
+                    const float4 *vel, float4 *rec, uint32_t *keys, hipStream_t st) {
+    predict_impl(c, d, ls, pos, vel, rec, keys, st);
+}
+void launch_findcells(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *sorted,
+                      int32_t *start, unsigned long long *end64, const uint32_t *epoch, hipStream_t st) {
+    findcells_impl(c, d, ls, sorted, start, end64, epoch, st);
+}
+void launch_neighbourcells(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *sorted,
+                           const int32_t *start, const unsigned long long *end64, int32_t *nbr9, hipStream_t st) {
+    neighbourcells_impl(c, d, ls, sorted, start, end64, nbr9, st);
+}
+void launch_lambda(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec,
+                   const int32_t *nbr9, float *lambda, const PairList &pl, hipStream_t st) {
+    lambda_impl(c, d, ls, rec, nbr9, lambda, pl, st);
+}
+void launch_updatepos(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec_in,
+                      const float *lambda, const int32_t *nbr9, const PairList &pl, float4 *rec_out,
+                      hipStream_t st) {
+    updatepos_impl(c, d, ls, rec_in, lambda, nbr9, pl, rec_out, st);
+}
+void launch_update(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec, float4 *pos,
+                   float4 *vel, float4 *vel_sorted, hipStream_t st) {
+    update_impl(c, d, ls, rec, pos, vel, vel_sorted, st);
+}
+void launch_vorticity_a(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec,
+                        const int32_t *nbr9, const float4 *vel_sorted, const float4 *vel_id, float4 *vxsph,
+                        float4 *omega, float *vort, const PairList &pl, hipStream_t st) {
+    vorticity_a_impl(c, d, ls, rec, nbr9, vel_sorted, vel_id, vxsph, omega, vort, pl, st);
+}
+void launch_vorticity_b(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec,
+                        const int32_t *nbr9, const PairList &pl, const float4 *vxsph, const float4 *omega,
+                        const float *vort, float4 *vel_id, hipStream_t st) {
+    vorticity_b_impl(c, d, ls, rec, nbr9, pl, vxsph, omega, vort, vel_id, st);
 }
 void launch_nbr9_to_12(int32_t n, const int32_t *nbr9, int32_t *nbr12, hipStream_t st) {
     if (n > 0) k_nbr9_to_12<<<blocks_for(n), BLOCK, 0, st>>>(n, nbr9, nbr12);

@@ -195,6 +195,43 @@ def cfg_adjoint_16k(seed=4, dt=0.016) -> Scene:
     return Scene("adjoint_16k", (64, 64, 64), config_params(1.0, dt, 3), pos, vel, 32, {"seed": seed})
 
 
+def batched_scenes(nscenes: int = 256, seed: int = 3, n_lo: int = 2048, n_hi: int = 65536, dt: float = 0.016,
+                   iterations: int = 3, steps: int = 100) -> list:
+    """Independent scenes of the batched form. Per scene: n log-uniform in
+    [n_lo, n_hi], rounded to a multiple of 512 (the plain solver's dispatch
+    size); a jittered 0.5h lattice block of ceil(cbrt n) x ceil(n / ceil(cbrt n)^2)
+    x ceil(cbrt n) sites (the first n in id order), centred in x and z on the
+    floor of a cubic box of side ceil(cbrt(n) 0.5h) 2 cells; walls one cell in
+    from the x and z faces; gravity (0, -1, 0) turned by N(0, 0.1) rad about x
+    and about z; XSPH c uniform in [0.005, 0.05]. Other constants as the
+    dam-break workloads, 3 iterations, dt 0.016."""
+    u = uniform(seed, 4 * nscenes).reshape(nscenes, 4)
+    g = normal(seed + 1, 2 * nscenes).reshape(nscenes, 2) * 0.1
+    out = []
+    for s in range(nscenes):
+        n = math.exp(math.log(n_lo) + u[s, 0] * (math.log(n_hi) - math.log(n_lo)))
+        n = int(min(max(round(n / 512.0) * 512, n_lo), n_hi))
+        side = int(math.ceil(n ** (1.0 / 3.0) * 0.5)) * 2
+        k = int(math.ceil(n ** (1.0 / 3.0) - 1e-9))
+        ky = int(math.ceil(n / float(k * k)))
+        corner = ((side - k * 0.5) / 2.0 + 0.25, 0.25, (side - k * 0.5) / 2.0 + 0.25)
+        pos = lattice_block((k, ky, k), 0.5, corner, 0.01, seed + 10 + s)
+        # first n sites in id order (x outer, z, y fastest) keeps whole columns
+        pos = pos[:n]
+        p = config_params(1.0, dt, iterations)
+        ax, az = float(g[s, 0]), float(g[s, 1])
+        # (0, -1, 0) turned by az about z, then by ax about x
+        gx, gy = math.sin(az), -math.cos(az)
+        gdir = (gx, gy * math.cos(ax), gy * math.sin(ax))
+        p.gravity_dir = tuple(float(np.float32(x)) for x in gdir)
+        p.xsph_viscosity_c = float(np.float32(0.005 + u[s, 1] * 0.045))
+        p.wall_lo = (1.0, 0.0, 1.0)
+        p.wall_hi_margin = (1.0, 0.0, 1.0)
+        out.append(Scene(f"batch{s}", (side, side, side), p, pos, np.zeros_like(pos), steps,
+                         {"seed": seed, "index": s}))
+    return out
+
+
 def count_sortbits(v: int) -> int:
     """``count_sortbits`` of src/RadixSort.cpp:24-30: bit length of v, at least 1."""
     r = 1

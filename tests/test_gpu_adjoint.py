@@ -1,0 +1,188 @@
+"""Adjoint (differentiable) form on the GPU, through the C ABI.
+
+The forward of the FP32 form equals the plain SPH step bit for bit; the FP64
+form's step equals the oracle's FP64 replay with the same fixed structure; the
+FP64 adjoint agrees with central finite differences of the FP64 replay (the
+map it differentiates), for initial velocities, XSPH c and the vorticity
+epsilon; the FP32 adjoint is close to the FP64 one; the torch.autograd path
+returns the C ABI's gradients.
+
+The adjoint workload uses the reference's default relaxation epsilon 5
+(src/SPH.cpp:138): with 0.01 the 16,384-particle block explodes within a few
+steps, and central differences of its 32-step loss do not converge at any step
+size (the map is not usefully differentiable there)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from pbf_amd import adjoint as A
+from pbf_amd import scene as S
+from pbf_amd import sph as P
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def adjoint_scene(seed=4):
+    sc = S.cfg_adjoint_16k(seed=seed)
+    sc.params.epsilon = 5.0
+    return sc
+
+
+def mse_grad(x, t):
+    g = np.zeros_like(x)
+    g[:, :3] = 2.0 * (x[:, :3] - t[:, :3]) / (3 * x.shape[0])
+    return g
+
+
+def mse(x, t):
+    return float(np.mean((x[:, :3].astype(np.float64) - t[:, :3]) ** 2))
+
+
+def test_fp32_forward_equals_plain_step(gpu_lib):
+    sc = adjoint_scene()
+    steps = 8
+    say(f"[adjoint fp32 forward] n={sc.n} steps={steps}")
+    r = A.Rollout.for_scene(sc, steps=steps, checkpoint_every=4)
+    x = r.forward(sc.pos, sc.vel)
+    v = r.velocities()
+    sph = P.SPH(sc.n, sc.grid)
+    sph.set_params(P.scene_params(sc))
+    sph.upload(sc.pos, sc.vel)
+    for _ in range(steps):
+        sph.Run()
+    wp, wv = sph.download()
+    assert np.array_equal(x[:, :3].view(np.int32), wp[:, :3].view(np.int32))
+    assert np.array_equal(v[:, :3].view(np.int32), wv[:, :3].view(np.int32))
+    sph.close()
+    r.close()
+
+
+def test_fp64_step_matches_oracle_replay(gpu_lib):
+    sc = adjoint_scene()
+    say(f"[adjoint fp64 step] n={sc.n}")
+    r = A.Rollout.for_scene(sc, steps=1, checkpoint_every=1, fp64=True)
+    x = r.forward(sc.pos, sc.vel, keep_structure=True)
+    cfg = O.cfg_for(sc)
+    _, _, _, _, st = O.step(cfg, sc.pos, sc.vel, stages=True)
+    perm = st["sorted"][:, 3].view(np.int32).copy()
+    wx, wv = O.step_fixed_f64(cfg, sc.pos, sc.vel, perm, st["nbr"])
+    assert np.abs(x[:, :3] - wx[:, :3]).max() < 1e-10
+    assert np.abs(r.velocities()[:, :3] - wv[:, :3]).max() < 1e-8
+    r.close()
+
+
+def fd_check(sc, steps, every, ncomp, delta, delta_coef, seed=0):
+    """Adjoint of the FP64 MSE loss against a target rollout vs central differences
+    (step delta for velocity components, delta_coef for the two coefficients)."""
+    tgt_v = adjoint_scene(seed=40).vel
+    r = A.Rollout.for_scene(sc, steps=steps, checkpoint_every=every, fp64=True)
+    c0, e0 = sc.params.xsph_viscosity_c, sc.params.vorticity_epsilon
+    t = r.forward(sc.pos, tgt_v)
+    x = r.forward(sc.pos, sc.vel, keep_structure=True)
+    gv, gc, ge = r.backward(mse_grad(x, t))
+    rng = np.random.default_rng(seed)
+    comps = rng.choice(sc.n * 3, ncomp, replace=False)
+    fd = np.empty(ncomp)
+    v64 = sc.vel.astype(np.float64)
+    for k, cidx in enumerate(comps):
+        i, a = divmod(int(cidx), 3)
+        vp, vm = v64.copy(), v64.copy()
+        vp[i, a] += delta
+        vm[i, a] -= delta
+        fd[k] = (mse(r.replay(sc.pos, vp), t) - mse(r.replay(sc.pos, vm), t)) / (2 * delta)
+    ad = np.array([gv[int(c) // 3, int(c) % 3] for c in comps])
+    fdc = []
+    d = delta_coef
+    for cc, ee in [(c0 + d, e0), (c0 - d, e0), (c0, e0 + d), (c0, e0 - d)]:
+        r.set_coefficients(cc, ee)
+        fdc.append(mse(r.replay(sc.pos, v64), t))
+    r.set_coefficients(c0, e0)
+    r.close()
+    return ad, fd, (gc, (fdc[0] - fdc[1]) / (2 * d)), (ge, (fdc[2] - fdc[3]) / (2 * d))
+
+
+def rel(a, b, floor):
+    return np.abs(a - b) / np.maximum(np.abs(b), floor)
+
+
+def test_adjoint_matches_finite_differences_short(gpu_lib):
+    """4 steps: central differences with step 1e-4 (1e-6 for the coefficients)
+    are converged to about 1e-8 here (smaller steps only add cancellation
+    noise: the per-component gradient of a mean over 49,152 terms is ~1e-8)."""
+    sc = adjoint_scene()
+    ad, fd, (gc, fc), (ge, fe) = fd_check(sc, steps=4, every=2, ncomp=32, delta=1e-4, delta_coef=1e-6)
+    floor = 1e-3 * np.abs(fd).max()
+    err = rel(ad, fd, floor)
+    say(f"[adjoint fd 4 steps] max rel {err.max():.3e} median {np.median(err):.3e}; "
+        f"c {gc:.9e} vs {fc:.9e}; eps {ge:.9e} vs {fe:.9e}")
+    assert err.max() < 1e-5
+    assert abs(gc - fc) <= 1e-5 * abs(fc)
+    assert abs(ge - fe) <= 1e-5 * abs(fe)
+
+
+def test_adjoint_matches_finite_differences_32_steps(gpu_lib):
+    """The workload: 32 steps x 3 iterations, checkpoints every 8 steps, 64
+    random velocity components, central differences with step 1e-4. Over 32
+    steps the central differences themselves move by 1e-5 to 1e-3 between steps
+    1e-4 and 1e-5 (the map amplifies perturbations), so agreement is asserted
+    for the median and for most components. The loss is far more nonlinear in
+    the two coefficients: their central differences are meaningless at 1e-4
+    (c: 3.2e-3 against an adjoint of 329) and settle only below a step of 1e-8,
+    where they still scatter by about 15 % from step to step (c: 377 at 1e-8,
+    360 at 1e-9, 332 at 1e-10); the adjoint is asserted within that scatter.
+    The coefficient derivatives are pinned to 1e-5 by the 4-step check."""
+    sc = adjoint_scene()
+    ad, fd, (gc, fc), (ge, fe) = fd_check(sc, steps=32, every=8, ncomp=64, delta=1e-4, delta_coef=1e-9)
+    floor = 1e-3 * np.abs(fd).max()
+    err = rel(ad, fd, floor)
+    say(f"[adjoint fd 32 steps] max rel {err.max():.3e} median {np.median(err):.3e} "
+        f"within 1e-4: {(err <= 1e-4).sum()}/{err.size}; c {gc:.9e} vs {fc:.9e}; eps {ge:.9e} vs {fe:.9e}")
+    assert np.median(err) < 1e-4
+    assert (err <= 1e-4).mean() >= 0.75
+    assert abs(gc - fc) <= 0.2 * abs(fc)
+    assert abs(ge - fe) <= 0.2 * abs(fe)
+
+
+def test_fp32_adjoint_close_to_fp64(gpu_lib):
+    sc = adjoint_scene()
+    steps = 4
+    out = []
+    for fp64 in (False, True):
+        r = A.Rollout.for_scene(sc, steps=steps, checkpoint_every=2, fp64=fp64)
+        t = r.forward(sc.pos, adjoint_scene(seed=40).vel)
+        x = r.forward(sc.pos, sc.vel)
+        out.append(r.backward(mse_grad(x.astype(np.float64), t.astype(np.float64))))
+        r.close()
+    (g32, c32, e32), (g64, c64, e64) = out
+    scale = np.abs(g64[:, :3]).max()
+    err = np.abs(g32[:, :3] - g64[:, :3]).max() / scale
+    say(f"[adjoint fp32 vs fp64] max err / max |g| {err:.3e}; c {c32:.6e} vs {c64:.6e}")
+    assert err < 5e-3
+    assert abs(c32 - c64) <= 5e-3 * abs(c64)
+
+
+def test_torch_autograd_function(gpu_lib):
+    import torch
+    sc = adjoint_scene()
+    steps = 4
+    r = A.Rollout.for_scene(sc, steps=steps, checkpoint_every=2)
+    dev = torch.device("cuda:0")
+    pos0 = torch.tensor(sc.pos[:, :3], device=dev)
+    target = torch.tensor(r.forward(sc.pos, adjoint_scene(seed=40).vel)[:, :3], device=dev)
+    v0 = torch.tensor(sc.vel[:, :3], device=dev, requires_grad=True)
+    c = torch.tensor(sc.params.xsph_viscosity_c, dtype=torch.float64, requires_grad=True)
+    e = torch.tensor(sc.params.vorticity_epsilon, dtype=torch.float64, requires_grad=True)
+    x = A.rollout_positions(v0, c, e, r, pos0)
+    loss = torch.mean((x - target) ** 2)
+    loss.backward()
+    torch.cuda.synchronize()
+    xr = r.forward(sc.pos, sc.vel)
+    assert np.array_equal(x.detach().cpu().numpy(), xr[:, :3])
+    gv, gc, ge = r.backward(mse_grad(xr, np.concatenate([target.cpu().numpy(), np.zeros((sc.n, 1), np.float32)], 1)))
+    assert np.allclose(v0.grad.cpu().numpy(), gv[:, :3], rtol=1e-5, atol=1e-12)
+    assert abs(c.grad.item() - gc) <= 1e-4 * abs(gc) and abs(e.grad.item() - ge) <= 1e-4 * max(abs(ge), 1e-20)
+    r.close()

@@ -1,0 +1,122 @@
+"""Batched form on the GPU, through the C ABI.
+
+Bar: every scene of a batch equals the plain SPH handle run on that scene
+alone, bit for bit (positions, velocities, end tables), over several steps;
+a hipGraph-captured step equals the eagerly launched one on every replay, with
+eager work and synchronisations between replays; the autotuner leaves the
+state where it found it and its choice changes no bit of the results."""
+import numpy as np
+import pytest
+
+from pbf_amd import batch as B
+from pbf_amd import scene as S
+from pbf_amd import sph as P
+from pbf_amd._lib import PbfError
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def same(a, b):
+    return np.array_equal(np.ascontiguousarray(a).view(np.int32), np.ascontiguousarray(b).view(np.int32))
+
+
+def small_batch(nscenes=6, seed=5):
+    return S.batched_scenes(nscenes=nscenes, seed=seed, n_lo=2048, n_hi=8192)
+
+
+def plain_run(scene, steps):
+    sph = P.SPH(scene.n, scene.grid)
+    sph.set_params(P.scene_params(scene))
+    sph.upload(scene.pos, scene.vel)
+    for _ in range(steps):
+        sph.Run()
+    pos, vel = sph.download()
+    end = sph.end_table()
+    sph.close()
+    return pos, vel, end
+
+
+def test_batched_equals_plain_per_scene(gpu_lib):
+    scenes = small_batch()
+    steps = 6
+    say(f"[batched vs plain] scenes={[s.n for s in scenes]} steps={steps}")
+    b = B.Batch(scenes)
+    b.upload()
+    b.step(steps)
+    pos, vel = b.download()
+    ends = b.end_tables()
+    b.close()
+    cell_off = np.concatenate([[0], np.cumsum([int(np.prod(s.grid)) for s in scenes])])
+    offs = np.concatenate([[0], np.cumsum([s.n for s in scenes])])
+    for k, sc in enumerate(scenes):
+        wp, wv, we = plain_run(sc, steps)
+        assert same(pos[offs[k]:offs[k + 1]], wp), f"scene {k} positions"
+        assert same(vel[offs[k]:offs[k + 1]], wv), f"scene {k} velocities"
+        assert np.array_equal(ends[cell_off[k]:cell_off[k + 1]], we), f"scene {k} end table"
+        moved = np.abs(wp[:, :3] - sc.pos[:, :3]).max()
+        assert moved > 1e-3
+
+
+def test_graph_replays_equal_eager_steps(gpu_lib):
+    scenes = small_batch(4, seed=6)
+    say(f"[graph] scenes={[s.n for s in scenes]}")
+    eager, graph = B.Batch(scenes), B.Batch(scenes)
+    eager.upload()
+    graph.upload()
+    graph.set_graph(True)
+    for k in range(5):
+        eager.step(1)
+        graph.step(1)
+        pe, ve = eager.download()
+        pg, vg = graph.download()
+        assert same(pe, pg) and same(ve, vg), f"replay {k}"
+        assert np.array_equal(eager.end_tables(), graph.end_tables()), f"replay {k} end tables"
+    graph.step(3)  # several replays queued back to back
+    eager.step(3)
+    pe, ve = eager.download()
+    pg, vg = graph.download()
+    assert same(pe, pg) and same(ve, vg)
+    eager.close()
+    graph.close()
+
+
+def test_autotune_restores_state_and_keeps_results(gpu_lib):
+    scenes = small_batch(3, seed=7)
+    say(f"[autotune] scenes={[s.n for s in scenes]}")
+    ref, tuned = B.Batch(scenes), B.Batch(scenes)
+    ref.upload()
+    tuned.upload()
+    ref.step(2)
+    tuned.step(2)
+    p0, v0 = tuned.download()
+    e0 = tuned.end_tables()
+    best, trials = tuned.autotune(1)
+    say(f"[autotune] best {best}, {len(trials)} trials")
+    assert len(trials) == 60 and all(t["step_ms"] > 0 for t in trials)
+    p1, v1 = tuned.download()
+    assert same(p0, p1) and same(v0, v1) and np.array_equal(e0, tuned.end_tables())
+    assert tuned.tuning() == {k: best[k] for k in ("neighbour_block", "stream_block", "stream_ppt")}
+    for shape in [(128, 64, 4), (512, 1024, 2), (256, 256, 1)]:
+        tuned.set_tuning(*shape)
+        tuned.step(1)
+        ref.step(1)
+        pr, vr = ref.download()
+        pt, vt = tuned.download()
+        assert same(pr, pt) and same(vr, vt), f"shape {shape}"
+    ref.close()
+    tuned.close()
+
+
+def test_batch_rejects_bad_input(gpu_lib):
+    sc = small_batch(1)[0]
+    bad = S.Scene("bad", sc.grid, sc.params, sc.pos[:1000], sc.vel[:1000])
+    with pytest.raises(PbfError):
+        B.Batch([bad])
+    b = B.Batch([sc])
+    with pytest.raises(Exception):
+        b.set_tuning(100, 256, 1)
+    b.close()
