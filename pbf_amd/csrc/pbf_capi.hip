@@ -360,17 +360,19 @@ int pbf_sph_create(pbf_sph **out, uint32_t n, int32_t gx, int32_t gy, int32_t gz
     TRY(dalloc(&s->pos, n));
     TRY(dalloc(&s->vel, n));
     TRY(dalloc(&s->xs, n));
-    TRY(dalloc(&s->rec0, n));
-    TRY(dalloc(&s->rec_a, n));
-    TRY(dalloc(&s->rec_b, n));
+    // sorted records, lambda and |omega| keep kPad zero entries after the n
+    // particles (the solver kernels read them for indices past n)
+    TRY(dalloc(&s->rec0, (size_t)n + kPad));
+    TRY(dalloc(&s->rec_a, (size_t)n + kPad));
+    TRY(dalloc(&s->rec_b, (size_t)n + kPad));
     s->rec = s->rec0;
     TRY(dalloc(&s->list_entries, (size_t)kListCap * n));
     TRY(dalloc(&s->list_count, n));
     TRY(dalloc(&s->vel_sorted, n));
     TRY(dalloc(&s->vxsph, n));
     TRY(dalloc(&s->omega, n));
-    TRY(dalloc(&s->lambda, n));
-    TRY(dalloc(&s->vort, n));
+    TRY(dalloc(&s->lambda, (size_t)n + kPad));
+    TRY(dalloc(&s->vort, (size_t)n + kPad));
     TRY(dalloc(&s->keys, n));
     TRY(dalloc(&s->perm, n));
     TRY(s->cells_t.alloc(n, s->cells));
@@ -379,6 +381,11 @@ int pbf_sph_create(pbf_sph **out, uint32_t n, int32_t gx, int32_t gy, int32_t gz
     TRY(hipMemset(s->pos, 0, (size_t)n * sizeof(float4)));
     TRY(hipMemset(s->vel, 0, (size_t)n * sizeof(float4)));
     TRY(hipMemset(s->omega, 0, (size_t)n * sizeof(float4)));
+    TRY(hipMemset(s->rec0, 0, ((size_t)n + kPad) * sizeof(float4)));
+    TRY(hipMemset(s->rec_a, 0, ((size_t)n + kPad) * sizeof(float4)));
+    TRY(hipMemset(s->rec_b, 0, ((size_t)n + kPad) * sizeof(float4)));
+    TRY(hipMemset(s->lambda, 0, ((size_t)n + kPad) * sizeof(float)));
+    TRY(hipMemset(s->vort, 0, ((size_t)n + kPad) * sizeof(float)));
     TRY(hipDeviceSynchronize());
 #undef TRY
     *out = s;
@@ -460,8 +467,8 @@ int pbf_sph_run(pbf_sph *s) {
     const float4 *cur = s->rec0;
     for (uint32_t it = 0; it < s->params.iterations; it++) {
         float4 *dst = (it & 1) ? s->rec_b : s->rec_a;
-        TIMED(KID_LAMBDA, launch_lambda(c, cur, t.nbr9, s->lambda, pl, st));
-        TIMED(KID_UPDATEPOS, launch_updatepos(c, cur, s->lambda, t.nbr9, pl, dst, st));
+        TIMED(KID_LAMBDA, launch_lambda(c, cur, t.nbr9, s->lambda, pl, st, true));
+        TIMED(KID_UPDATEPOS, launch_updatepos(c, cur, s->lambda, t.nbr9, pl, dst, st, true));
         cur = dst;
     }
     s->rec = const_cast<float4 *>(cur);
@@ -470,8 +477,9 @@ int pbf_sph_run(pbf_sph *s) {
     TIMED(KID_UPDATE, launch_update(c, s->rec, s->pos, s->vel, s->params.vorticity ? s->vel_sorted : nullptr, st));
     if (s->params.vorticity) {
         TIMED(KID_VORTICITY_A,
-              launch_vorticity_a(c, s->rec, t.nbr9, s->vel_sorted, s->vel, s->vxsph, s->omega, s->vort, pl, st));
-        TIMED(KID_VORTICITY_B, launch_vorticity_b(c, s->rec, t.nbr9, pl, s->vxsph, s->omega, s->vort, s->vel, st));
+              launch_vorticity_a(c, s->rec, t.nbr9, s->vel_sorted, s->vel, s->vxsph, s->omega, s->vort, pl, st, true));
+        TIMED(KID_VORTICITY_B,
+              launch_vorticity_b(c, s->rec, t.nbr9, pl, s->vxsph, s->omega, s->vort, s->vel, st, true));
     }
 #undef TIMED
     PBF_HIP(hipEventRecord(s->ev[5], st));

@@ -79,24 +79,31 @@ void launch_neighbourcells(const SolverConsts &c, const float4 *sorted, const in
 // All pointers null: the replaying pass scans its candidates.
 // Capacity of the per-particle neighbour list that the lambda pass records for
 // the position update (particles with more neighbours within h scan instead).
-constexpr int32_t kListCap = 64;
+constexpr int32_t kListCap = 128;
+
+// Zero entries kept after the n particles of the one-scene solver's sorted
+// records, lambda and |omega| arrays: a neighbour row starts below n and holds
+// fewer than 128 entries, scanned in batches of 4, so no index reaches n + kPad.
+constexpr int32_t kPad = 256;
 
 struct PairList {
     uint16_t *entries = nullptr;
     uint16_t *count = nullptr;
     int32_t cap = 0;
 };
+// padded: rec, lambda and vort hold kPad zero entries after the n particles.
 void launch_lambda(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, float *lambda, const PairList &pl,
-                   hipStream_t st);
+                   hipStream_t st, bool padded = false);
 void launch_updatepos(const SolverConsts &c, const float4 *rec_in, const float *lambda, const int32_t *nbr9,
-                      const PairList &pl, float4 *rec_out, hipStream_t st);
+                      const PairList &pl, float4 *rec_out, hipStream_t st, bool padded = false);
 void launch_update(const SolverConsts &c, const float4 *rec, float4 *pos, float4 *vel, float4 *vel_sorted,
                    hipStream_t st);
 void launch_vorticity_a(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, const float4 *vel_sorted,
                         const float4 *vel_id, float4 *vxsph, float4 *omega, float *vort, const PairList &pl,
-                        hipStream_t st);
+                        hipStream_t st, bool padded = false);
 void launch_vorticity_b(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, const PairList &pl,
-                        const float4 *vxsph, const float4 *omega, const float *vort, float4 *vel_id, hipStream_t st);
+                        const float4 *vxsph, const float4 *omega, const float *vort, float4 *vel_id, hipStream_t st,
+                        bool padded = false);
 // Launch shape of the batched form: workgroup size of the neighbour passes
 // (128, 256, 512), workgroup size (64 .. 1024, a power of two) and particles
 // per thread (1 .. 4) of the streaming passes. The one-scene form uses the

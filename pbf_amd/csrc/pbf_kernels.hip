@@ -261,8 +261,17 @@ __global__ __launch_bounds__(1024) void k_neighbourcells(SolverConsts c, Dom dom
 // Neighbour records j (scene-local) of the packed ranges; indices past the
 // scene's n (a wrapped count) read a zero record, as the reference's robust
 // buffer access does.
+// PAD: the arrays hold at least kPad zero entries after the last particle
+// (one scene), so indices past n read the zero record without a test.
+template <bool PAD>
 __device__ __forceinline__ float4 rec_at(const float4 *__restrict__ rec, const Seg &sg, int32_t j) {
+    if (PAD) return rec[j];
     return j < sg.n ? rec[sg.off + j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+template <bool PAD>
+__device__ __forceinline__ float val_at(const float *__restrict__ a, const Seg &sg, int32_t j) {
+    if (PAD) return a[j];
+    return j < sg.n ? a[sg.off + j] : 0.0f;
 }
 
 // Neighbour iteration in two phases. FOR_EACH_NEIGHBOUR (foreachneighbour.glsl:
@@ -292,8 +301,10 @@ __device__ __forceinline__ bool maybe_near(const SolverConsts &c, const float4 &
 }
 
 // Full candidate scan of global slot i (scene-local index li); body(j, code)
-// for every candidate j (scene-local) that may lie within h.
-template <int NB, class Body>
+// for every candidate j (scene-local) that may lie within h. Appends go to
+// the slot after the last kept one whatever the test gives, and the count only
+// advances for kept candidates (slot LCAP absorbs the last overwrite).
+template <int NB, bool PAD, class Body>
 __device__ __forceinline__ void scan_neighbours(const SolverConsts &c, const Seg &sg, const float4 *__restrict__ rec,
                                                 const int32_t *__restrict__ nbr9, int32_t i, const float4 &pi,
                                                 NbrLds<NB> &lds, Body &&body) {
@@ -312,21 +323,24 @@ __device__ __forceinline__ void scan_neighbours(const SolverConsts &c, const Seg
         const int32_t d = nbr9[(size_t)o * c.n + i];
         const int32_t s0 = d & 0xFFFFFF, e = d >> 24;
         lds.rowstart[o * NB + t] = s0;
-        // Loads stay in bounds: indices past the row or past the scene repeat
-        // the last valid one, and candidates past the scene are the zero record.
+        // Loads stay in bounds: with PAD, a row ends at most 130 entries past
+        // n (start < n, count < 128, batches of 4), inside the zero padding;
+        // without, indices past the row or past the scene repeat the last valid
+        // one, and candidates past the scene are the zero record.
         const int32_t last = max(min(s0 + e, sg.n) - 1, 0);
+        const float4 *__restrict__ row = srec + s0;
         for (int32_t q = 0; q < e; q += 4) {
             if (cnt > LCAP - 4) flush();
             const int32_t j0 = s0 + q;
             float4 p[4];
 #pragma unroll
-            for (int u = 0; u < 4; u++) p[u] = srec[min(j0 + u, last)];
+            for (int u = 0; u < 4; u++) p[u] = PAD ? row[q + u] : srec[min(j0 + u, last)];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 const int32_t j = j0 + u;
-                const float4 pj = j < sg.n ? p[u] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                const float4 pj = (PAD || j < sg.n) ? p[u] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 const bool f = (q + u < e) & (j != li) & maybe_near(c, pi, pj);
-                lds.list[(f ? cnt : LCAP) * NB + t] = (uint16_t)((o << 8) | (q + u));
+                lds.list[cnt * NB + t] = (uint16_t)((o << 8) | (q + u));
                 cnt += f;
             }
         }
@@ -363,7 +377,7 @@ struct PairListOut {
 // pass) or for candidates that may be within h (scanned); j is scene-local;
 // extra(j) gathers the kernel's per-neighbour operand so that, on the list
 // path, the loads of four entries are in flight together.
-template <int NB, class Extra, class Body>
+template <int NB, bool PAD, class Extra, class Body>
 __device__ __forceinline__ void replay_neighbours(const SolverConsts &c, const Seg &sg, const float4 *__restrict__ rec,
                                                   const int32_t *__restrict__ nbr9, const uint16_t *__restrict__ entries,
                                                   const uint16_t *__restrict__ count, int32_t i, const float4 &pi,
@@ -371,8 +385,8 @@ __device__ __forceinline__ void replay_neighbours(const SolverConsts &c, const S
     const int t = threadIdx.x;
     const uint32_t cnt = entries ? count[i] : LIST_NONE;
     if (cnt == LIST_NONE) {
-        scan_neighbours<NB>(c, sg, rec, nbr9, i, pi, lds,
-                        [&](int32_t j, uint32_t) { body(j, rec_at(rec, sg, j), extra(j), false); });
+        scan_neighbours<NB, PAD>(c, sg, rec, nbr9, i, pi, lds,
+                                 [&](int32_t j, uint32_t) { body(j, rec_at<PAD>(rec, sg, j), extra(j), false); });
         return;
     }
     for (int o = 0; o < 9; o++) lds.rowstart[o * NB + t] = nbr9[(size_t)o * c.n + i] & 0xFFFFFF;
@@ -389,7 +403,7 @@ __device__ __forceinline__ void replay_neighbours(const SolverConsts &c, const S
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             const bool v = k + u < cnt;
-            pj[u] = v ? rec_at(rec, sg, j[u]) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            pj[u] = v ? rec_at<PAD>(rec, sg, j[u]) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             x[u] = v ? extra(j[u]) : X();
         }
 #pragma unroll
@@ -399,7 +413,7 @@ __device__ __forceinline__ void replay_neighbours(const SolverConsts &c, const S
 }
 
 // calclambda.glsl:66-103; records the neighbours within h for the position update.
-template <bool H1, int NB, class Dom>
+template <bool H1, int NB, bool PAD, class Dom>
 __global__ __launch_bounds__(NB) void k_lambda(SolverConsts c, Dom dom, const float4 *__restrict__ rec,
                                                      const int32_t *__restrict__ nbr9, float *__restrict__ lambda,
                                                      uint16_t *__restrict__ list_entries,
@@ -411,8 +425,8 @@ __global__ __launch_bounds__(NB) void k_lambda(SolverConsts c, Dom dom, const fl
     const float4 pi = rec[i];
     float rho = 0.0f, sk = 0.0f, gix = 0.0f, giy = 0.0f, giz = 0.0f;
     PairListOut out = {list_entries, list_count, list_cap, 0};
-    scan_neighbours<NB>(c, sg, rec, nbr9, i, pi, lds, [&](int32_t j, uint32_t code) {
-        const float4 pj = rec_at(rec, sg, j);
+    scan_neighbours<NB, PAD>(c, sg, rec, nbr9, i, pi, lds, [&](int32_t j, uint32_t code) {
+        const float4 pj = rec_at<PAD>(rec, sg, j);
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const float sq = sq_len(dx, dy, dz);
         if (sq > c.h2cut) return;
@@ -439,7 +453,7 @@ __global__ __launch_bounds__(NB) void k_lambda(SolverConsts c, Dom dom, const fl
 
 // updatepos.glsl:42-105, race-free: neighbours read from rec_in, result to
 // rec_out; replays the lambda pass's neighbour list (same positions).
-template <bool H1, int NB, class Dom>
+template <bool H1, int NB, bool PAD, class Dom>
 __global__ __launch_bounds__(NB) void k_updatepos(SolverConsts c, Dom dom, const float4 *__restrict__ rec_in,
                                                         const float *__restrict__ lambda,
                                                         const int32_t *__restrict__ nbr9,
@@ -452,10 +466,9 @@ __global__ __launch_bounds__(NB) void k_updatepos(SolverConsts c, Dom dom, const
     const Seg sg = dom.at(i);
     const float4 pi = rec_in[i];
     const float li = lambda[i];
-    const float *__restrict__ slam = lambda + sg.off;
     float dpx = 0.0f, dpy = 0.0f, dpz = 0.0f;
-    replay_neighbours<NB>(c, sg, rec_in, nbr9, list_entries, list_count, i, pi, lds,
-                      [&](int32_t j) { return j < sg.n ? slam[j] : 0.0f; },
+    replay_neighbours<NB, PAD>(c, sg, rec_in, nbr9, list_entries, list_count, i, pi, lds,
+                      [&](int32_t j) { return val_at<PAD>(lambda, sg, j); },
                       [&](int32_t, const float4 &pj, float lj, bool listed) {
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const float sq = sq_len(dx, dy, dz);
@@ -501,7 +514,7 @@ __global__ __launch_bounds__(1024) void k_update(SolverConsts c, Dom dom, int32_
 // vorticity.glsl:34-60 (first half): XSPH and omega from the velocities before
 // this pass; |omega| to the vorticity buffer (vorticities[], :60). Records the
 // neighbours within h for the second half.
-template <bool H1, int NB, class Dom>
+template <bool H1, int NB, bool PAD, class Dom>
 __global__ __launch_bounds__(NB) void k_vorticity_a(SolverConsts c, Dom dom, const float4 *__restrict__ rec,
                                                           const int32_t *__restrict__ nbr9,
                                                           const float4 *__restrict__ vel_sorted,
@@ -517,8 +530,8 @@ __global__ __launch_bounds__(NB) void k_vorticity_a(SolverConsts c, Dom dom, con
     const float4 vi = vel_sorted[i];
     float sx = 0.0f, sy = 0.0f, sz = 0.0f, wx = 0.0f, wy = 0.0f, wz = 0.0f;
     PairListOut out = {list_entries, list_count, list_cap, 0};
-    scan_neighbours<NB>(c, sg, rec, nbr9, i, pi, lds, [&](int32_t j, uint32_t code) {
-        const float4 pj = rec_at(rec, sg, j);
+    scan_neighbours<NB, PAD>(c, sg, rec, nbr9, i, pi, lds, [&](int32_t j, uint32_t code) {
+        const float4 pj = rec_at<PAD>(rec, sg, j);
         const float px = pi.x - pj.x, py = pi.y - pj.y, pz = pi.z - pj.z;
         const float sq = sq_len(px, py, pz);
         if (sq > c.h2cut) return;
@@ -547,7 +560,7 @@ __global__ __launch_bounds__(NB) void k_vorticity_a(SolverConsts c, Dom dom, con
 
 // vorticity.glsl:65-85 (second half): confinement from this step's |omega|;
 // replays the first half's neighbour list.
-template <bool H1, int NB, class Dom>
+template <bool H1, int NB, bool PAD, class Dom>
 __global__ __launch_bounds__(NB) void k_vorticity_b(SolverConsts c, Dom dom, const float4 *__restrict__ rec,
                                                           const int32_t *__restrict__ nbr9,
                                                           const uint16_t *__restrict__ list_entries,
@@ -561,10 +574,9 @@ __global__ __launch_bounds__(NB) void k_vorticity_b(SolverConsts c, Dom dom, con
     if (i >= c.n) return;
     const Seg sg = dom.at(i);
     const float4 pi = rec[i];
-    const float *__restrict__ svort = vort + sg.off;
     float nx = 0.0f, ny = 0.0f, nz = 0.0f;
-    replay_neighbours<NB>(c, sg, rec, nbr9, list_entries, list_count, i, pi, lds,
-                      [&](int32_t j) { return j < sg.n ? svort[j] : 0.0f; },
+    replay_neighbours<NB, PAD>(c, sg, rec, nbr9, list_entries, list_count, i, pi, lds,
+                      [&](int32_t j) { return val_at<PAD>(vort, sg, j); },
                       [&](int32_t, const float4 &pj, float om, bool listed) {
         const float px = pi.x - pj.x, py = pi.y - pj.y, pz = pi.z - pj.z;
         const float sq = sq_len(px, py, pz);
@@ -673,13 +685,23 @@ unsigned stream_blocks(int32_t n, const LaunchShape &ls) {
 }
 unsigned nb_blocks(int32_t n, int nb) { return (unsigned)((n + nb - 1) / nb); }
 
-// Calls f with the neighbour workgroup size as a compile-time constant.
-template <class F>
-void with_nb(int nb, F &&f) {
-    switch (nb) {
-        case 128: f(std::integral_constant<int, 128>()); break;
-        case 512: f(std::integral_constant<int, 512>()); break;
-        default: f(std::integral_constant<int, 256>()); break;
+// Calls f with the neighbour workgroup size and the padding flag as
+// compile-time constants: one scene runs 256-wide, padded when its arrays are
+// (the SPH handle's are, caller buffers of the stage entry points are not);
+// many scenes take the tuned width, unpadded.
+template <class Dom, class F>
+void with_nb(int nb, bool pad, F &&f) {
+    if constexpr (std::is_same<Dom, DomOne>::value) {
+        if (pad)
+            f(std::integral_constant<int, 256>(), std::true_type());
+        else
+            f(std::integral_constant<int, 256>(), std::false_type());
+    } else {
+        switch (nb) {
+            case 128: f(std::integral_constant<int, 128>(), std::false_type()); break;
+            case 512: f(std::integral_constant<int, 512>(), std::false_type()); break;
+            default: f(std::integral_constant<int, 256>(), std::false_type()); break;
+        }
     }
 }
 
@@ -706,61 +728,65 @@ void update_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, con
     if (c.n > 0) k_update<Dom><<<stream_blocks(c.n, ls), ls.sb, 0, st>>>(c, d, ls.ppt, rec, pos, vel, vel_sorted);
 }
 template <class Dom>
-void lambda_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, const float4 *rec, const int32_t *nbr9,
+void lambda_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, bool pad, const float4 *rec, const int32_t *nbr9,
                  float *lambda, const PairList &pl, hipStream_t st) {
     if (c.n <= 0) return;
-    with_nb(ls.nb, [&](auto nbc) {
+    with_nb<Dom>(ls.nb, pad, [&](auto nbc, auto padc) {
         constexpr int NB = decltype(nbc)::value;
+        constexpr bool PAD = decltype(padc)::value;
         if (c.h_is_one)
-            k_lambda<true, NB, Dom><<<nb_blocks(c.n, NB), NB, 0, st>>>(c, d, rec, nbr9, lambda, pl.entries, pl.count,
+            k_lambda<true, NB, PAD, Dom><<<nb_blocks(c.n, NB), NB, 0, st>>>(c, d, rec, nbr9, lambda, pl.entries, pl.count,
                                                                        pl.cap);
         else
-            k_lambda<false, NB, Dom><<<nb_blocks(c.n, NB), NB, 0, st>>>(c, d, rec, nbr9, lambda, pl.entries,
+            k_lambda<false, NB, PAD, Dom><<<nb_blocks(c.n, NB), NB, 0, st>>>(c, d, rec, nbr9, lambda, pl.entries,
                                                                         pl.count, pl.cap);
     });
 }
 template <class Dom>
-void updatepos_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, const float4 *rec_in,
+void updatepos_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, bool pad, const float4 *rec_in,
                     const float *lambda, const int32_t *nbr9, const PairList &pl, float4 *rec_out, hipStream_t st) {
     if (c.n <= 0) return;
-    with_nb(ls.nb, [&](auto nbc) {
+    with_nb<Dom>(ls.nb, pad, [&](auto nbc, auto padc) {
         constexpr int NB = decltype(nbc)::value;
+        constexpr bool PAD = decltype(padc)::value;
         if (c.h_is_one)
-            k_updatepos<true, NB, Dom><<<nb_blocks(c.n, NB), NB, 0, st>>>(c, d, rec_in, lambda, nbr9, pl.entries,
+            k_updatepos<true, NB, PAD, Dom><<<nb_blocks(c.n, NB), NB, 0, st>>>(c, d, rec_in, lambda, nbr9, pl.entries,
                                                                           pl.count, rec_out);
         else
-            k_updatepos<false, NB, Dom><<<nb_blocks(c.n, NB), NB, 0, st>>>(c, d, rec_in, lambda, nbr9, pl.entries,
+            k_updatepos<false, NB, PAD, Dom><<<nb_blocks(c.n, NB), NB, 0, st>>>(c, d, rec_in, lambda, nbr9, pl.entries,
                                                                            pl.count, rec_out);
     });
 }
 template <class Dom>
-void vorticity_a_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, const float4 *rec,
+void vorticity_a_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, bool pad, const float4 *rec,
                       const int32_t *nbr9, const float4 *vel_sorted, const float4 *vel_id, float4 *vxsph,
                       float4 *omega, float *vort, const PairList &pl, hipStream_t st) {
     if (c.n <= 0) return;
-    with_nb(ls.nb, [&](auto nbc) {
+    with_nb<Dom>(ls.nb, pad, [&](auto nbc, auto padc) {
         constexpr int NB = decltype(nbc)::value;
+        constexpr bool PAD = decltype(padc)::value;
         if (c.h_is_one)
-            k_vorticity_a<true, NB, Dom><<<nb_blocks(c.n, NB), NB, 0, st>>>(c, d, rec, nbr9, vel_sorted, vel_id, vxsph,
+            k_vorticity_a<true, NB, PAD, Dom><<<nb_blocks(c.n, NB), NB, 0, st>>>(c, d, rec, nbr9, vel_sorted, vel_id, vxsph,
                                                                             omega, vort, pl.entries, pl.count, pl.cap);
         else
-            k_vorticity_a<false, NB, Dom><<<nb_blocks(c.n, NB), NB, 0, st>>>(c, d, rec, nbr9, vel_sorted, vel_id,
+            k_vorticity_a<false, NB, PAD, Dom><<<nb_blocks(c.n, NB), NB, 0, st>>>(c, d, rec, nbr9, vel_sorted, vel_id,
                                                                              vxsph, omega, vort, pl.entries, pl.count,
                                                                              pl.cap);
     });
 }
 template <class Dom>
-void vorticity_b_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, const float4 *rec,
+void vorticity_b_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, bool pad, const float4 *rec,
                       const int32_t *nbr9, const PairList &pl, const float4 *vxsph, const float4 *omega,
                       const float *vort, float4 *vel_id, hipStream_t st) {
     if (c.n <= 0) return;
-    with_nb(ls.nb, [&](auto nbc) {
+    with_nb<Dom>(ls.nb, pad, [&](auto nbc, auto padc) {
         constexpr int NB = decltype(nbc)::value;
+        constexpr bool PAD = decltype(padc)::value;
         if (c.h_is_one)
-            k_vorticity_b<true, NB, Dom><<<nb_blocks(c.n, NB), NB, 0, st>>>(c, d, rec, nbr9, pl.entries, pl.count,
+            k_vorticity_b<true, NB, PAD, Dom><<<nb_blocks(c.n, NB), NB, 0, st>>>(c, d, rec, nbr9, pl.entries, pl.count,
                                                                             vxsph, omega, vort, vel_id);
         else
-            k_vorticity_b<false, NB, Dom><<<nb_blocks(c.n, NB), NB, 0, st>>>(c, d, rec, nbr9, pl.entries, pl.count,
+            k_vorticity_b<false, NB, PAD, Dom><<<nb_blocks(c.n, NB), NB, 0, st>>>(c, d, rec, nbr9, pl.entries, pl.count,
                                                                              vxsph, omega, vort, vel_id);
     });
 }
@@ -795,12 +821,12 @@ void launch_neighbourcells(const SolverConsts &c, const float4 *sorted, const in
     neighbourcells_impl(c, one(c), kDefaultShape, sorted, start, end64, nbr9, st);
 }
 void launch_lambda(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, float *lambda, const PairList &pl,
-                   hipStream_t st) {
-    lambda_impl(c, one(c), kDefaultShape, rec, nbr9, lambda, pl, st);
+                   hipStream_t st, bool padded) {
+    lambda_impl(c, one(c), kDefaultShape, padded, rec, nbr9, lambda, pl, st);
 }
 void launch_updatepos(const SolverConsts &c, const float4 *rec_in, const float *lambda, const int32_t *nbr9,
-                      const PairList &pl, float4 *rec_out, hipStream_t st) {
-    updatepos_impl(c, one(c), kDefaultShape, rec_in, lambda, nbr9, pl, rec_out, st);
+                      const PairList &pl, float4 *rec_out, hipStream_t st, bool padded) {
+    updatepos_impl(c, one(c), kDefaultShape, padded, rec_in, lambda, nbr9, pl, rec_out, st);
 }
 void launch_update(const SolverConsts &c, const float4 *rec, float4 *pos, float4 *vel, float4 *vel_sorted,
                    hipStream_t st) {
@@ -808,12 +834,13 @@ void launch_update(const SolverConsts &c, const float4 *rec, float4 *pos, float4
 }
 void launch_vorticity_a(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, const float4 *vel_sorted,
                         const float4 *vel_id, float4 *vxsph, float4 *omega, float *vort, const PairList &pl,
-                        hipStream_t st) {
-    vorticity_a_impl(c, one(c), kDefaultShape, rec, nbr9, vel_sorted, vel_id, vxsph, omega, vort, pl, st);
+                        hipStream_t st, bool padded) {
+    vorticity_a_impl(c, one(c), kDefaultShape, padded, rec, nbr9, vel_sorted, vel_id, vxsph, omega, vort, pl, st);
 }
 void launch_vorticity_b(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, const PairList &pl,
-                        const float4 *vxsph, const float4 *omega, const float *vort, float4 *vel_id, hipStream_t st) {
-    vorticity_b_impl(c, one(c), kDefaultShape, rec, nbr9, pl, vxsph, omega, vort, vel_id, st);
+                        const float4 *vxsph, const float4 *omega, const float *vort, float4 *vel_id, hipStream_t st,
+                        bool padded) {
+    vorticity_b_impl(c, one(c), kDefaultShape, padded, rec, nbr9, pl, vxsph, omega, vort, vel_id, st);
 }
 
 // many scenes
@@ -831,12 +858,12 @@ void launch_neighbourcells(const SolverConsts &c, const DomMany &d, const Launch
 }
 void launch_lambda(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec,
                    const int32_t *nbr9, float *lambda, const PairList &pl, hipStream_t st) {
-    lambda_impl(c, d, ls, rec, nbr9, lambda, pl, st);
+    lambda_impl(c, d, ls, false, rec, nbr9, lambda, pl, st);
 }
 void launch_updatepos(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec_in,
                       const float *lambda, const int32_t *nbr9, const PairList &pl, float4 *rec_out,
                       hipStream_t st) {
-    updatepos_impl(c, d, ls, rec_in, lambda, nbr9, pl, rec_out, st);
+    updatepos_impl(c, d, ls, false, rec_in, lambda, nbr9, pl, rec_out, st);
 }
 void launch_update(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec, float4 *pos,
                    float4 *vel, float4 *vel_sorted, hipStream_t st) {
@@ -845,12 +872,12 @@ void launch_update(const SolverConsts &c, const DomMany &d, const LaunchShape &l
 void launch_vorticity_a(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec,
                         const int32_t *nbr9, const float4 *vel_sorted, const float4 *vel_id, float4 *vxsph,
                         float4 *omega, float *vort, const PairList &pl, hipStream_t st) {
-    vorticity_a_impl(c, d, ls, rec, nbr9, vel_sorted, vel_id, vxsph, omega, vort, pl, st);
+    vorticity_a_impl(c, d, ls, false, rec, nbr9, vel_sorted, vel_id, vxsph, omega, vort, pl, st);
 }
 void launch_vorticity_b(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec,
                         const int32_t *nbr9, const PairList &pl, const float4 *vxsph, const float4 *omega,
                         const float *vort, float4 *vel_id, hipStream_t st) {
-    vorticity_b_impl(c, d, ls, rec, nbr9, pl, vxsph, omega, vort, vel_id, st);
+    vorticity_b_impl(c, d, ls, false, rec, nbr9, pl, vxsph, omega, vort, vel_id, st);
 }
 void launch_nbr9_to_12(int32_t n, const int32_t *nbr9, int32_t *nbr12, hipStream_t st) {
     if (n > 0) k_nbr9_to_12<<<blocks_for(n), BLOCK, 0, st>>>(n, nbr9, nbr12);

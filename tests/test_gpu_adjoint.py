@@ -183,6 +183,9 @@ def test_torch_autograd_function(gpu_lib):
     xr = r.forward(sc.pos, sc.vel)
     assert np.array_equal(x.detach().cpu().numpy(), xr[:, :3])
     gv, gc, ge = r.backward(mse_grad(xr, np.concatenate([target.cpu().numpy(), np.zeros((sc.n, 1), np.float32)], 1)))
-    assert np.allclose(v0.grad.cpu().numpy(), gv[:, :3], rtol=1e-5, atol=1e-12)
+    # the reverse kernels scatter with atomic adds, so two FP32 backward calls
+    # sum in different orders: equal to rounding, not bit for bit
+    scale = np.abs(gv[:, :3]).max()
+    assert np.abs(v0.grad.cpu().numpy() - gv[:, :3]).max() <= 1e-4 * scale
     assert abs(c.grad.item() - gc) <= 1e-4 * abs(gc) and abs(e.grad.item() - ge) <= 1e-4 * max(abs(ge), 1e-20)
     r.close()
