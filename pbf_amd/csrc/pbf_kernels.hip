@@ -68,13 +68,20 @@ __device__ __forceinline__ uint32_t cell_hash(const Seg &s, float x, float y, fl
 __device__ __forceinline__ int32_t id_of(const float4 &r) { return __float_as_int(r.w); }
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
-// Renumbering them so that every XCD runs one contiguous range of particles
-// keeps that range's neighbour rows in its L2 instead of all XCDs sharing a
-// strided view of the whole array. Bijective on [0, gridDim.x).
+// Renumbering them so that every XCD runs chunks of kXcdChunk consecutive
+// workgroups keeps a chunk's neighbour rows in one L2, while consecutive chunks
+// go to different XCDs, so a dense region of the sorted array (whose
+// particles have many more neighbours than the rest) is shared by all XCDs
+// instead of landing on one. Bijective on [0, gridDim.x): full groups of
+// 8 kXcdChunk workgroups are permuted, the remainder keeps its numbers.
+constexpr uint32_t kXcdChunk = 8;
+
 __device__ __forceinline__ uint32_t xcd_block() {
     const uint32_t nb = gridDim.x, b = blockIdx.x;
-    const uint32_t x = b % 8u, k = b / 8u, q = nb / 8u, r = nb % 8u;
-    return x * q + (x < r ? x : r) + k;
+    const uint32_t group = 8u * kXcdChunk;
+    if (b >= nb / group * group) return b;
+    const uint32_t x = b % 8u, k = b / 8u;
+    return (k / kXcdChunk) * group + x * kXcdChunk + (k % kXcdChunk);
 }
 
 // ----------------------------------------------------------------- predict
@@ -300,22 +307,40 @@ __device__ __forceinline__ bool maybe_near(const SolverConsts &c, const float4 &
     return fmaf(dx, dx, fmaf(dy, dy, dz * dz)) <= c.h2loose;
 }
 
-// Full candidate scan of global slot i (scene-local index li); body(j, code)
-// for every candidate j (scene-local) that may lie within h. Appends go to
-// the slot after the last kept one whatever the test gives, and the count only
-// advances for kept candidates (slot LCAP absorbs the last overwrite).
-template <int NB, bool PAD, class Body>
+// Full candidate scan of global slot i (scene-local index li); body(j, code,
+// rec[j], extra(j)) for every candidate j (scene-local) that may lie within h,
+// in visiting order. Appends go to the slot after the last kept one whatever
+// the test gives, and the count only advances for kept candidates. The kept
+// candidates are evaluated four at a time with their loads issued together,
+// so a particle with hundreds of neighbours waits for one round trip per four
+// of them rather than per neighbour.
+template <int NB, bool PAD, class Extra, class Body>
 __device__ __forceinline__ void scan_neighbours(const SolverConsts &c, const Seg &sg, const float4 *__restrict__ rec,
                                                 const int32_t *__restrict__ nbr9, int32_t i, const float4 &pi,
-                                                NbrLds<NB> &lds, Body &&body) {
+                                                NbrLds<NB> &lds, Extra &&extra, Body &&body) {
     const int t = threadIdx.x;
     const int32_t li = i - sg.off;
     const float4 *__restrict__ srec = rec + sg.off;
+    using X = decltype(extra(0));
     int cnt = 0;
     auto flush = [&]() {
-        for (int k = 0; k < cnt; k++) {
-            const uint32_t e = lds.list[k * NB + t];
-            body(lds.rowstart[(e >> 8) * NB + t] + (int32_t)(e & 255u), e);
+        for (int k = 0; k < cnt; k += 4) {
+            uint32_t e[4];
+            int32_t j[4];
+            float4 pj[4];
+            X x[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) e[u] = lds.list[min(k + u, cnt - 1) * NB + t];
+#pragma unroll
+            for (int u = 0; u < 4; u++) j[u] = lds.rowstart[(e[u] >> 8) * NB + t] + (int32_t)(e[u] & 255u);
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                pj[u] = rec_at<PAD>(rec, sg, j[u]);
+                x[u] = extra(j[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (k + u < cnt) body(j[u], e[u], pj[u], x[u]);
         }
         cnt = 0;
     };
@@ -385,8 +410,8 @@ __device__ __forceinline__ void replay_neighbours(const SolverConsts &c, const S
     const int t = threadIdx.x;
     const uint32_t cnt = entries ? count[i] : LIST_NONE;
     if (cnt == LIST_NONE) {
-        scan_neighbours<NB, PAD>(c, sg, rec, nbr9, i, pi, lds,
-                                 [&](int32_t j, uint32_t) { body(j, rec_at<PAD>(rec, sg, j), extra(j), false); });
+        scan_neighbours<NB, PAD>(c, sg, rec, nbr9, i, pi, lds, extra,
+                                 [&](int32_t j, uint32_t, const float4 &pj, auto x) { body(j, pj, x, false); });
         return;
     }
     for (int o = 0; o < 9; o++) lds.rowstart[o * NB + t] = nbr9[(size_t)o * c.n + i] & 0xFFFFFF;
@@ -425,8 +450,8 @@ __global__ __launch_bounds__(NB) void k_lambda(SolverConsts c, Dom dom, const fl
     const float4 pi = rec[i];
     float rho = 0.0f, sk = 0.0f, gix = 0.0f, giy = 0.0f, giz = 0.0f;
     PairListOut out = {list_entries, list_count, list_cap, 0};
-    scan_neighbours<NB, PAD>(c, sg, rec, nbr9, i, pi, lds, [&](int32_t j, uint32_t code) {
-        const float4 pj = rec_at<PAD>(rec, sg, j);
+    scan_neighbours<NB, PAD>(c, sg, rec, nbr9, i, pi, lds, [](int32_t) { return 0; },
+                             [&](int32_t, uint32_t code, const float4 &pj, int) {
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const float sq = sq_len(dx, dy, dz);
         if (sq > c.h2cut) return;
@@ -530,16 +555,16 @@ __global__ __launch_bounds__(NB) void k_vorticity_a(SolverConsts c, Dom dom, con
     const float4 vi = vel_sorted[i];
     float sx = 0.0f, sy = 0.0f, sz = 0.0f, wx = 0.0f, wy = 0.0f, wz = 0.0f;
     PairListOut out = {list_entries, list_count, list_cap, 0};
-    scan_neighbours<NB, PAD>(c, sg, rec, nbr9, i, pi, lds, [&](int32_t j, uint32_t code) {
-        const float4 pj = rec_at<PAD>(rec, sg, j);
+    // a zero record past the scene has id 0: imageLoad (velocitytexture, 0)
+    scan_neighbours<NB, PAD>(c, sg, rec, nbr9, i, pi, lds,
+                             [&](int32_t j) { return j < sg.n ? vel_sorted[sg.off + j] : vel_id[sg.off]; },
+                             [&](int32_t, uint32_t code, const float4 &pj, const float4 &vj) {
         const float px = pi.x - pj.x, py = pi.y - pj.y, pz = pi.z - pj.z;
         const float sq = sq_len(px, py, pz);
         if (sq > c.h2cut) return;
         const float r = sqrt_rn(sq);
         if (r > c.h) return;
         out.add(i, c.n, code);
-        // a zero record past the scene has id 0: imageLoad (velocitytexture, 0)
-        const float4 vj = j < sg.n ? vel_sorted[sg.off + j] : vel_id[sg.off];
         const float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
         const float w = wpoly6<H1>(c, sq, r);
         sx += ux * w;
