@@ -293,6 +293,11 @@ __device__ __forceinline__ float val_at(const float *__restrict__ a, const Seg &
 // reference's order, and candidates with fl(sqrt(sq)) > h add exact zeros
 // there, so skipping them changes no bit of the result.
 constexpr int LCAP = 48;
+// Candidates (and listed neighbours) whose loads are issued together: a
+// particle in a dense clump walks hundreds of them serially, and its wave
+// alone can set the kernel's duration, so memory-level parallelism per lane
+// matters more than registers here.
+constexpr int kBatch = 8;
 
 template <int NB>
 struct NbrLds {
@@ -324,22 +329,22 @@ __device__ __forceinline__ void scan_neighbours(const SolverConsts &c, const Seg
     using X = decltype(extra(0));
     int cnt = 0;
     auto flush = [&]() {
-        for (int k = 0; k < cnt; k += 4) {
-            uint32_t e[4];
-            int32_t j[4];
-            float4 pj[4];
-            X x[4];
+        for (int k = 0; k < cnt; k += kBatch) {
+            uint32_t e[kBatch];
+            int32_t j[kBatch];
+            float4 pj[kBatch];
+            X x[kBatch];
 #pragma unroll
-            for (int u = 0; u < 4; u++) e[u] = lds.list[min(k + u, cnt - 1) * NB + t];
+            for (int u = 0; u < kBatch; u++) e[u] = lds.list[min(k + u, cnt - 1) * NB + t];
 #pragma unroll
-            for (int u = 0; u < 4; u++) j[u] = lds.rowstart[(e[u] >> 8) * NB + t] + (int32_t)(e[u] & 255u);
+            for (int u = 0; u < kBatch; u++) j[u] = lds.rowstart[(e[u] >> 8) * NB + t] + (int32_t)(e[u] & 255u);
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
+            for (int u = 0; u < kBatch; u++) {
                 pj[u] = rec_at<PAD>(rec, sg, j[u]);
                 x[u] = extra(j[u]);
             }
 #pragma unroll
-            for (int u = 0; u < 4; u++)
+            for (int u = 0; u < kBatch; u++)
                 if (k + u < cnt) body(j[u], e[u], pj[u], x[u]);
         }
         cnt = 0;
@@ -348,20 +353,20 @@ __device__ __forceinline__ void scan_neighbours(const SolverConsts &c, const Seg
         const int32_t d = nbr9[(size_t)o * c.n + i];
         const int32_t s0 = d & 0xFFFFFF, e = d >> 24;
         lds.rowstart[o * NB + t] = s0;
-        // Loads stay in bounds: with PAD, a row ends at most 130 entries past
-        // n (start < n, count < 128, batches of 4), inside the zero padding;
+        // Loads stay in bounds: with PAD, a row ends at most 134 entries past
+        // n (start < n, count < 128, batches of 8), inside the zero padding;
         // without, indices past the row or past the scene repeat the last valid
         // one, and candidates past the scene are the zero record.
         const int32_t last = max(min(s0 + e, sg.n) - 1, 0);
         const float4 *__restrict__ row = srec + s0;
-        for (int32_t q = 0; q < e; q += 4) {
-            if (cnt > LCAP - 4) flush();
+        for (int32_t q = 0; q < e; q += kBatch) {
+            if (cnt > LCAP - kBatch) flush();
             const int32_t j0 = s0 + q;
-            float4 p[4];
+            float4 p[kBatch];
 #pragma unroll
-            for (int u = 0; u < 4; u++) p[u] = PAD ? row[q + u] : srec[min(j0 + u, last)];
+            for (int u = 0; u < kBatch; u++) p[u] = PAD ? row[q + u] : srec[min(j0 + u, last)];
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
+            for (int u = 0; u < kBatch; u++) {
                 const int32_t j = j0 + u;
                 const float4 pj = (PAD || j < sg.n) ? p[u] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 const bool f = (q + u < e) & (j != li) & maybe_near(c, pi, pj);
@@ -416,23 +421,23 @@ __device__ __forceinline__ void replay_neighbours(const SolverConsts &c, const S
     }
     for (int o = 0; o < 9; o++) lds.rowstart[o * NB + t] = nbr9[(size_t)o * c.n + i] & 0xFFFFFF;
     using X = decltype(extra(0));
-    for (uint32_t k = 0; k < cnt; k += 4) {
-        uint32_t e[4];
-        int32_t j[4];
-        float4 pj[4];
-        X x[4];
+    for (uint32_t k = 0; k < cnt; k += kBatch) {
+        uint32_t e[kBatch];
+        int32_t j[kBatch];
+        float4 pj[kBatch];
+        X x[kBatch];
 #pragma unroll
-        for (int u = 0; u < 4; u++) e[u] = k + u < cnt ? entries[(size_t)(k + u) * c.n + i] : 0u;
+        for (int u = 0; u < kBatch; u++) e[u] = k + u < cnt ? entries[(size_t)(k + u) * c.n + i] : 0u;
 #pragma unroll
-        for (int u = 0; u < 4; u++) j[u] = lds.rowstart[(e[u] >> 8) * NB + t] + (int32_t)(e[u] & 255u);
+        for (int u = 0; u < kBatch; u++) j[u] = lds.rowstart[(e[u] >> 8) * NB + t] + (int32_t)(e[u] & 255u);
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
+        for (int u = 0; u < kBatch; u++) {
             const bool v = k + u < cnt;
             pj[u] = v ? rec_at<PAD>(rec, sg, j[u]) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             x[u] = v ? extra(j[u]) : X();
         }
 #pragma unroll
-        for (int u = 0; u < 4; u++)
+        for (int u = 0; u < kBatch; u++)
             if (k + u < cnt) body(j[u], pj[u], x[u], true);
     }
 }
