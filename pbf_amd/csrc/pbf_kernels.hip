@@ -40,14 +40,16 @@ __device__ __forceinline__ float wpoly6(const SolverConsts &c, float sq, float r
 
 // gradWspiky (calclambda.glsl:57-64) for l = sqrt(sq) with 0 < l <= h:
 // (-3 * 15/pi * t * t) * r / (l * h*h*h*h*h*h), t = h - l; the three
-// quotients correctly rounded (pbf_math.h).
+// quotients correctly rounded (pbf_math.h). With h = 1 the denominator l is at
+// most 1 and the numerators at most 14.33 (1 + 2^-22) in magnitude (each
+// component of r within l up to rounding), inside the fast upper bounds.
 template <bool H1>
 __device__ __forceinline__ void grad_spiky(const SolverConsts &c, float dx, float dy, float dz, float l, float &gx,
                                            float &gy, float &gz) {
     const float t = c.h - l;
     const float s = (SPIKY_GRAD * t) * t;
     const float den = H1 ? l : (((((l * c.h) * c.h) * c.h) * c.h) * c.h) * c.h;
-    div3_rn(s * dx, s * dy, s * dz, den, gx, gy, gz);
+    div3_rn<H1>(s * dx, s * dy, s * dz, den, gx, gy, gz);
 }
 
 __device__ __forceinline__ float clamp_glsl(float x, float lo, float hi) {

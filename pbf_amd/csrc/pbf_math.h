@@ -40,10 +40,14 @@ __device__ __forceinline__ float sqrt_rn_fast(float x) {
 // condition, so that rare fallbacks cost no per-lane mask bookkeeping.
 __device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
 
+// The empty volatile asm keeps the compiler from speculating the general
+// sequence into every call (it would otherwise if-convert the rare branch and
+// run both square roots for every operand).
 __device__ __forceinline__ float sqrt_rn(float x) {
     float r = sqrt_rn_fast(x);
     const bool slow = !(x >= kSqrtLo) | !(x < __builtin_inff());
     if (wave_any(slow)) {
+        asm volatile("" ::: "memory");
         if (slow) r = sqrtf(x);
     }
     return r;
@@ -76,15 +80,27 @@ __device__ __forceinline__ bool fast_numerator(float a) {
     return (aa == 0.0f) | ((aa >= kNumLo) & (aa <= kNumHi));
 }
 
+// True when a is zero or |a| >= kNumLo (callers that know |a| <= kNumHi).
+__device__ __forceinline__ bool fast_numerator_lo(float a) {
+    const float aa = fabsf(a);
+    return (aa == 0.0f) | (aa >= kNumLo);
+}
+
 // RN(a_k / b), k = 0..2, for b > 0: the fast path for every lane, replaced by
 // the IEEE division in the lanes whose operands leave the fast ranges.
+// BOUNDED: the caller guarantees b <= kDenHi and |a_k| <= kNumHi, so only the
+// lower bounds are tested.
+template <bool BOUNDED = false>
 __device__ __forceinline__ void div3_rn(float a0, float a1, float a2, float b, float &q0, float &q1, float &q2) {
     const float y = rcp_rn_fast(b);
     q0 = div_markstein(a0, b, y);
     q1 = div_markstein(a1, b, y);
     q2 = div_markstein(a2, b, y);
-    const bool fast = (b >= kDenLo) & (b <= kDenHi) & fast_numerator(a0) & fast_numerator(a1) & fast_numerator(a2);
+    const bool fast = BOUNDED ? (b >= kDenLo) & fast_numerator_lo(a0) & fast_numerator_lo(a1) & fast_numerator_lo(a2)
+                              : (b >= kDenLo) & (b <= kDenHi) & fast_numerator(a0) & fast_numerator(a1) &
+                                    fast_numerator(a2);
     if (wave_any(!fast)) {
+        asm volatile("" ::: "memory");
         if (!fast) {
             q0 = a0 / b;
             q1 = a1 / b;

@@ -3,7 +3,8 @@
 //   reciprocal  every float b with |b| in [2^-60, 2^60)           (exhaustive)
 //   square root every float x in [2^-96, 2^8)                      (exhaustive)
 //   division    pseudo-random a, b of the fast ranges, including the
-//               solver's operands a = s dx, b = l in (0, 1]         (sampled)
+//               solver's operands a = s dx, b = l in (0, 1]         (sampled;
+//               the bounded form on the classes inside its bounds)
 #include "pbf_internal.h"
 #include "pbf_math.h"
 
@@ -93,6 +94,10 @@ __global__ __launch_bounds__(ST_BLOCK) void k_test_div(uint64_t count, uint64_t 
             float q0, q1, q2;
             div3_rn(a, a, a, b, q0, q1, q2);
             mis = __float_as_uint(q0) != __float_as_uint(a / b);
+            if ((k & 3) != 1) {  // classes within b <= 2^30, |a| <= 2^60: the bounded form too
+                div3_rn<true>(a, a, a, b, q0, q1, q2);
+                mis |= __float_as_uint(q0) != __float_as_uint(a / b);
+            }
         }
         count_mismatch(mis, bad);
     }
