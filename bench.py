@@ -48,11 +48,11 @@ ALG_BYTES = {
 # lambda writes its pair lists (2 B per in-range neighbour) and updatepos reads
 # them back; both also read the 9-row neighbour table (36 B per particle).
 MEASURED_TRAFFIC_BLOCK_1M = {
-    "lambda": 129.3,
-    "updatepos": 150.0,
-    "vorticity_a": 212.7,
-    "vorticity_b": 194.8,
-    "update": 87.9,
+    "lambda": 134.2,
+    "updatepos": 154.2,
+    "vorticity_a": 203.0,
+    "vorticity_b": 191.9,
+    "update": 86.2,
     "neighbourcells": 60.6,
     "predict": 52.0,
 }
