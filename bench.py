@@ -193,11 +193,30 @@ def bench_adjoint(reps: int):
         tf += t1 - t0
         tb += t2 - t1
     r.close()
+    # the plain step (SPH::Run) on the same scene, for comparison
+    from pbf_amd import _lib
+    from pbf_amd import sph as P
+    lib = _lib.load()
+    h = P.SPH(sc.n, sc.grid)
+    h.set_params(P.scene_params(sc))
+    h.upload(sc.pos, sc.vel)
+    for _ in range(4):
+        h.Run()
+    _lib.check(lib.pbf_synchronize(h.stream()))
+    h.upload(sc.pos, sc.vel)
+    t0 = time.perf_counter()
+    for _ in range(reps * 32):
+        h.Run()
+    _lib.check(lib.pbf_synchronize(h.stream()))
+    plain = 1e3 * (time.perf_counter() - t0) / (reps * 32)
+    h.close()
     fwd = 1e3 * tf / (reps * 32)
     bwd = 1e3 * tb / (reps * 32)
-    log(f"[bench] adjoint workload: forward {fwd:.3f} ms/step, reverse {bwd:.3f} ms/step, ratio {bwd / fwd:.2f}")
+    log(f"[bench] adjoint workload: forward {fwd:.3f} ms/step, reverse {bwd:.3f} ms/step, ratio {bwd / fwd:.2f}; "
+        f"plain step {plain:.3f} ms")
     return {"workload": "16384 particles, 32 steps x 3 iterations, checkpoint every 8, FP32, epsilon 5",
-            "forward_ms_per_step": fwd, "adjoint_ms_per_step": bwd, "adjoint_over_forward": bwd / fwd}
+            "forward_ms_per_step": fwd, "adjoint_ms_per_step": bwd, "adjoint_over_forward": bwd / fwd,
+            "plain_step_ms": plain, "adjoint_over_plain_step": bwd / plain}
 
 
 def reset(sph, scene):

@@ -114,7 +114,7 @@ SIGNATURES = {
     "pbf_adjoint_create": ([C.POINTER(vp), u32, i32, i32, i32, pp, i32, i32, i32], C.c_int),
     "pbf_adjoint_destroy": ([vp], C.c_int),
     "pbf_adjoint_set_coefficients": ([vp, C.c_double, C.c_double], C.c_int),
-    "pbf_adjoint_forward": ([vp, vp, vp, C.c_int, C.c_int], C.c_int),
+    "pbf_adjoint_forward": ([vp, vp, vp, C.c_int], C.c_int),
     "pbf_adjoint_positions": ([vp, vp, C.c_int], C.c_int),
     "pbf_adjoint_velocities": ([vp, vp, C.c_int], C.c_int),
     "pbf_adjoint_backward": ([vp, vp, C.c_int, vp, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),

@@ -66,9 +66,9 @@ class Rollout:
         check(self._lib.pbf_adjoint_set_stream(self._h, stream_handle))
 
     # host (numpy) interface
-    def forward(self, pos4, vel4, keep_structure=False):
+    def forward(self, pos4, vel4):
         pos4, vel4 = self._arr(pos4), self._arr(vel4)
-        check(self._lib.pbf_adjoint_forward(self._h, pos4.ctypes.data, vel4.ctypes.data, 1, int(keep_structure)))
+        check(self._lib.pbf_adjoint_forward(self._h, pos4.ctypes.data, vel4.ctypes.data, 1))
         out = np.empty((self.n, 4), self.dtype)
         check(self._lib.pbf_adjoint_positions(self._h, out.ctypes.data, 1))
         return out
@@ -99,7 +99,7 @@ class Rollout:
     def forward_device(self, pos4, vel4):
         import torch
         torch.cuda.current_stream().synchronize()
-        check(self._lib.pbf_adjoint_forward(self._h, pos4.data_ptr(), vel4.data_ptr(), 0, 0))
+        check(self._lib.pbf_adjoint_forward(self._h, pos4.data_ptr(), vel4.data_ptr(), 0))
         out = torch.empty_like(pos4)
         check(self._lib.pbf_adjoint_positions(self._h, out.data_ptr(), 0))
         self._sync()

@@ -126,14 +126,59 @@ __device__ __forceinline__ void atomic_add3(T4<T> *p, int32_t j, T x, T y, T z) 
 }
 
 // FOR_EACH_NEIGHBOUR (foreachneighbour.glsl:1-10) over the fixed table:
-// body(j) for every candidate j != i of the 9 rows, in the reference's order.
-template <class F>
-__device__ __forceinline__ void each_candidate(const int32_t *__restrict__ nbr9, int32_t n, int32_t i, F &&body) {
+// body(j, p[j]) for every candidate j != i of the 9 rows, in the reference's
+// order, with the positions of eight candidates loaded together (indices past
+// n read the zero record). At the adjoint's sizes a launch has few waves, each
+// walking its candidates serially, so load latency rather than arithmetic
+// sets the time.
+constexpr int kAdjBatch = 8;
+
+template <class T, class F>
+__device__ __forceinline__ void each_candidate(const int32_t *__restrict__ nbr9, int32_t n, int32_t i,
+                                               const T4<T> *__restrict__ p, F &&body) {
     for (int o = 0; o < 9; o++) {
         const int32_t d = nbr9[(size_t)o * n + i];
         const int32_t s0 = d & 0xFFFFFF, e = d >> 24;
-        for (int32_t j = s0; j < s0 + e; j++)
-            if (j != i) body(j);
+        const int32_t last = max(min(s0 + e, n) - 1, 0);
+        for (int32_t q = 0; q < e; q += kAdjBatch) {
+            T4<T> pj[kAdjBatch];
+#pragma unroll
+            for (int u = 0; u < kAdjBatch; u++) {
+                const int32_t j = s0 + q + u;
+                pj[u] = p[min(j, last)];
+                if (j >= n) pj[u] = T4<T>{0, 0, 0, 0};
+            }
+#pragma unroll
+            for (int u = 0; u < kAdjBatch; u++) {
+                const int32_t j = s0 + q + u;
+                if (q + u < e && j != i) body(j, pj[u]);
+            }
+        }
+    }
+}
+
+// The reverse kernels run one lane per (particle, row) pair: thread t of the
+// launch takes row t / n of particle t % n (consecutive lanes, consecutive
+// particles of one row). The order of their scattered sums does not matter.
+template <class T, class F>
+__device__ __forceinline__ void row_candidates(const int32_t *__restrict__ nbr9, int32_t n, int32_t i, int32_t o,
+                                               const T4<T> *__restrict__ p, F &&body) {
+    const int32_t d = nbr9[(size_t)o * n + i];
+    const int32_t s0 = d & 0xFFFFFF, e = d >> 24;
+    const int32_t last = max(min(s0 + e, n) - 1, 0);
+    for (int32_t q = 0; q < e; q += kAdjBatch) {
+        T4<T> pj[kAdjBatch];
+#pragma unroll
+        for (int u = 0; u < kAdjBatch; u++) {
+            const int32_t j = s0 + q + u;
+            pj[u] = p[min(j, last)];
+            if (j >= n) pj[u] = T4<T>{0, 0, 0, 0};
+        }
+#pragma unroll
+        for (int u = 0; u < kAdjBatch; u++) {
+            const int32_t j = s0 + q + u;
+            if (q + u < e && j != i) body(j, pj[u]);
+        }
     }
 }
 
@@ -193,8 +238,7 @@ __global__ __launch_bounds__(AB) void ka_lambda(AConsts<T> c, const T4<T> *__res
     if (i >= c.n) return;
     const T4<T> pi = p[i];
     T rho = 0, sk = 0, gx = 0, gy = 0, gz = 0;
-    each_candidate(nbr9, c.n, i, [&](int32_t j) {
-        const T4<T> pj = load_nb(p, j, c.n);
+    each_candidate(nbr9, c.n, i, p, [&](int32_t j, const T4<T> &pj) {
         const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const T sq = sqlen(dx, dy, dz);
         const T r = sqrt(sq);
@@ -230,8 +274,7 @@ __global__ __launch_bounds__(AB) void ka_updatepos(AConsts<T> c, const T4<T> *__
     const T4<T> pi = p[i];
     const T li = lam[i];
     T dx = 0, dy = 0, dz = 0;
-    each_candidate(nbr9, c.n, i, [&](int32_t j) {
-        const T4<T> pj = load_nb(p, j, c.n);
+    each_candidate(nbr9, c.n, i, p, [&](int32_t j, const T4<T> &pj) {
         const T ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
         const T sq = sqlen(ex, ey, ez);
         const T r = sqrt(sq);
@@ -284,8 +327,7 @@ __global__ __launch_bounds__(AB) void ka_vort_a(AConsts<T> c, const T4<T> *__res
     if (i >= c.n) return;
     const T4<T> pi = p[i], vi = vs[i];
     T sx = 0, sy = 0, sz = 0, wx = 0, wy = 0, wz = 0;
-    each_candidate(nbr9, c.n, i, [&](int32_t j) {
-        const T4<T> pj = load_nb(p, j, c.n);
+    each_candidate(nbr9, c.n, i, p, [&](int32_t j, const T4<T> &pj) {
         const T ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
         const T sq = sqlen(ex, ey, ez);
         const T r = sqrt(sq);
@@ -313,13 +355,13 @@ template <class T>
 __global__ __launch_bounds__(AB) void ka_vort_b(AConsts<T> c, const uint32_t *__restrict__ perm,
                                                 const T4<T> *__restrict__ p, const int32_t *__restrict__ nbr9,
                                                 const T4<T> *__restrict__ vx, const T4<T> *__restrict__ om,
-                                                const T *__restrict__ o, T4<T> *__restrict__ v) {
+                                                const T *__restrict__ o, T4<T> *__restrict__ v,
+                                                T4<T> *__restrict__ eta_out) {
     const int32_t i = blockIdx.x * AB + threadIdx.x;
     if (i >= c.n) return;
     const T4<T> pi = p[i];
     T nx = 0, ny = 0, nz = 0;
-    each_candidate(nbr9, c.n, i, [&](int32_t j) {
-        const T4<T> pj = load_nb(p, j, c.n);
+    each_candidate(nbr9, c.n, i, p, [&](int32_t j, const T4<T> &pj) {
         const T ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
         const T sq = sqlen(ex, ey, ez);
         const T r = sqrt(sq);
@@ -331,6 +373,7 @@ __global__ __launch_bounds__(AB) void ka_vort_b(AConsts<T> c, const uint32_t *__
         ny += oj * gy;
         nz += oj * gz;
     });
+    if (eta_out) eta_out[i] = T4<T>{nx, ny, nz, 0};
     const T l = sqrt(sqlen(nx, ny, nz));
     if (l > (T)0) {
         nx = nx / l;
@@ -345,62 +388,47 @@ __global__ __launch_bounds__(AB) void ka_vort_b(AConsts<T> c, const uint32_t *__
 
 // ------------------------------------------------------------------ reverse
 // vorticity pass B reversed: from vbar (by id) to vxbar, ombar (own part),
-// obar and pbar (scattered), and d/d eps_v.
+// obar and pbar (scattered), and d/d eps_v; eta from the forward's tape.
 template <class T>
 __global__ __launch_bounds__(AB) void kr_vort_b(AConsts<T> c, const uint32_t *__restrict__ perm,
                                                 const T4<T> *__restrict__ p, const int32_t *__restrict__ nbr9,
                                                 const T4<T> *__restrict__ om, const T *__restrict__ o,
-                                                const T4<T> *__restrict__ vbar, T4<T> *__restrict__ vxbar,
-                                                T4<T> *__restrict__ ombar, T *__restrict__ obar,
-                                                T4<T> *__restrict__ pbar, double *__restrict__ acc) {
-    const int32_t i = blockIdx.x * AB + threadIdx.x;
+                                                const T4<T> *__restrict__ eta, const T4<T> *__restrict__ vbar,
+                                                T4<T> *__restrict__ vxbar, T4<T> *__restrict__ ombar,
+                                                T *__restrict__ obar, T4<T> *__restrict__ pbar,
+                                                double *__restrict__ acc) {
+    const int64_t tid = (int64_t)blockIdx.x * AB + threadIdx.x;
+    const int32_t row = (int32_t)(tid / c.n), i = (int32_t)(tid - (int64_t)row * c.n);
     double deps = 0.0;
-    if (i < c.n) {
+    if (row < 9) {
         const T4<T> pi = p[i];
-        // recompute eta
-        T nx = 0, ny = 0, nz = 0;
-        each_candidate(nbr9, c.n, i, [&](int32_t j) {
-            const T4<T> pj = load_nb(p, j, c.n);
-            const T ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
-            const T sq = sqlen(ex, ey, ez);
-            const T r = sqrt(sq);
-            if (r > c.h || r == (T)0) return;
-            const T oj = j < c.n ? o[j] : (T)0;
-            const T s = gspiky(c, r);
-            nx += oj * (s * ex);
-            ny += oj * (s * ey);
-            nz += oj * (s * ez);
-        });
-        const T l = sqrt(sqlen(nx, ny, nz));
+        const T4<T> et = eta[i];
+        const T l = sqrt(sqlen(et.x, et.y, et.z));
         T Nx = 0, Ny = 0, Nz = 0;
         if (l > (T)0) {
-            Nx = nx / l;
-            Ny = ny / l;
-            Nz = nz / l;
+            Nx = et.x / l;
+            Ny = et.y / l;
+            Nz = et.z / l;
         }
         const T4<T> w = om[i];
         const T4<T> a = vbar[perm[i]];
-        vxbar[i] = T4<T>{a.x, a.y, a.z, 0};
         const T sc = c.dt * c.vort_eps;
-        // d/d eps_v of a . (dt eps (N x w))
-        const T cx = Ny * w.z - Nz * w.y, cy = Nz * w.x - Nx * w.z, cz = Nx * w.y - Ny * w.x;
-        deps = (double)(c.dt * (a.x * cx + a.y * cy + a.z * cz));
-        // a . (N x w) = N . (w x a) = w . (a x N)
-        const T Nbx = sc * (w.y * a.z - w.z * a.y), Nby = sc * (w.z * a.x - w.x * a.z),
-                Nbz = sc * (w.x * a.y - w.y * a.x);
-        ombar[i] = T4<T>{sc * (a.y * Nz - a.z * Ny), sc * (a.z * Nx - a.x * Nz), sc * (a.x * Ny - a.y * Nx), 0};
-        // N = eta / |eta|: etabar = (Nbar - N (N . Nbar)) / |eta|
-        T ebx = 0, eby = 0, ebz = 0;
-        if (l > (T)0) {
-            const T nn = Nx * Nbx + Ny * Nby + Nz * Nbz;
-            ebx = (Nbx - Nx * nn) / l;
-            eby = (Nby - Ny * nn) / l;
-            ebz = (Nbz - Nz * nn) / l;
+        if (row == 0) {
+            vxbar[i] = T4<T>{a.x, a.y, a.z, 0};
+            // d/d eps_v of a . (dt eps (N x w))
+            const T cx = Ny * w.z - Nz * w.y, cy = Nz * w.x - Nx * w.z, cz = Nx * w.y - Ny * w.x;
+            deps = (double)(c.dt * (a.x * cx + a.y * cy + a.z * cz));
+            // a . (N x w) = w . (a x N)
+            ombar[i] = T4<T>{sc * (a.y * Nz - a.z * Ny), sc * (a.z * Nx - a.x * Nz), sc * (a.x * Ny - a.y * Nx), 0};
         }
+        // a . (N x w) = N . (w x a); N = eta / |eta|: etabar = (Nbar - N (N . Nbar)) / |eta|
         if (l > (T)0) {
+            const T Nbx = sc * (w.y * a.z - w.z * a.y), Nby = sc * (w.z * a.x - w.x * a.z),
+                    Nbz = sc * (w.x * a.y - w.y * a.x);
+            const T nn = Nx * Nbx + Ny * Nby + Nz * Nbz;
+            const T ebx = (Nbx - Nx * nn) / l, eby = (Nby - Ny * nn) / l, ebz = (Nbz - Nz * nn) / l;
             T pbx = 0, pby = 0, pbz = 0;
-            each_candidate(nbr9, c.n, i, [&](int32_t j) {
-                const T4<T> pj = load_nb(p, j, c.n);
+            row_candidates(nbr9, c.n, i, row, p, [&](int32_t j, const T4<T> &pj) {
                 const T ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
                 const T sq = sqlen(ex, ey, ez);
                 const T r = sqrt(sq);
@@ -447,15 +475,15 @@ __global__ __launch_bounds__(AB) void kr_vort_a(AConsts<T> c, const T4<T> *__res
                                                 const T4<T> *__restrict__ vxbar, const T4<T> *__restrict__ ombar,
                                                 T4<T> *__restrict__ vsbar, T4<T> *__restrict__ pbar,
                                                 double *__restrict__ acc) {
-    const int32_t i = blockIdx.x * AB + threadIdx.x;
+    const int64_t tid = (int64_t)blockIdx.x * AB + threadIdx.x;
+    const int32_t row = (int32_t)(tid / c.n), i = (int32_t)(tid - (int64_t)row * c.n);
     double dc = 0.0;
-    if (i < c.n) {
+    if (row < 9) {
         const T4<T> pi = p[i], vi = vs[i], a = vxbar[i], wb = ombar[i];
         T sx = 0, sy = 0, sz = 0;
-        T vbx = a.x, vby = a.y, vbz = a.z;  // vx = v + ...
+        T vbx = row == 0 ? a.x : (T)0, vby = row == 0 ? a.y : (T)0, vbz = row == 0 ? a.z : (T)0;  // vx = v + ...
         T pbx = 0, pby = 0, pbz = 0;
-        each_candidate(nbr9, c.n, i, [&](int32_t j) {
-            const T4<T> pj = load_nb(p, j, c.n);
+        row_candidates(nbr9, c.n, i, row, p, [&](int32_t j, const T4<T> &pj) {
             const T ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
             const T sq = sqlen(ex, ey, ez);
             const T r = sqrt(sq);
@@ -497,7 +525,7 @@ __global__ __launch_bounds__(AB) void kr_vort_a(AConsts<T> c, const T4<T> *__res
         });
         atomic_add3(vsbar, i, vbx, vby, vbz);
         atomic_add3(pbar, i, pbx, pby, pbz);
-        dc = (double)(a.x * sx + a.y * sy + a.z * sz);
+        dc = (double)(a.x * sx + a.y * sy + a.z * sz);  // this row's share of a . sum u W
     }
     block_add2(dc, 0.0, acc);
 }
@@ -529,18 +557,18 @@ __global__ __launch_bounds__(AB) void kr_updatepos(AConsts<T> c, const T4<T> *__
                                                    const T *__restrict__ lam, const int32_t *__restrict__ nbr9,
                                                    const uint8_t *__restrict__ mask, const T4<T> *__restrict__ pbn,
                                                    T4<T> *__restrict__ pbp, T *__restrict__ lambar) {
-    const int32_t i = blockIdx.x * AB + threadIdx.x;
-    if (i >= c.n) return;
+    const int64_t tid = (int64_t)blockIdx.x * AB + threadIdx.x;
+    const int32_t row = (int32_t)(tid / c.n), i = (int32_t)(tid - (int64_t)row * c.n);
+    if (row >= 9) return;
     const T4<T> pi = p[i];
     const uint8_t m = mask[i];
     const T4<T> b = pbn[i];
     const T ax = (m & 1) ? b.x : (T)0, ay = (m & 2) ? b.y : (T)0, az = (m & 4) ? b.z : (T)0;
-    T pbx = ax, pby = ay, pbz = az;
+    T pbx = row == 0 ? ax : (T)0, pby = row == 0 ? ay : (T)0, pbz = row == 0 ? az : (T)0;  // identity part once
     const T li = lam[i];
     T lb = 0;
     const T k4 = (T)-4 * c.k * c.scale * c.scale * c.scale * c.scale;
-    each_candidate(nbr9, c.n, i, [&](int32_t j) {
-        const T4<T> pj = load_nb(p, j, c.n);
+    row_candidates(nbr9, c.n, i, row, p, [&](int32_t j, const T4<T> &pj) {
         const T ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
         const T sq = sqlen(ex, ey, ez);
         const T r = sqrt(sq);
@@ -568,7 +596,7 @@ __global__ __launch_bounds__(AB) void kr_updatepos(AConsts<T> c, const T4<T> *__
         pbz += dz;
         if (j < c.n) atomic_add3(pbp, j, -dx, -dy, -dz);
     });
-    atomicAdd(&lambar[i], lb);
+    if (lb != (T)0) atomicAdd(&lambar[i], lb);
     atomic_add3(pbp, i, pbx, pby, pbz);
 }
 
@@ -578,8 +606,9 @@ __global__ __launch_bounds__(AB) void kr_lambda(AConsts<T> c, const T4<T> *__res
                                                 const int32_t *__restrict__ nbr9, const T *__restrict__ lam,
                                                 const T *__restrict__ sden, const T4<T> *__restrict__ G,
                                                 const T *__restrict__ lambar, T4<T> *__restrict__ pbar) {
-    const int32_t i = blockIdx.x * AB + threadIdx.x;
-    if (i >= c.n) return;
+    const int64_t tid = (int64_t)blockIdx.x * AB + threadIdx.x;
+    const int32_t row = (int32_t)(tid / c.n), i = (int32_t)(tid - (int64_t)row * c.n);
+    if (row >= 9) return;
     const T lb = lambar[i];
     if (lb == (T)0) return;
     const T4<T> pi = p[i];
@@ -589,8 +618,7 @@ __global__ __launch_bounds__(AB) void kr_lambda(AConsts<T> c, const T4<T> *__res
     const T sb = -lam[i] * lb / S;
     const T4<T> Gi = G[i];
     T pbx = 0, pby = 0, pbz = 0;
-    each_candidate(nbr9, c.n, i, [&](int32_t j) {
-        const T4<T> pj = load_nb(p, j, c.n);
+    row_candidates(nbr9, c.n, i, row, p, [&](int32_t j, const T4<T> &pj) {
         const T ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
         const T sq = sqlen(ex, ey, ez);
         const T r = sqrt(sq);
@@ -648,7 +676,7 @@ unsigned zblocks(size_t n) {
 // ------------------------------------------------------------------ host
 struct AdjBase {
     virtual ~AdjBase() {}
-    virtual int forward(const void *pos4, const void *vel4, int host, int keep) = 0;
+    virtual int forward(const void *pos4, const void *vel4, int host) = 0;
     virtual int positions(void *pos4, int host) = 0;
     virtual int velocities(void *vel4, int host) = 0;
     virtual int backward(const void *gpos4, int host, void *gvel4, double *gc, double *geps) = 0;
@@ -660,13 +688,13 @@ struct AdjBase {
 
 template <class T>
 struct StepTape {
-    uint32_t *perm = nullptr;
-    int32_t *nbr9 = nullptr;
+    const uint32_t *perm = nullptr;  // the step's structure (kept by the forward pass)
+    const int32_t *nbr9 = nullptr;
     T4<T> *p = nullptr;      // (I + 1) x n positions of the iterations (sorted)
     T *lam = nullptr, *sden = nullptr;  // I x n
     T4<T> *G = nullptr;      // I x n
     uint8_t *mask = nullptr; // I x n
-    T4<T> *vs = nullptr, *vx = nullptr, *om = nullptr;
+    T4<T> *vs = nullptr, *vx = nullptr, *om = nullptr, *eta = nullptr;
     T *o = nullptr;
 };
 
@@ -729,8 +757,6 @@ struct Adj : AdjBase {
         dfree(all_perm);
         dfree(all_nbr);
         for (auto &t : tape) {
-            dfree(t.perm);
-            dfree(t.nbr9);
             dfree(t.p);
             dfree(t.lam);
             dfree(t.sden);
@@ -740,6 +766,7 @@ struct Adj : AdjBase {
             dfree(t.vx);
             dfree(t.om);
             dfree(t.o);
+            dfree(t.eta);
         }
         dfree(xs);
         dfree(lam_tmp);
@@ -790,11 +817,10 @@ struct Adj : AdjBase {
         if ((e = dalloc(&all_perm, (size_t)steps * n)) || (e = dalloc(&all_nbr, (size_t)steps * 9 * n))) return e;
         tape.assign(every, StepTape<T>());
         for (auto &t : tape) {
-            if ((e = dalloc(&t.perm, n)) || (e = dalloc(&t.nbr9, (size_t)9 * n)) ||
-                (e = dalloc(&t.p, (size_t)(iters + 1) * n)) || (e = dalloc(&t.lam, (size_t)iters * n)) ||
+            if ((e = dalloc(&t.p, (size_t)(iters + 1) * n)) || (e = dalloc(&t.lam, (size_t)iters * n)) ||
                 (e = dalloc(&t.sden, (size_t)iters * n)) || (e = dalloc(&t.G, (size_t)iters * n)) ||
                 (e = dalloc(&t.mask, (size_t)iters * n)) || (e = dalloc(&t.vs, n)) || (e = dalloc(&t.vx, n)) ||
-                (e = dalloc(&t.om, n)) || (e = dalloc(&t.o, n)))
+                (e = dalloc(&t.om, n)) || (e = dalloc(&t.o, n)) || (e = dalloc(&t.eta, n)))
                 return e;
         }
         if ((e = dalloc(&xs, n)) || (e = dalloc(&lam_tmp, n))) return e;
@@ -877,7 +903,7 @@ struct Adj : AdjBase {
             T *o = tp ? tp->o : ob;
             // a zero record's velocity is that of id 0 (imageLoad (velocitytexture, 0))
             ka_vort_a<T><<<nb, AB, 0, st>>>(c, cur, nbr9, vs, v, vx, om, o);
-            ka_vort_b<T><<<nb, AB, 0, st>>>(c, perm, cur, nbr9, vx, om, o, v);
+            ka_vort_b<T><<<nb, AB, 0, st>>>(c, perm, cur, nbr9, vx, om, o, v, tp ? tp->eta : nullptr);
         }
     }
 
@@ -904,20 +930,20 @@ struct Adj : AdjBase {
         return host ? hipStreamSynchronize(st) : hipSuccess;
     }
 
-    int forward(const void *pos4, const void *vel4, int host, int keep) override {
+    int forward(const void *pos4, const void *vel4, int host) override {
         PBF_HIP(upload(x, v, pos4, vel4, host));
         // a fresh end table, as a new SPH handle's
         PBF_HIP(hipMemsetAsync(end64, 0, cells * 8, st));
         PBF_HIP(hipMemsetAsync(epoch, 0, 4, st));
+        // every step's structure is kept: the reverse pass recomputes the
+        // states and tapes of a segment from its checkpoint with it
         for (int32_t s = 0; s < steps; s++) {
             if (s % every == 0) PBF_HIP(save_ck(s / every));
-            uint32_t *perm = all_perm + (size_t)(keep ? s : 0) * n;
-            int32_t *nbr = all_nbr + (size_t)(keep ? s : 0) * 9 * n;
-            step(nullptr, perm, nbr, true);
+            step(nullptr, all_perm + (size_t)s * n, all_nbr + (size_t)s * 9 * n, true);
         }
         PBF_HIP(hipMemcpyAsync(xf, x, n * sizeof(T4<T>), hipMemcpyDeviceToDevice, st));
         PBF_HIP(hipMemcpyAsync(vf, v, n * sizeof(T4<T>), hipMemcpyDeviceToDevice, st));
-        have_all = keep != 0;
+        have_all = true;
         forwarded = true;
         PBF_HIP(hipGetLastError());
         return PBF_OK;
@@ -939,15 +965,16 @@ struct Adj : AdjBase {
     // Reverse pass of one step from its tape: (xb, vb) after the step -> (xb, vb) before it.
     void reverse_step(const StepTape<T> &t) {
         const unsigned nb = ablocks(n);
+        const unsigned nr = ablocks((size_t)9 * n);  // one lane per (particle, row)
         const T4<T> *pI = t.p + (size_t)iters * n;
         zero(reinterpret_cast<T *>(pb_a), 4 * (size_t)n);
         const T4<T> *vsbar = nullptr;
         if (c.vorticity) {
             zero(ob, n);
             zero(reinterpret_cast<T *>(vsb), 4 * (size_t)n);
-            kr_vort_b<T><<<nb, AB, 0, st>>>(c, t.perm, pI, t.nbr9, t.om, t.o, vb, vxb, omb, ob, pb_a, acc);
+            kr_vort_b<T><<<nr, AB, 0, st>>>(c, t.perm, pI, t.nbr9, t.om, t.o, t.eta, vb, vxb, omb, ob, pb_a, acc);
             kr_omega_norm<T><<<nb, AB, 0, st>>>((int32_t)n, t.om, t.o, ob, omb);
-            kr_vort_a<T><<<nb, AB, 0, st>>>(c, pI, t.nbr9, t.vs, vxb, omb, vsb, pb_a, acc);
+            kr_vort_a<T><<<nr, AB, 0, st>>>(c, pI, t.nbr9, t.vs, vxb, omb, vsb, pb_a, acc);
             vsbar = vsb;
         }
         kr_update<T><<<nb, AB, 0, st>>>(c, t.perm, xb, vb, vsbar, pb_a, xb2);
@@ -957,8 +984,8 @@ struct Adj : AdjBase {
             const T *lam = t.lam + (size_t)it * n;
             zero(reinterpret_cast<T *>(prev), 4 * (size_t)n);
             zero(lb, n);
-            kr_updatepos<T><<<nb, AB, 0, st>>>(c, p, lam, t.nbr9, t.mask + (size_t)it * n, next, prev, lb);
-            kr_lambda<T><<<nb, AB, 0, st>>>(c, p, t.nbr9, lam, t.sden + (size_t)it * n, t.G + (size_t)it * n, lb,
+            kr_updatepos<T><<<nr, AB, 0, st>>>(c, p, lam, t.nbr9, t.mask + (size_t)it * n, next, prev, lb);
+            kr_lambda<T><<<nr, AB, 0, st>>>(c, p, t.nbr9, lam, t.sden + (size_t)it * n, t.G + (size_t)it * n, lb,
                                             prev);
             std::swap(next, prev);
         }
@@ -979,7 +1006,11 @@ struct Adj : AdjBase {
             PBF_HIP(load_ck(k));
             for (int32_t s = s0; s < s1; s++) {
                 StepTape<T> &t = tape[s - s0];
-                step(&t, t.perm, t.nbr9, true);
+                uint32_t *perm = all_perm + (size_t)s * n;
+                int32_t *nbr = all_nbr + (size_t)s * 9 * n;
+                t.perm = perm;
+                t.nbr9 = nbr;
+                step(&t, perm, nbr, false);
             }
             for (int32_t s = s1 - 1; s >= s0; s--) reverse_step(tape[s - s0]);
         }
@@ -996,7 +1027,7 @@ struct Adj : AdjBase {
     }
 
     int replay(const void *pos4, const void *vel4, int host, void *pos_out4) override {
-        if (!have_all) return invalid("pbf_adjoint_replay: the last forward kept no structure");
+        if (!have_all) return invalid("pbf_adjoint_replay: no forward rollout");
         PBF_HIP(upload(x, v, pos4, vel4, host));
         for (int32_t s = 0; s < steps; s++) step(nullptr, all_perm + (size_t)s * n, all_nbr + (size_t)s * 9 * n, false);
         PBF_HIP(hipGetLastError());
@@ -1069,9 +1100,9 @@ int pbf_adjoint_set_coefficients(pbf_adjoint *a, double xsph_c, double vorticity
     return PBF_OK;
 }
 
-int pbf_adjoint_forward(pbf_adjoint *a, const void *pos4, const void *vel4, int host, int keep_structure) {
+int pbf_adjoint_forward(pbf_adjoint *a, const void *pos4, const void *vel4, int host) {
     if (!a || !pos4 || !vel4) return invalid("null argument");
-    return a->impl->forward(pos4, vel4, host, keep_structure);
+    return a->impl->forward(pos4, vel4, host);
 }
 
 int pbf_adjoint_positions(pbf_adjoint *a, void *pos4, int host) {

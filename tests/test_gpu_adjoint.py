@@ -65,7 +65,7 @@ def test_fp64_step_matches_oracle_replay(gpu_lib):
     sc = adjoint_scene()
     say(f"[adjoint fp64 step] n={sc.n}")
     r = A.Rollout.for_scene(sc, steps=1, checkpoint_every=1, fp64=True)
-    x = r.forward(sc.pos, sc.vel, keep_structure=True)
+    x = r.forward(sc.pos, sc.vel)
     cfg = O.cfg_for(sc)
     _, _, _, _, st = O.step(cfg, sc.pos, sc.vel, stages=True)
     perm = st["sorted"][:, 3].view(np.int32).copy()
@@ -82,7 +82,7 @@ def fd_check(sc, steps, every, ncomp, delta, delta_coef, seed=0):
     r = A.Rollout.for_scene(sc, steps=steps, checkpoint_every=every, fp64=True)
     c0, e0 = sc.params.xsph_viscosity_c, sc.params.vorticity_epsilon
     t = r.forward(sc.pos, tgt_v)
-    x = r.forward(sc.pos, sc.vel, keep_structure=True)
+    x = r.forward(sc.pos, sc.vel)
     gv, gc, ge = r.backward(mse_grad(x, t))
     rng = np.random.default_rng(seed)
     comps = rng.choice(sc.n * 3, ncomp, replace=False)
