@@ -212,7 +212,7 @@ typedef struct pbf_scene_desc {
 } pbf_scene_desc;
 
 /* Launch shapes the autotuner picks from: workgroup size of the neighbour
- * passes (128, 256 or 512), and workgroup size (64 .. 1024) and particles per
+ * passes (64, 128, 256 or 512), and workgroup size (64 .. 1024) and particles per
  * thread (1 .. 4) of the streaming passes (predict, cell table, update). */
 typedef struct pbf_batch_tuning {
     int32_t neighbour_block;

@@ -87,7 +87,7 @@ class Batch:
         check(self._lib.pbf_batch_get_tuning(self._h, C.byref(t)))
         return {"neighbour_block": t.neighbour_block, "stream_block": t.stream_block, "stream_ppt": t.stream_ppt}
 
-    def set_tuning(self, neighbour_block=256, stream_block=256, stream_ppt=1):
+    def set_tuning(self, neighbour_block=64, stream_block=256, stream_ppt=1):
         t = PbfBatchTuning(int(neighbour_block), int(stream_block), int(stream_ppt), 0.0)
         check(self._lib.pbf_batch_set_tuning(self._h, C.byref(t)))
 
@@ -95,14 +95,14 @@ class Batch:
         """Times every launch shape from the current state and keeps the fastest.
         Returns (best, trials) as dicts with step_ms."""
         best = PbfBatchTuning()
-        res = (PbfBatchTuning * 64)()
+        res = (PbfBatchTuning * 128)()
         cnt = C.c_int32()
-        check(self._lib.pbf_batch_autotune(self._h, int(steps_per_trial), C.byref(best), res, 64, C.byref(cnt)))
+        check(self._lib.pbf_batch_autotune(self._h, int(steps_per_trial), C.byref(best), res, 128, C.byref(cnt)))
 
         def d(t):
             return {"neighbour_block": t.neighbour_block, "stream_block": t.stream_block, "stream_ppt": t.stream_ppt,
                     "step_ms": t.step_ms}
-        return d(best), [d(res[k]) for k in range(min(cnt.value, 64))]
+        return d(best), [d(res[k]) for k in range(min(cnt.value, 128))]
 
     def end_tables(self) -> np.ndarray:
         out = np.empty(sum(self.cells), np.int32)

@@ -315,7 +315,7 @@ int pbf_batch_set_tuning(pbf_batch *b, const pbf_batch_tuning *t) {
     ls.sb = t->stream_block;
     ls.ppt = t->stream_ppt;
     if (!valid_shape(ls))
-        return invalid("tuning: neighbour_block 128/256/512, stream_block a power of two in 64..1024, ppt 1..4");
+        return invalid("tuning: neighbour_block 64/128/256/512, stream_block a power of two in 64..1024, ppt 1..4");
     if (b->stream) PBF_HIP(hipStreamSynchronize(b->stream));
     b->drop_graph();
     b->shape = ls;
@@ -382,7 +382,7 @@ int pbf_batch_autotune(pbf_batch *b, int32_t steps_per_trial, pbf_batch_tuning *
     LaunchShape best_shape = keep;
     double best_ms = 1e300;
     int32_t k = 0;
-    static const int nbs[3] = {128, 256, 512};
+    static const int nbs[4] = {64, 128, 256, 512};
     static const int sbs[5] = {64, 128, 256, 512, 1024};
     for (int nb : nbs)
         for (int sb : sbs)

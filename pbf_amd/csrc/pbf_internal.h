@@ -109,7 +109,7 @@ void launch_vorticity_b(const SolverConsts &c, const float4 *rec, const int32_t 
 // per thread (1 .. 4) of the streaming passes. The one-scene form uses the
 // default.
 struct LaunchShape {
-    int32_t nb = 256;
+    int32_t nb = 64;
     int32_t sb = 256;
     int32_t ppt = 1;
 };

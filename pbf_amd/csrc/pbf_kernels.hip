@@ -70,20 +70,23 @@ __device__ __forceinline__ uint32_t cell_hash(const Seg &s, float x, float y, fl
 __device__ __forceinline__ int32_t id_of(const float4 &r) { return __float_as_int(r.w); }
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
-// Renumbering them so that every XCD runs chunks of kXcdChunk consecutive
-// workgroups keeps a chunk's neighbour rows in one L2, while consecutive chunks
-// go to different XCDs, so a dense region of the sorted array (whose
-// particles have many more neighbours than the rest) is shared by all XCDs
-// instead of landing on one. Bijective on [0, gridDim.x): full groups of
-// 8 kXcdChunk workgroups are permuted, the remainder keeps its numbers.
-constexpr uint32_t kXcdChunk = 8;
+// Renumbering them so that every XCD runs chunks of kXcdChunkParticles
+// consecutive sorted particles keeps a chunk's neighbour rows in one L2, while
+// consecutive chunks go to different XCDs, so a dense region of the sorted
+// array (whose particles have many more neighbours than the rest) is shared by
+// all XCDs instead of landing on one. Bijective on [0, gridDim.x): full groups
+// of 8 chunks are permuted, the remainder keeps its numbers.
+constexpr uint32_t kXcdChunkParticles = 2048;
 
+template <int NB>
 __device__ __forceinline__ uint32_t xcd_block() {
+    constexpr uint32_t chunk = kXcdChunkParticles / NB;
+    static_assert(chunk >= 1 && chunk * NB == kXcdChunkParticles, "chunk of whole workgroups");
     const uint32_t nb = gridDim.x, b = blockIdx.x;
-    const uint32_t group = 8u * kXcdChunk;
+    const uint32_t group = 8u * chunk;
     if (b >= nb / group * group) return b;
     const uint32_t x = b % 8u, k = b / 8u;
-    return (k / kXcdChunk) * group + x * kXcdChunk + (k % kXcdChunk);
+    return (k / chunk) * group + x * chunk + (k % chunk);
 }
 
 // ----------------------------------------------------------------- predict
@@ -288,13 +291,19 @@ __device__ __forceinline__ float val_at(const float *__restrict__ a, const Seg &
 // within h. Evaluating the kernels under a per-candidate branch runs them for
 // the whole wave on nearly every candidate (some lane is almost always in
 // range), so each thread first scans its candidates with a cheap conservative
-// distance test, four loads in flight at a time (phase 1), and appends the
+// distance test, kBatch loads in flight at a time (phase 1), and appends the
 // survivors, in the reference's visiting order, to its own column of an LDS
 // list as 16-bit (row, offset) codes; phase 2 runs the exact arithmetic over
 // the list. A column that fills up is evaluated early and reused. Sums keep the
 // reference's order, and candidates with fl(sqrt(sq)) > h add exact zeros
 // there, so skipping them changes no bit of the result.
-constexpr int LCAP = 48;
+// LCAP entries per column: the LDS of a workgroup is (LCAP + 1) * 2 + 36 bytes
+// per thread, and LDS is what limits the resident waves per CU.
+#ifndef PBF_LIST_LDS
+#define PBF_LIST_LDS 48
+#endif
+constexpr int LCAP = PBF_LIST_LDS;
+static_assert(LCAP >= 16 && LCAP % 8 == 0, "list column must hold at least two batches");
 // Candidates (and listed neighbours) whose loads are issued together: a
 // particle in a dense clump walks hundreds of them serially, and its wave
 // alone can set the kernel's duration, so memory-level parallelism per lane
@@ -362,7 +371,12 @@ __device__ __forceinline__ void scan_neighbours(const SolverConsts &c, const Seg
         const int32_t last = max(min(s0 + e, sg.n) - 1, 0);
         const float4 *__restrict__ row = srec + s0;
         for (int32_t q = 0; q < e; q += kBatch) {
-            if (cnt > LCAP - kBatch) flush();
+            // All lanes flush together when one column is full: a per-lane
+            // branch would run the flush once for every subset of lanes that
+            // fills up at a different candidate, which in a dense clump is
+            // most of the wave's time. Each lane still evaluates its own list
+            // in order.
+            if (wave_any(cnt > LCAP - kBatch)) flush();
             const int32_t j0 = s0 + q;
             float4 p[kBatch];
 #pragma unroll
@@ -451,7 +465,7 @@ __global__ __launch_bounds__(NB) void k_lambda(SolverConsts c, Dom dom, const fl
                                                      uint16_t *__restrict__ list_entries,
                                                      uint16_t *__restrict__ list_count, int32_t list_cap) {
     __shared__ NbrLds<NB> lds;
-    const int32_t i = xcd_block() * NB + threadIdx.x;
+    const int32_t i = xcd_block<NB>() * NB + threadIdx.x;
     if (i >= c.n) return;
     const Seg sg = dom.at(i);
     const float4 pi = rec[i];
@@ -493,7 +507,7 @@ __global__ __launch_bounds__(NB) void k_updatepos(SolverConsts c, Dom dom, const
                                                         const uint16_t *__restrict__ list_count,
                                                         float4 *__restrict__ rec_out) {
     __shared__ NbrLds<NB> lds;
-    const int32_t i = xcd_block() * NB + threadIdx.x;
+    const int32_t i = xcd_block<NB>() * NB + threadIdx.x;
     if (i >= c.n) return;
     const Seg sg = dom.at(i);
     const float4 pi = rec_in[i];
@@ -555,7 +569,7 @@ __global__ __launch_bounds__(NB) void k_vorticity_a(SolverConsts c, Dom dom, con
                                                           float *__restrict__ vort, uint16_t *__restrict__ list_entries,
                                                           uint16_t *__restrict__ list_count, int32_t list_cap) {
     __shared__ NbrLds<NB> lds;
-    const int32_t i = xcd_block() * NB + threadIdx.x;
+    const int32_t i = xcd_block<NB>() * NB + threadIdx.x;
     if (i >= c.n) return;
     const Seg sg = dom.at(i);
     const float4 pi = rec[i];
@@ -602,7 +616,7 @@ __global__ __launch_bounds__(NB) void k_vorticity_b(SolverConsts c, Dom dom, con
                                                           const float *__restrict__ vort,
                                                           float4 *__restrict__ vel_id) {
     __shared__ NbrLds<NB> lds;
-    const int32_t i = xcd_block() * NB + threadIdx.x;
+    const int32_t i = xcd_block<NB>() * NB + threadIdx.x;
     if (i >= c.n) return;
     const Seg sg = dom.at(i);
     const float4 pi = rec[i];
@@ -701,7 +715,7 @@ Seg seg_of(const SolverConsts &c) {
 }
 
 bool valid_shape(const LaunchShape &ls) {
-    const bool nb = ls.nb == 128 || ls.nb == 256 || ls.nb == 512;
+    const bool nb = ls.nb == 64 || ls.nb == 128 || ls.nb == 256 || ls.nb == 512;
     const bool sb = ls.sb >= 64 && ls.sb <= 1024 && (ls.sb & (ls.sb - 1)) == 0;
     return nb && sb && ls.ppt >= 1 && ls.ppt <= 4;
 }
@@ -718,18 +732,23 @@ unsigned stream_blocks(int32_t n, const LaunchShape &ls) {
 unsigned nb_blocks(int32_t n, int nb) { return (unsigned)((n + nb - 1) / nb); }
 
 // Calls f with the neighbour workgroup size and the padding flag as
-// compile-time constants: one scene runs 256-wide, padded when its arrays are
+// compile-time constants: one scene runs PBF_NB_ONE-wide, padded when its arrays are
 // (the SPH handle's are, caller buffers of the stage entry points are not);
 // many scenes take the tuned width, unpadded.
+#ifndef PBF_NB_ONE
+#define PBF_NB_ONE 64
+#endif
+static_assert(PBF_NB_ONE == 64 || PBF_NB_ONE == 128 || PBF_NB_ONE == 256, "neighbour workgroup width");
 template <class Dom, class F>
 void with_nb(int nb, bool pad, F &&f) {
     if constexpr (std::is_same<Dom, DomOne>::value) {
         if (pad)
-            f(std::integral_constant<int, 256>(), std::true_type());
+            f(std::integral_constant<int, PBF_NB_ONE>(), std::true_type());
         else
-            f(std::integral_constant<int, 256>(), std::false_type());
+            f(std::integral_constant<int, PBF_NB_ONE>(), std::false_type());
     } else {
         switch (nb) {
+            case 64: f(std::integral_constant<int, 64>(), std::false_type()); break;
             case 128: f(std::integral_constant<int, 128>(), std::false_type()); break;
             case 512: f(std::integral_constant<int, 512>(), std::false_type()); break;
             default: f(std::integral_constant<int, 256>(), std::false_type()); break;

@@ -96,7 +96,7 @@ def test_autotune_restores_state_and_keeps_results(gpu_lib):
     e0 = tuned.end_tables()
     best, trials = tuned.autotune(1)
     say(f"[autotune] best {best}, {len(trials)} trials")
-    assert len(trials) == 60 and all(t["step_ms"] > 0 for t in trials)
+    assert len(trials) == 80 and all(t["step_ms"] > 0 for t in trials)
     p1, v1 = tuned.download()
     assert same(p0, p1) and same(v0, v1) and np.array_equal(e0, tuned.end_tables())
     assert tuned.tuning() == {k: best[k] for k in ("neighbour_block", "stream_block", "stream_ppt")}
