@@ -210,6 +210,7 @@ struct pbf_sph {
     float4 *rec_a = nullptr, *rec_b = nullptr;   // sorted records of the solver iterations (Jacobi)
     float4 *rec = nullptr;                       // sorted records after the last iteration (borrowed)
     uint16_t *list_entries = nullptr, *list_count = nullptr;  // neighbours within h, lambda -> update
+    DenseList dense;                             // the step's particles with the most candidates
     float4 *vel_sorted = nullptr, *vxsph = nullptr, *omega = nullptr;
     float *lambda = nullptr, *vort = nullptr;
     uint32_t *keys = nullptr, *perm = nullptr;
@@ -230,6 +231,9 @@ struct pbf_sph {
         rec = nullptr;
         dfree(list_entries);
         dfree(list_count);
+        dfree(dense.cand);
+        dfree(dense.levels);
+        dfree(dense.threshold);
         dfree(vel_sorted);
         dfree(vxsph);
         dfree(omega);
@@ -368,6 +372,10 @@ int pbf_sph_create(pbf_sph **out, uint32_t n, int32_t gx, int32_t gy, int32_t gz
     s->rec = s->rec0;
     TRY(dalloc(&s->list_entries, (size_t)kListCap * n));
     TRY(dalloc(&s->list_count, n));
+    TRY(dalloc(&s->dense.cand, n));
+    TRY(dalloc(&s->dense.levels, (size_t)kDenseLevels * kDenseCopies));
+    TRY(dalloc(&s->dense.threshold, 1));
+    TRY(hipMemset(s->dense.levels, 0, (size_t)kDenseLevels * kDenseCopies * sizeof(uint32_t)));
     TRY(dalloc(&s->vel_sorted, n));
     TRY(dalloc(&s->vxsph, n));
     TRY(dalloc(&s->omega, n));
@@ -455,9 +463,9 @@ int pbf_sph_run(pbf_sph *s) {
     PBF_HIP(hipEventRecord(s->ev[2], st));
     // neighbour cells (src/SPH.cpp:270-275; src/NeighbourCellFinder.cpp:113-148)
     CellTables &t = s->cells_t;
-    TIMED(KID_CLEAR_START, launch_begin_cells(t.start, t.cells, t.epoch, st));
+    TIMED(KID_CLEAR_START, launch_begin_cells(t.start, t.cells, t.epoch, st, &s->dense));
     TIMED(KID_FINDCELLS, launch_findcells(c, s->rec0, t.start, t.end64, t.epoch, st));
-    TIMED(KID_NEIGHBOURCELLS, launch_neighbourcells(c, s->rec0, t.start, t.end64, t.nbr9, st));
+    TIMED(KID_NEIGHBOURCELLS, launch_neighbourcells(c, s->rec0, t.start, t.end64, t.nbr9, st, &s->dense));
     PBF_HIP(hipEventRecord(s->ev[3], st));
     // solver iterations (src/SPH.cpp:303-311), race-free: x* -> a -> b -> a ...
     PairList pl;
@@ -467,8 +475,8 @@ int pbf_sph_run(pbf_sph *s) {
     const float4 *cur = s->rec0;
     for (uint32_t it = 0; it < s->params.iterations; it++) {
         float4 *dst = (it & 1) ? s->rec_b : s->rec_a;
-        TIMED(KID_LAMBDA, launch_lambda(c, cur, t.nbr9, s->lambda, pl, st, true));
-        TIMED(KID_UPDATEPOS, launch_updatepos(c, cur, s->lambda, t.nbr9, pl, dst, st, true));
+        TIMED(KID_LAMBDA, launch_lambda(c, cur, t.nbr9, s->lambda, pl, st, true, &s->dense));
+        TIMED(KID_UPDATEPOS, launch_updatepos(c, cur, s->lambda, t.nbr9, pl, dst, st, true, &s->dense));
         cur = dst;
     }
     s->rec = const_cast<float4 *>(cur);
@@ -477,9 +485,10 @@ int pbf_sph_run(pbf_sph *s) {
     TIMED(KID_UPDATE, launch_update(c, s->rec, s->pos, s->vel, s->params.vorticity ? s->vel_sorted : nullptr, st));
     if (s->params.vorticity) {
         TIMED(KID_VORTICITY_A,
-              launch_vorticity_a(c, s->rec, t.nbr9, s->vel_sorted, s->vel, s->vxsph, s->omega, s->vort, pl, st, true));
+              launch_vorticity_a(c, s->rec, t.nbr9, s->vel_sorted, s->vel, s->vxsph, s->omega, s->vort, pl, st, true,
+                                 &s->dense));
         TIMED(KID_VORTICITY_B,
-              launch_vorticity_b(c, s->rec, t.nbr9, pl, s->vxsph, s->omega, s->vort, s->vel, st, true));
+              launch_vorticity_b(c, s->rec, t.nbr9, pl, s->vxsph, s->omega, s->vort, s->vel, st, true, &s->dense));
     }
 #undef TIMED
     PBF_HIP(hipEventRecord(s->ev[5], st));

@@ -68,11 +68,15 @@ void launch_keys_of_records(const SolverConsts &c, const float4 *rec, uint32_t *
 void launch_fill_i32(int32_t *p, int32_t value, size_t count, hipStream_t st);
 // FindNeighbourCells' clear of the start table (src/NeighbourCellFinder.cpp:
 // 115-126), which also advances the device-side tag of the step's end writes.
-void launch_begin_cells(int32_t *start, size_t cells, uint32_t *epoch, hipStream_t st);
+struct DenseList;
+void launch_begin_cells(int32_t *start, size_t cells, uint32_t *epoch, hipStream_t st,
+                        const DenseList *dl = nullptr);
 void launch_findcells(const SolverConsts &c, const float4 *sorted, int32_t *start, unsigned long long *end64,
                       const uint32_t *epoch, hipStream_t st);
+// dl: also list the step's dense particles and count the dense levels.
 void launch_neighbourcells(const SolverConsts &c, const float4 *sorted, const int32_t *start,
-                           const unsigned long long *end64, int32_t *nbr9, hipStream_t st);
+                           const unsigned long long *end64, int32_t *nbr9, hipStream_t st,
+                           const DenseList *dl = nullptr);
 // Per-particle list of the neighbours within h at one set of positions:
 // written by the lambda pass (vorticity pass A), replayed by the position
 // update (vorticity pass B). entries [cap][n] of 16-bit (row, offset) codes.
@@ -83,7 +87,7 @@ constexpr int32_t kListCap = 128;
 
 // Zero entries kept after the n particles of the one-scene solver's sorted
 // records, lambda and |omega| arrays: a neighbour row starts below n and holds
-// fewer than 128 entries, scanned in batches of 4, so no index reaches n + kPad.
+// fewer than 128 entries, scanned in batches of at most 130, so no index reaches n + kPad.
 constexpr int32_t kPad = 256;
 
 struct PairList {
@@ -91,19 +95,44 @@ struct PairList {
     uint16_t *count = nullptr;
     int32_t cap = 0;
 };
+// The particles of a step with the most neighbour candidates, handed from
+// the one-thread-per-particle path to the dense-particle workgroups of the same
+// launches. Each step, the neighbour-cell pass keeps every particle's
+// candidate count in cand and counts the particles above each of kDenseLevels
+// candidate levels (levels[level * kDenseCopies + copy], cleared by
+// launch_begin_cells); launch_dense_select then sets threshold to the lowest
+// level with at most kDenseBudget particles above it (the highest level if
+// none). Which particles are dense changes no result: both paths compute
+// every particle bit for bit alike.
+constexpr int kDenseLevels = 8;
+constexpr int kDenseCopies = 256;
+#ifndef PBF_DENSE_BUDGET
+#define PBF_DENSE_BUDGET 16384
+#endif
+constexpr uint32_t kDenseBudget = PBF_DENSE_BUDGET;
+struct DenseList {
+    uint16_t *cand = nullptr;        // n entries
+    uint32_t *levels = nullptr;      // kDenseLevels * kDenseCopies words
+    int32_t *threshold = nullptr;    // 1 word
+};
+void launch_dense_select(const DenseList &dl, hipStream_t st);
 // padded: rec, lambda and vort hold kPad zero entries after the n particles.
+// dl: the step's dense list, whose particles the launch's first workgroups
+// handle, kDenseGroup lanes per particle. Null: every particle takes the
+// one-thread-per-particle path.
 void launch_lambda(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, float *lambda, const PairList &pl,
-                   hipStream_t st, bool padded = false);
+                   hipStream_t st, bool padded = false, const DenseList *dl = nullptr);
 void launch_updatepos(const SolverConsts &c, const float4 *rec_in, const float *lambda, const int32_t *nbr9,
-                      const PairList &pl, float4 *rec_out, hipStream_t st, bool padded = false);
+                      const PairList &pl, float4 *rec_out, hipStream_t st, bool padded = false,
+                      const DenseList *dl = nullptr);
 void launch_update(const SolverConsts &c, const float4 *rec, float4 *pos, float4 *vel, float4 *vel_sorted,
                    hipStream_t st);
 void launch_vorticity_a(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, const float4 *vel_sorted,
                         const float4 *vel_id, float4 *vxsph, float4 *omega, float *vort, const PairList &pl,
-                        hipStream_t st, bool padded = false);
+                        hipStream_t st, bool padded = false, const DenseList *dl = nullptr);
 void launch_vorticity_b(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, const PairList &pl,
                         const float4 *vxsph, const float4 *omega, const float *vort, float4 *vel_id, hipStream_t st,
-                        bool padded = false);
+                        bool padded = false, const DenseList *dl = nullptr);
 // Launch shape of the batched form: workgroup size of the neighbour passes
 // (128, 256, 512), workgroup size (64 .. 1024, a power of two) and particles
 // per thread (1 .. 4) of the streaming passes. The one-scene form uses the

@@ -78,11 +78,11 @@ __device__ __forceinline__ int32_t id_of(const float4 &r) { return __float_as_in
 // of 8 chunks are permuted, the remainder keeps its numbers.
 constexpr uint32_t kXcdChunkParticles = 2048;
 
+// b: the workgroup's number among nb (counted from a multiple of 8).
 template <int NB>
-__device__ __forceinline__ uint32_t xcd_block() {
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
     constexpr uint32_t chunk = kXcdChunkParticles / NB;
     static_assert(chunk >= 1 && chunk * NB == kXcdChunkParticles, "chunk of whole workgroups");
-    const uint32_t nb = gridDim.x, b = blockIdx.x;
     const uint32_t group = 8u * chunk;
     if (b >= nb / group * group) return b;
     const uint32_t x = b % 8u, k = b / 8u;
@@ -164,11 +164,16 @@ __global__ __launch_bounds__(BLOCK) void k_fill_i32(int32_t *__restrict__ p, int
     }
 }
 
-// The start table cleared to -1, and the tag of this step's end writes
-// advanced by one (read by the cell finder that follows on the stream).
+// The start table cleared to -1, the tag of this step's end writes advanced
+// by one (read by the cell finder that follows on the stream), and the dense
+// level counts emptied.
 __global__ __launch_bounds__(BLOCK) void k_begin_cells(int32_t *__restrict__ start, size_t count,
-                                                       uint32_t *__restrict__ epoch) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) epoch[0] = epoch[0] + 1u;
+                                                       uint32_t *__restrict__ epoch, DenseList dl) {
+    if (blockIdx.x == 0) {
+        if (threadIdx.x == 0) epoch[0] = epoch[0] + 1u;
+        if (dl.levels)
+            for (int k = (int)threadIdx.x; k < kDenseLevels * kDenseCopies; k += BLOCK) dl.levels[k] = 0u;
+    }
     const size_t stride = (size_t)gridDim.x * BLOCK * 4;
     for (size_t i = ((size_t)blockIdx.x * BLOCK + threadIdx.x) * 4; i < count; i += stride) {
         if (i + 4 <= count) {
@@ -231,11 +236,12 @@ __device__ __forceinline__ bool in_grid(const Seg &s, int32_t x, int32_t y, int3
 // neighbourcells.glsl:52-92: per particle, 9 (dy,dz) rows merging x-1..x+1;
 // texel fetches outside the grid read 0 (as llvmpipe does).
 template <class Dom>
-__device__ __forceinline__ void neighbourcells_one(const SolverConsts &c, const Dom &dom, int32_t i,
-                                                   const float4 *__restrict__ sorted,
-                                                   const int32_t *__restrict__ start,
-                                                   const unsigned long long *__restrict__ end64,
-                                                   int32_t *__restrict__ nbr9) {
+__device__ __forceinline__ int32_t neighbourcells_one(const SolverConsts &c, const Dom &dom, int32_t i,
+                                                      const float4 *__restrict__ sorted,
+                                                      const int32_t *__restrict__ start,
+                                                      const unsigned long long *__restrict__ end64,
+                                                      int32_t *__restrict__ nbr9) {
+    int32_t cand = 0;
     const Seg sg = dom.at(i);
     const float4 p = sorted[i];
     const int32_t px = (int32_t)p.x, py = (int32_t)p.y, pz = (int32_t)p.z;  // ivec3 (pos): truncation
@@ -256,17 +262,78 @@ __device__ __forceinline__ void neighbourcells_one(const SolverConsts &c, const 
                 entries += e - s;
             }
         }
-        nbr9[(size_t)o * c.n + i] = (int32_t)((uint32_t)cell + ((uint32_t)entries << 24));
+        const int32_t word = (int32_t)((uint32_t)cell + ((uint32_t)entries << 24));
+        nbr9[(size_t)o * c.n + i] = word;
+        cand += max(word >> 24, 0);
     }
+    return cand;
 }
 
+// Candidate levels of the dense-particle selection.
+__constant__ int32_t kDenseLevelValue[kDenseLevels] = {24, 32, 48, 64, 96, 128, 192, 256};
+
+// Also keeps each particle's candidate count and counts the particles above
+// each dense level: per workgroup in LDS, then one atomic per level into one
+// of kDenseCopies words (few workgroups per word: atomics on one word
+// serialize).
 template <class Dom>
 __global__ __launch_bounds__(1024) void k_neighbourcells(SolverConsts c, Dom dom, int32_t ppt,
                                                          const float4 *__restrict__ sorted,
                                                          const int32_t *__restrict__ start,
                                                          const unsigned long long *__restrict__ end64,
-                                                         int32_t *__restrict__ nbr9) {
-    PBF_STREAM_FOR(i) neighbourcells_one(c, dom, i, sorted, start, end64, nbr9);
+                                                         int32_t *__restrict__ nbr9, DenseList dl) {
+    __shared__ uint32_t above[kDenseLevels];
+    if (threadIdx.x < kDenseLevels) above[threadIdx.x] = 0u;
+    __syncthreads();
+    PBF_STREAM_FOR(i) {
+        const int32_t cand = neighbourcells_one(c, dom, i, sorted, start, end64, nbr9);
+        if (dl.cand) {
+            dl.cand[i] = (uint16_t)cand;  // at most 9 * 127
+            const int lane = (int)(threadIdx.x & 63u);
+            const int leader = __builtin_ctzll(__ballot(1));
+#pragma unroll
+            for (int l = 0; l < kDenseLevels; l++) {
+                const uint64_t m = __ballot(cand > kDenseLevelValue[l]);
+                if (lane == leader && m) atomicAdd(&above[l], (uint32_t)__popcll(m));
+            }
+        }
+    }
+    __syncthreads();
+    if (dl.cand && threadIdx.x < kDenseLevels && above[threadIdx.x])
+        atomicAdd(&dl.levels[threadIdx.x * kDenseCopies + blockIdx.x % kDenseCopies], above[threadIdx.x]);
+}
+
+// The step's dense threshold: the lowest level with at most kDenseBudget
+// particles above it, else the highest level. One workgroup of
+// kDenseSelectThreads sums the copies of each level.
+constexpr int kDenseSelectThreads = 1024;
+__global__ __launch_bounds__(kDenseSelectThreads) void k_dense_select(DenseList dl) {
+    constexpr int per = kDenseSelectThreads / kDenseLevels;
+    static_assert(kDenseCopies % per == 0, "copies split evenly over the threads of a level");
+    __shared__ uint32_t part[kDenseLevels][per];
+    __shared__ uint32_t above[kDenseLevels];
+    const int t = threadIdx.x;
+    const int l = t / per, k0 = t % per;
+    uint32_t sum = 0;
+    for (int k = k0; k < kDenseCopies; k += per) sum += dl.levels[l * kDenseCopies + k];
+    part[l][k0] = sum;
+    __syncthreads();
+    if (t < kDenseLevels) {
+        uint32_t tot = 0;
+        for (int k = 0; k < per; k++) tot += part[t][k];
+        above[t] = tot;
+    }
+    __syncthreads();
+    if (t == 0) {
+        int32_t th = kDenseLevelValue[kDenseLevels - 1];
+        for (int l = 0; l < kDenseLevels; l++) {
+            if (above[l] <= kDenseBudget) {
+                th = kDenseLevelValue[l];
+                break;
+            }
+        }
+        dl.threshold[0] = th;
+    }
 }
 
 // ----------------------------------------------------------------- solver
@@ -297,23 +364,29 @@ __device__ __forceinline__ float val_at(const float *__restrict__ a, const Seg &
 // the list. A column that fills up is evaluated early and reused. Sums keep the
 // reference's order, and candidates with fl(sqrt(sq)) > h add exact zeros
 // there, so skipping them changes no bit of the result.
-// LCAP entries per column: the LDS of a workgroup is (LCAP + 1) * 2 + 36 bytes
+// LCAP entries per column: the LDS of a workgroup is (LCAP + 1) * 2 + 45 bytes
 // per thread, and LDS is what limits the resident waves per CU.
 #ifndef PBF_LIST_LDS
 #define PBF_LIST_LDS 48
 #endif
 constexpr int LCAP = PBF_LIST_LDS;
-static_assert(LCAP >= 16 && LCAP % 8 == 0, "list column must hold at least two batches");
+static_assert(LCAP % 8 == 0, "list column of whole batches");
 // Candidates (and listed neighbours) whose loads are issued together: a
 // particle in a dense clump walks hundreds of them serially, and its wave
 // alone can set the kernel's duration, so memory-level parallelism per lane
 // matters more than registers here.
-constexpr int kBatch = 8;
+#ifndef PBF_BATCH
+#define PBF_BATCH 8
+#endif
+constexpr int kBatch = PBF_BATCH;
+static_assert(LCAP >= 2 * kBatch, "list column must hold at least two batches");
+static_assert(126 + kBatch <= kPad, "batches past a row stay inside the zero padding");
 
 template <int NB>
 struct NbrLds {
     uint16_t list[(LCAP + 1) * NB];  // row LCAP: sink of the branch-free appends
     int32_t rowstart[9 * NB];
+    int8_t rowcount[9 * NB];  // start + (count << 24) >> 24: in [-128, 127]
 };
 
 // True when the candidate may satisfy sq <= h2cut: the fused multiply-add
@@ -323,17 +396,43 @@ __device__ __forceinline__ bool maybe_near(const SolverConsts &c, const float4 &
     return fmaf(dx, dx, fmaf(dy, dy, dz * dz)) <= c.h2loose;
 }
 
-// Full candidate scan of global slot i (scene-local index li); body(j, code,
-// rec[j], extra(j)) for every candidate j (scene-local) that may lie within h,
-// in visiting order. Appends go to the slot after the last kept one whatever
-// the test gives, and the count only advances for kept candidates. The kept
-// candidates are evaluated four at a time with their loads issued together,
-// so a particle with hundreds of neighbours waits for one round trip per four
-// of them rather than per neighbour.
+// Row word o of particle i: start + (count << 24). The count is the
+// reference's wrapped one, read back with an arithmetic shift.
+__device__ __forceinline__ int32_t row_count(int32_t word) { return word >> 24; }
+__device__ __forceinline__ int32_t row_start(int32_t word) { return word & 0xFFFFFF; }
+
+// The nine row words of slot i, loaded together (one round trip rather than
+// one per row ahead of that row's candidates) into the thread's LDS columns;
+// returns the number of candidates (the sum of the positive counts).
+template <int NB>
+__device__ __forceinline__ int32_t load_rows(const SolverConsts &c, const int32_t *__restrict__ nbr9, int32_t i,
+                                             NbrLds<NB> &lds) {
+    const int t = threadIdx.x;
+    int32_t d[9];
+#pragma unroll
+    for (int o = 0; o < 9; o++) d[o] = nbr9[(size_t)o * c.n + i];
+    int32_t cand = 0;
+#pragma unroll
+    for (int o = 0; o < 9; o++) {
+        lds.rowstart[o * NB + t] = row_start(d[o]);
+        lds.rowcount[o * NB + t] = (int8_t)row_count(d[o]);
+        cand += max(row_count(d[o]), 0);
+    }
+    return cand;
+}
+
+// Full candidate scan of global slot i (scene-local index li) over the rows
+// that load_rows put in LDS; body(j, code, rec[j], extra(j)) for every
+// candidate j (scene-local) that may lie within h, in visiting order. Appends
+// go to the slot after the last kept one whatever the test gives, and the
+// count only advances for kept candidates. The kept candidates are evaluated
+// kBatch at a time with their loads issued together, so a particle with
+// hundreds of neighbours waits for one round trip per batch rather than per
+// neighbour.
 template <int NB, bool PAD, class Extra, class Body>
 __device__ __forceinline__ void scan_neighbours(const SolverConsts &c, const Seg &sg, const float4 *__restrict__ rec,
-                                                const int32_t *__restrict__ nbr9, int32_t i, const float4 &pi,
-                                                NbrLds<NB> &lds, Extra &&extra, Body &&body) {
+                                                int32_t i, const float4 &pi, NbrLds<NB> &lds, Extra &&extra,
+                                                Body &&body) {
     const int t = threadIdx.x;
     const int32_t li = i - sg.off;
     const float4 *__restrict__ srec = rec + sg.off;
@@ -361,11 +460,10 @@ __device__ __forceinline__ void scan_neighbours(const SolverConsts &c, const Seg
         cnt = 0;
     };
     for (int o = 0; o < 9; o++) {
-        const int32_t d = nbr9[(size_t)o * c.n + i];
-        const int32_t s0 = d & 0xFFFFFF, e = d >> 24;
-        lds.rowstart[o * NB + t] = s0;
-        // Loads stay in bounds: with PAD, a row ends at most 134 entries past
-        // n (start < n, count < 128, batches of 8), inside the zero padding;
+        const int32_t s0 = lds.rowstart[o * NB + t], e = (int32_t)lds.rowcount[o * NB + t];
+        // Loads stay in bounds: with PAD, a row's batches end at most
+        // 126 + kBatch entries past n (start < n, count < 128), inside the
+        // zero padding;
         // without, indices past the row or past the scene repeat the last valid
         // one, and candidates past the scene are the zero record.
         const int32_t last = max(min(s0 + e, sg.n) - 1, 0);
@@ -425,25 +523,32 @@ struct PairListOut {
 // path, the loads of four entries are in flight together.
 template <int NB, bool PAD, class Extra, class Body>
 __device__ __forceinline__ void replay_neighbours(const SolverConsts &c, const Seg &sg, const float4 *__restrict__ rec,
-                                                  const int32_t *__restrict__ nbr9, const uint16_t *__restrict__ entries,
+                                                  const uint16_t *__restrict__ entries,
                                                   const uint16_t *__restrict__ count, int32_t i, const float4 &pi,
                                                   NbrLds<NB> &lds, Extra &&extra, Body &&body) {
     const int t = threadIdx.x;
     const uint32_t cnt = entries ? count[i] : LIST_NONE;
     if (cnt == LIST_NONE) {
-        scan_neighbours<NB, PAD>(c, sg, rec, nbr9, i, pi, lds, extra,
+        scan_neighbours<NB, PAD>(c, sg, rec, i, pi, lds, extra,
                                  [&](int32_t j, uint32_t, const float4 &pj, auto x) { body(j, pj, x, false); });
         return;
     }
-    for (int o = 0; o < 9; o++) lds.rowstart[o * NB + t] = nbr9[(size_t)o * c.n + i] & 0xFFFFFF;
     using X = decltype(extra(0));
+    // The entries of the next batch are requested before this batch's
+    // neighbours, so each batch waits for one round trip rather than two.
+    uint32_t next[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; u++) next[u] = (uint32_t)u < cnt ? entries[(size_t)u * c.n + i] : 0u;
     for (uint32_t k = 0; k < cnt; k += kBatch) {
         uint32_t e[kBatch];
         int32_t j[kBatch];
         float4 pj[kBatch];
         X x[kBatch];
 #pragma unroll
-        for (int u = 0; u < kBatch; u++) e[u] = k + u < cnt ? entries[(size_t)(k + u) * c.n + i] : 0u;
+        for (int u = 0; u < kBatch; u++) e[u] = next[u];
+#pragma unroll
+        for (int u = 0; u < kBatch; u++)
+            next[u] = k + kBatch + u < cnt ? entries[(size_t)(k + kBatch + u) * c.n + i] : 0u;
 #pragma unroll
         for (int u = 0; u < kBatch; u++) j[u] = lds.rowstart[(e[u] >> 8) * NB + t] + (int32_t)(e[u] & 255u);
 #pragma unroll
@@ -458,20 +563,410 @@ __device__ __forceinline__ void replay_neighbours(const SolverConsts &c, const S
     }
 }
 
+// ----------------------------------------------------------------- dense particles
+// A particle in a dense clump has hundreds of candidates, and its thread walks
+// them serially: one such wave, running alone at the end of a launch, sets the
+// launch's duration (tens of microseconds while the rest of the launch takes
+// about ten). The step's particles with the most candidates (DenseList) are
+// skipped by the one-thread path, and the first workgroups of the same launch
+// give each a group of kDenseGroup lanes, so that they run beside the bulk
+// rather than after it: the group walks the particle's
+// candidates in visiting order, kDenseSlots per round (rows flattened, so short
+// rows do not cost a round each), evaluates the pair terms in parallel with the
+// same arithmetic as the one-thread passes, and one lane per accumulator then
+// adds the round's in-range terms in visiting order. Every sum is the same
+// sequence of the same operations, so results equal the one-thread passes bit
+// for bit. The pair lists are recorded and replayed in the same format.
+constexpr int kDenseGroup = 16;
+constexpr int kDenseBatch = 4;
+constexpr int kDenseSlots = kDenseGroup * kDenseBatch;
+constexpr int kDenseGroups = 64 / kDenseGroup;
+// Workgroups of a neighbour launch that work through the dense list (a
+// multiple of 8, so that the rest keep their XCD round robin).
+#ifndef PBF_DENSE_BLOCKS
+#define PBF_DENSE_BLOCKS 2048
+#endif
+constexpr uint32_t kDenseBlocks = PBF_DENSE_BLOCKS;
+static_assert(kDenseBlocks % 8 == 0, "dense workgroups keep the XCD round robin of the rest");
+
+// Terms are stored accumulator-major, so an accumulator's lane reads a
+// round's terms four at a time.
+template <int NP, int NM>
+struct DenseLds {
+    int32_t s0[kDenseGroups][9];
+    int32_t cum[kDenseGroups][10];
+    int32_t total[kDenseGroups];
+    float4 p[kDenseGroups][NP][kDenseSlots / 4];
+    float4 m[kDenseGroups][NM > 0 ? NM : 1][kDenseSlots / 4];
+    float acc[kDenseGroups][NP];
+};
+
+// Group-local bits of a wave ballot.
+__device__ __forceinline__ uint64_t group_bits(uint64_t ballot, int g) {
+    return (ballot >> (kDenseGroup * g)) & ((1ull << kDenseGroup) - 1ull);
+}
+
+// Drives one pass over the dense list with workgroups block of blocks. Pass
+// supplies NP accumulators (NM of them of the form acc = (acc + p) - m, the
+// rest acc + p), RECORD (the pass scans and records the pair list) or replays
+// the recorded list, begin(i, sg), extra(sg, j), eval(pj, x, listed, P, M)
+// -> in range, and finish(i, sg, acc). The workgroup is one wave; every
+// barrier is reached by all of its lanes.
+template <class Pass, bool PAD, class Dom>
+__device__ __forceinline__ void dense_pass(const SolverConsts &c, const Dom &dom, const float4 *__restrict__ rec,
+                                           const int32_t *__restrict__ nbr9, const DenseList &dl,
+                                           uint16_t *__restrict__ list_entries, uint16_t *__restrict__ list_count,
+                                           int32_t list_cap, Pass &ps, DenseLds<Pass::NP, Pass::NM> &lds,
+                                           uint32_t block, uint32_t blocks) {
+    constexpr int NP = Pass::NP, NM = Pass::NM;
+    const int lane = threadIdx.x;
+    const int g = lane / kDenseGroup, lg = lane % kDenseGroup;
+    const int32_t th = dl.threshold[0];
+    // This workgroup's share: particles block, block + blocks, ... (clumps,
+    // contiguous in the sorted order, spread over all dense workgroups),
+    // examined 64 at a time; the groups take the dense ones four at a time.
+    for (uint64_t m0 = 0; m0 * blocks < (uint64_t)c.n; m0 += 64) {
+        const uint64_t ii = (uint64_t)block + (uint64_t)blocks * (m0 + (uint64_t)lane);
+        uint64_t pending = __ballot(ii < (uint64_t)c.n && (int32_t)dl.cand[ii] > th);
+        while (pending) {
+            uint64_t mine = pending;
+            for (int q = 0; q < g; q++) mine &= mine - 1ull;
+            for (int q = 0; q < kDenseGroups; q++) pending &= pending - 1ull;
+            const bool act = mine != 0ull;
+            const uint64_t slot = m0 + (uint64_t)(act ? __builtin_ctzll(mine) : 0);
+            const int32_t i = act ? (int32_t)((uint64_t)block + (uint64_t)blocks * slot) : 0;
+            const Seg sg = dom.at(i);
+            const int32_t li = i - sg.off;
+            if (lg < 9) {
+                const int32_t d = act ? nbr9[(size_t)lg * c.n + i] : -1;  // -1: no candidates
+                lds.s0[g][lg] = row_start(d);
+                lds.cum[g][lg + 1] = max(row_count(d), 0);
+            }
+            __syncthreads();
+            if (lg == 0) {
+                int32_t run = 0;
+                lds.cum[g][0] = 0;
+                for (int o = 0; o < 9; o++) {
+                    run += lds.cum[g][o + 1];
+                    lds.cum[g][o + 1] = run;
+                }
+            }
+            __syncthreads();
+            uint32_t lc = LIST_NONE;
+            if (!Pass::RECORD && act && list_entries) lc = list_count[i];
+            const bool listed = !Pass::RECORD && lc != LIST_NONE;
+            if (lg == 0) lds.total[g] = !act ? 0 : (listed ? (int32_t)lc : lds.cum[g][9]);
+            __syncthreads();
+            const int32_t cn = lds.total[g];
+            int32_t rounds = 0;
+    #pragma unroll
+            for (int q = 0; q < kDenseGroups; q++) rounds = max(rounds, (lds.total[q] + kDenseSlots - 1) / kDenseSlots);
+            ps.begin(i, sg);
+            float acc = 0.0f;
+            int32_t kept = 0;
+            int o = 0;
+            using X = decltype(ps.extra(sg, 0));
+            for (int r = 0; r < rounds; r++) {
+                int32_t j[kDenseBatch];
+                uint32_t code[kDenseBatch];
+                bool valid[kDenseBatch];
+    #pragma unroll
+                for (int b = 0; b < kDenseBatch; b++) {
+                    const int32_t k = r * kDenseSlots + b * kDenseGroup + lg;
+                    valid[b] = k < cn;
+                    if (listed) {
+                        code[b] = valid[b] ? list_entries[(size_t)k * c.n + i] : 0u;
+                        j[b] = lds.s0[g][code[b] >> 8] + (int32_t)(code[b] & 255u);
+                    } else {
+                        if (valid[b])
+                            while (o < 8 && k >= lds.cum[g][o + 1]) o++;
+                        const int32_t q = k - lds.cum[g][o];
+                        j[b] = lds.s0[g][o] + q;
+                        code[b] = ((uint32_t)o << 8) | (uint32_t)(q & 255);
+                        valid[b] = valid[b] & (j[b] != li);
+                    }
+                }
+                float4 pj[kDenseBatch];
+                X x[kDenseBatch];
+    #pragma unroll
+                for (int b = 0; b < kDenseBatch; b++) {
+                    pj[b] = valid[b] ? rec_at<PAD>(rec, sg, j[b]) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    x[b] = valid[b] ? ps.extra(sg, j[b]) : X();
+                }
+                uint64_t mask = 0;
+                bool in[kDenseBatch];
+    #pragma unroll
+                for (int b = 0; b < kDenseBatch; b++) {
+                    float P[NP], M[NM > 0 ? NM : 1];
+    #pragma unroll
+                    for (int a = 0; a < NP; a++) P[a] = 0.0f;
+    #pragma unroll
+                    for (int a = 0; a < (NM > 0 ? NM : 1); a++) M[a] = 0.0f;
+                    in[b] = valid[b] && ps.eval(pj[b], x[b], listed, P, M);
+                    const int slot = b * kDenseGroup + lg;
+    #pragma unroll
+                    for (int a = 0; a < NP; a++)
+                        reinterpret_cast<float *>(lds.p[g][a])[slot] = in[b] ? P[a] : 0.0f;
+                    if (NM > 0) {
+    #pragma unroll
+                        for (int a = 0; a < NM; a++)
+                            reinterpret_cast<float *>(lds.m[g][a])[slot] = in[b] ? M[a] : 0.0f;
+                    }
+                    mask |= group_bits(__ballot(in[b]), g) << (kDenseGroup * b);
+                }
+                if (Pass::RECORD && list_entries) {
+    #pragma unroll
+                    for (int b = 0; b < kDenseBatch; b++) {
+                        const int slot = b * kDenseGroup + lg;
+                        const int32_t pos = kept + __popcll(mask & ((1ull << slot) - 1ull));
+                        if (in[b] && pos < list_cap) list_entries[(size_t)pos * c.n + i] = (uint16_t)code[b];
+                    }
+                }
+                __syncthreads();
+                // Slots without an in-range pair hold +0.0, and an accumulator
+                // that starts at +0.0 is never -0.0 (round to nearest), so adding
+                // them changes nothing: every slot up to the round's last
+                // candidate is added in order.
+                const int used = min(kDenseSlots, cn - r * kDenseSlots);
+                if (lg < NP && used > 0) {
+                    for (int q = 0; q < (used + 3) / 4; q++) {
+                        const float4 v = lds.p[g][lg][q];
+                        if (NM > 0) {
+                            const float4 w = lds.m[g][NM > 0 ? lg : 0][q];
+                            acc = (acc + v.x) - w.x;
+                            acc = (acc + v.y) - w.y;
+                            acc = (acc + v.z) - w.z;
+                            acc = (acc + v.w) - w.w;
+                        } else {
+                            acc = acc + v.x;
+                            acc = acc + v.y;
+                            acc = acc + v.z;
+                            acc = acc + v.w;
+                        }
+                    }
+                }
+                kept += __popcll(mask);
+                __syncthreads();
+            }
+            if (lg < NP) lds.acc[g][lg] = acc;
+            __syncthreads();
+            if (act && lg == 0) {
+                ps.finish(i, sg, lds.acc[g]);
+                if (Pass::RECORD && list_entries) list_count[i] = kept <= list_cap ? (uint16_t)kept : LIST_NONE;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// calclambda.glsl:66-103 for the dense particles (terms of k_lambda).
+template <bool H1, bool PAD>
+struct LambdaPass {
+    static constexpr int NP = 5, NM = 0;
+    static constexpr bool RECORD = true;
+    const SolverConsts &c;
+    const float4 *__restrict__ rec;
+    float *__restrict__ lambda;
+    float4 pi;
+    __device__ __forceinline__ void begin(int32_t i, const Seg &) { pi = rec[i]; }
+    __device__ __forceinline__ int extra(const Seg &, int32_t) const { return 0; }
+    __device__ __forceinline__ bool eval(const float4 &pj, int, bool, float *P, float *) const {
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const float sq = sq_len(dx, dy, dz);
+        if (sq > c.h2cut) return false;
+        const float r = sqrt_rn(sq);
+        if (r > c.h) return false;
+        P[0] = wpoly6<H1>(c, sq, r);
+        if (r == 0.0f) return true;
+        float gx, gy, gz;
+        grad_spiky<H1>(c, dx, dy, dz, r, gx, gy, gz);
+        gx = gx * c.inv_rho0;
+        gy = gy * c.inv_rho0;
+        gz = gz * c.inv_rho0;
+        P[1] = sq_len(gx, gy, gz);
+        P[2] = gx;
+        P[3] = gy;
+        P[4] = gz;
+        return true;
+    }
+    __device__ __forceinline__ void finish(int32_t i, const Seg &, const float *a) const {
+        const float sk = a[1] + sq_len(a[2], a[3], a[4]);
+        const float ci = a[0] * c.inv_rho0 - 1.0f;
+        lambda[i] = -ci / (sk + c.eps);
+    }
+};
+
+// updatepos.glsl:42-105 for the dense particles (terms of k_updatepos).
+template <bool H1, bool PAD>
+struct UpdatePosPass {
+    static constexpr int NP = 3, NM = 0;
+    static constexpr bool RECORD = false;
+    const SolverConsts &c;
+    const float4 *__restrict__ rec_in;
+    const float *__restrict__ lambda;
+    float4 *__restrict__ rec_out;
+    float4 pi;
+    float li;
+    __device__ __forceinline__ void begin(int32_t i, const Seg &) {
+        pi = rec_in[i];
+        li = lambda[i];
+    }
+    __device__ __forceinline__ float extra(const Seg &sg, int32_t j) const { return val_at<PAD>(lambda, sg, j); }
+    __device__ __forceinline__ bool eval(const float4 &pj, float lj, bool listed, float *P, float *) const {
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const float sq = sq_len(dx, dy, dz);
+        if (!listed && sq > c.h2cut) return false;
+        const float r = sqrt_rn(sq);
+        if ((!listed && r > c.h) || r == 0.0f) return false;
+        float sc = c.scale * wpoly6<H1>(c, sq, r);
+        sc *= sc;
+        sc *= sc;
+        sc = -c.k * sc;
+        float gx, gy, gz;
+        grad_spiky<H1>(c, dx, dy, dz, r, gx, gy, gz);
+        const float f = li + lj + sc;
+        P[0] = f * gx;
+        P[1] = f * gy;
+        P[2] = f * gz;
+        return true;
+    }
+    __device__ __forceinline__ void finish(int32_t i, const Seg &sg, const float *a) const {
+        float x = pi.x + c.inv_rho0 * a[0], y = pi.y + c.inv_rho0 * a[1], z = pi.z + c.inv_rho0 * a[2];
+        x = clamp_glsl(x, sg.lo[0], sg.hi[0]);
+        y = clamp_glsl(y, sg.lo[1], sg.hi[1]);
+        z = clamp_glsl(z, sg.lo[2], sg.hi[2]);
+        rec_out[i] = make_float4(x, y, z, pi.w);
+    }
+};
+
+// vorticity.glsl:34-64 for the dense particles (terms of k_vorticity_a).
+template <bool H1, bool PAD>
+struct VorticityAPass {
+    static constexpr int NP = 6, NM = 6;
+    static constexpr bool RECORD = true;
+    const SolverConsts &c;
+    const float4 *__restrict__ rec;
+    const float4 *__restrict__ vel_sorted;
+    const float4 *__restrict__ vel_id;
+    float4 *__restrict__ vxsph;
+    float4 *__restrict__ omega;
+    float *__restrict__ vort;
+    float4 pi, vi;
+    __device__ __forceinline__ void begin(int32_t i, const Seg &) {
+        pi = rec[i];
+        vi = vel_sorted[i];
+    }
+    __device__ __forceinline__ float4 extra(const Seg &sg, int32_t j) const {
+        return j < sg.n ? vel_sorted[sg.off + j] : vel_id[sg.off];
+    }
+    __device__ __forceinline__ bool eval(const float4 &pj, const float4 &vj, bool, float *P, float *M) const {
+        const float px = pi.x - pj.x, py = pi.y - pj.y, pz = pi.z - pj.z;
+        const float sq = sq_len(px, py, pz);
+        if (sq > c.h2cut) return false;
+        const float r = sqrt_rn(sq);
+        if (r > c.h) return false;
+        const float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
+        const float w = wpoly6<H1>(c, sq, r);
+        P[0] = ux * w;
+        P[1] = uy * w;
+        P[2] = uz * w;
+        if (r == 0.0f) return true;
+        float gx, gy, gz;
+        grad_spiky<H1>(c, px, py, pz, r, gx, gy, gz);
+        P[3] = uy * gz;
+        M[3] = uz * gy;
+        P[4] = uz * gx;
+        M[4] = ux * gz;
+        P[5] = ux * gy;
+        M[5] = uy * gx;
+        return true;
+    }
+    __device__ __forceinline__ void finish(int32_t i, const Seg &sg, const float *a) const {
+        vxsph[i] = make_float4(vi.x + sg.xsph_c * a[0], vi.y + sg.xsph_c * a[1], vi.z + sg.xsph_c * a[2], 0.0f);
+        omega[i] = make_float4(a[3], a[4], a[5], 0.0f);
+        vort[i] = sqrtf(sq_len(a[3], a[4], a[5]));
+    }
+};
+
+// vorticity.glsl:65-85 for the dense particles (terms of k_vorticity_b).
+template <bool H1, bool PAD>
+struct VorticityBPass {
+    static constexpr int NP = 3, NM = 0;
+    static constexpr bool RECORD = false;
+    const SolverConsts &c;
+    const float4 *__restrict__ rec;
+    const float4 *__restrict__ vxsph;
+    const float4 *__restrict__ omega;
+    const float *__restrict__ vort;
+    float4 *__restrict__ vel_id;
+    float4 pi;
+    __device__ __forceinline__ void begin(int32_t i, const Seg &) { pi = rec[i]; }
+    __device__ __forceinline__ float extra(const Seg &sg, int32_t j) const { return val_at<PAD>(vort, sg, j); }
+    __device__ __forceinline__ bool eval(const float4 &pj, float om, bool listed, float *P, float *) const {
+        const float px = pi.x - pj.x, py = pi.y - pj.y, pz = pi.z - pj.z;
+        const float sq = sq_len(px, py, pz);
+        if (!listed && sq > c.h2cut) return false;
+        const float r = sqrt_rn(sq);
+        if ((!listed && r > c.h) || r == 0.0f) return false;
+        float gx, gy, gz;
+        grad_spiky<H1>(c, px, py, pz, r, gx, gy, gz);
+        P[0] = om * gx;
+        P[1] = om * gy;
+        P[2] = om * gz;
+        return true;
+    }
+    __device__ __forceinline__ void finish(int32_t i, const Seg &sg, const float *a) const {
+        float nx = a[0], ny = a[1], nz = a[2];
+        const float l = sqrtf(sq_len(nx, ny, nz));
+        if (l > 0.0f) {
+            nx = nx / l;
+            ny = ny / l;
+            nz = nz / l;
+        }
+        const float4 w = omega[i];
+        const float4 v = vxsph[i];
+        const float sc = c.dt * c.vort_eps;
+        const float ox = v.x + sc * (ny * w.z - nz * w.y);
+        const float oy = v.y + sc * (nz * w.x - nx * w.z);
+        const float oz = v.z + sc * (nx * w.y - ny * w.x);
+        vel_id[sg.off + id_of(pi)] = make_float4(ox, oy, oz, 0.0f);
+    }
+};
+
+// True for a particle on the step's dense list (more candidates than the
+// step's threshold), which the launch's dense workgroups handle.
+__device__ __forceinline__ bool leave_dense(int32_t cand, const DenseList &dl) {
+    return dl.threshold && cand > dl.threshold[0];
+}
+
 // calclambda.glsl:66-103; records the neighbours within h for the position update.
 template <bool H1, int NB, bool PAD, class Dom>
 __global__ __launch_bounds__(NB) void k_lambda(SolverConsts c, Dom dom, const float4 *__restrict__ rec,
                                                      const int32_t *__restrict__ nbr9, float *__restrict__ lambda,
                                                      uint16_t *__restrict__ list_entries,
-                                                     uint16_t *__restrict__ list_count, int32_t list_cap) {
-    __shared__ NbrLds<NB> lds;
-    const int32_t i = xcd_block<NB>() * NB + threadIdx.x;
+                                                     uint16_t *__restrict__ list_count, int32_t list_cap,
+                                                     DenseList dl, uint32_t dense_blocks) {
+    __shared__ union {
+        NbrLds<NB> light;
+        DenseLds<5, 0> dense;
+    } u;
+    if constexpr (std::is_same<Dom, DomOne>::value) {
+        if (blockIdx.x < dense_blocks) {
+            LambdaPass<H1, PAD> ps{c, rec, lambda, {}};
+            dense_pass<LambdaPass<H1, PAD>, PAD>(c, dom, rec, nbr9, dl, list_entries, list_count, list_cap, ps,
+                                                 u.dense, blockIdx.x, dense_blocks);
+            return;
+        }
+    }
+    NbrLds<NB> &lds = u.light;
+    const int32_t i = xcd_block<NB>(blockIdx.x - dense_blocks, gridDim.x - dense_blocks) * NB + threadIdx.x;
     if (i >= c.n) return;
+    if (leave_dense(load_rows<NB>(c, nbr9, i, lds), dl)) return;
     const Seg sg = dom.at(i);
     const float4 pi = rec[i];
     float rho = 0.0f, sk = 0.0f, gix = 0.0f, giy = 0.0f, giz = 0.0f;
     PairListOut out = {list_entries, list_count, list_cap, 0};
-    scan_neighbours<NB, PAD>(c, sg, rec, nbr9, i, pi, lds, [](int32_t) { return 0; },
+    scan_neighbours<NB, PAD>(c, sg, rec, i, pi, lds, [](int32_t) { return 0; },
                              [&](int32_t, uint32_t code, const float4 &pj, int) {
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const float sq = sq_len(dx, dy, dz);
@@ -505,15 +1000,30 @@ __global__ __launch_bounds__(NB) void k_updatepos(SolverConsts c, Dom dom, const
                                                         const int32_t *__restrict__ nbr9,
                                                         const uint16_t *__restrict__ list_entries,
                                                         const uint16_t *__restrict__ list_count,
-                                                        float4 *__restrict__ rec_out) {
-    __shared__ NbrLds<NB> lds;
-    const int32_t i = xcd_block<NB>() * NB + threadIdx.x;
+                                                        float4 *__restrict__ rec_out, DenseList dl,
+                                                        uint32_t dense_blocks) {
+    __shared__ union {
+        NbrLds<NB> light;
+        DenseLds<3, 0> dense;
+    } u;
+    if constexpr (std::is_same<Dom, DomOne>::value) {
+        if (blockIdx.x < dense_blocks) {
+            UpdatePosPass<H1, PAD> ps{c, rec_in, lambda, rec_out, {}, 0.0f};
+            dense_pass<UpdatePosPass<H1, PAD>, PAD>(c, dom, rec_in, nbr9, dl, const_cast<uint16_t *>(list_entries),
+                                                    const_cast<uint16_t *>(list_count), 0, ps, u.dense, blockIdx.x,
+                                                    dense_blocks);
+            return;
+        }
+    }
+    NbrLds<NB> &lds = u.light;
+    const int32_t i = xcd_block<NB>(blockIdx.x - dense_blocks, gridDim.x - dense_blocks) * NB + threadIdx.x;
     if (i >= c.n) return;
+    if (leave_dense(load_rows<NB>(c, nbr9, i, lds), dl)) return;
     const Seg sg = dom.at(i);
     const float4 pi = rec_in[i];
     const float li = lambda[i];
     float dpx = 0.0f, dpy = 0.0f, dpz = 0.0f;
-    replay_neighbours<NB, PAD>(c, sg, rec_in, nbr9, list_entries, list_count, i, pi, lds,
+    replay_neighbours<NB, PAD>(c, sg, rec_in, list_entries, list_count, i, pi, lds,
                       [&](int32_t j) { return val_at<PAD>(lambda, sg, j); },
                       [&](int32_t, const float4 &pj, float lj, bool listed) {
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
@@ -567,17 +1077,31 @@ __global__ __launch_bounds__(NB) void k_vorticity_a(SolverConsts c, Dom dom, con
                                                           const float4 *__restrict__ vel_id,
                                                           float4 *__restrict__ vxsph, float4 *__restrict__ omega,
                                                           float *__restrict__ vort, uint16_t *__restrict__ list_entries,
-                                                          uint16_t *__restrict__ list_count, int32_t list_cap) {
-    __shared__ NbrLds<NB> lds;
-    const int32_t i = xcd_block<NB>() * NB + threadIdx.x;
+                                                          uint16_t *__restrict__ list_count, int32_t list_cap,
+                                                          DenseList dl, uint32_t dense_blocks) {
+    __shared__ union {
+        NbrLds<NB> light;
+        DenseLds<6, 6> dense;
+    } u;
+    if constexpr (std::is_same<Dom, DomOne>::value) {
+        if (blockIdx.x < dense_blocks) {
+            VorticityAPass<H1, PAD> ps{c, rec, vel_sorted, vel_id, vxsph, omega, vort, {}, {}};
+            dense_pass<VorticityAPass<H1, PAD>, PAD>(c, dom, rec, nbr9, dl, list_entries, list_count, list_cap, ps,
+                                                     u.dense, blockIdx.x, dense_blocks);
+            return;
+        }
+    }
+    NbrLds<NB> &lds = u.light;
+    const int32_t i = xcd_block<NB>(blockIdx.x - dense_blocks, gridDim.x - dense_blocks) * NB + threadIdx.x;
     if (i >= c.n) return;
+    if (leave_dense(load_rows<NB>(c, nbr9, i, lds), dl)) return;
     const Seg sg = dom.at(i);
     const float4 pi = rec[i];
     const float4 vi = vel_sorted[i];
     float sx = 0.0f, sy = 0.0f, sz = 0.0f, wx = 0.0f, wy = 0.0f, wz = 0.0f;
     PairListOut out = {list_entries, list_count, list_cap, 0};
     // a zero record past the scene has id 0: imageLoad (velocitytexture, 0)
-    scan_neighbours<NB, PAD>(c, sg, rec, nbr9, i, pi, lds,
+    scan_neighbours<NB, PAD>(c, sg, rec, i, pi, lds,
                              [&](int32_t j) { return j < sg.n ? vel_sorted[sg.off + j] : vel_id[sg.off]; },
                              [&](int32_t, uint32_t code, const float4 &pj, const float4 &vj) {
         const float px = pi.x - pj.x, py = pi.y - pj.y, pz = pi.z - pj.z;
@@ -614,14 +1138,29 @@ __global__ __launch_bounds__(NB) void k_vorticity_b(SolverConsts c, Dom dom, con
                                                           const float4 *__restrict__ vxsph,
                                                           const float4 *__restrict__ omega,
                                                           const float *__restrict__ vort,
-                                                          float4 *__restrict__ vel_id) {
-    __shared__ NbrLds<NB> lds;
-    const int32_t i = xcd_block<NB>() * NB + threadIdx.x;
+                                                          float4 *__restrict__ vel_id, DenseList dl,
+                                                          uint32_t dense_blocks) {
+    __shared__ union {
+        NbrLds<NB> light;
+        DenseLds<3, 0> dense;
+    } u;
+    if constexpr (std::is_same<Dom, DomOne>::value) {
+        if (blockIdx.x < dense_blocks) {
+            VorticityBPass<H1, PAD> ps{c, rec, vxsph, omega, vort, vel_id, {}};
+            dense_pass<VorticityBPass<H1, PAD>, PAD>(c, dom, rec, nbr9, dl, const_cast<uint16_t *>(list_entries),
+                                                     const_cast<uint16_t *>(list_count), 0, ps, u.dense, blockIdx.x,
+                                                     dense_blocks);
+            return;
+        }
+    }
+    NbrLds<NB> &lds = u.light;
+    const int32_t i = xcd_block<NB>(blockIdx.x - dense_blocks, gridDim.x - dense_blocks) * NB + threadIdx.x;
     if (i >= c.n) return;
+    if (leave_dense(load_rows<NB>(c, nbr9, i, lds), dl)) return;
     const Seg sg = dom.at(i);
     const float4 pi = rec[i];
     float nx = 0.0f, ny = 0.0f, nz = 0.0f;
-    replay_neighbours<NB, PAD>(c, sg, rec, nbr9, list_entries, list_count, i, pi, lds,
+    replay_neighbours<NB, PAD>(c, sg, rec, list_entries, list_count, i, pi, lds,
                       [&](int32_t j) { return val_at<PAD>(vort, sg, j); },
                       [&](int32_t, const float4 &pj, float om, bool listed) {
         const float px = pi.x - pj.x, py = pi.y - pj.y, pz = pi.z - pj.z;
@@ -769,9 +1308,19 @@ void findcells_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, 
 }
 template <class Dom>
 void neighbourcells_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, const float4 *sorted,
-                         const int32_t *start, const unsigned long long *end64, int32_t *nbr9, hipStream_t st) {
+                         const int32_t *start, const unsigned long long *end64, int32_t *nbr9, hipStream_t st,
+                         const DenseList *dl = nullptr) {
     if (c.n > 0)
-        k_neighbourcells<Dom><<<stream_blocks(c.n, ls), ls.sb, 0, st>>>(c, d, ls.ppt, sorted, start, end64, nbr9);
+        k_neighbourcells<Dom><<<stream_blocks(c.n, ls), ls.sb, 0, st>>>(c, d, ls.ppt, sorted, start, end64, nbr9,
+                                                                         dl ? *dl : DenseList());
+    if (dl && c.n > 0) launch_dense_select(*dl, st);
+}
+
+// Workgroups of a neighbour launch given to the dense list (none without one).
+unsigned dense_blocks_of(const DenseList *dl, int32_t n) {
+    if (!dl) return 0u;
+    const int64_t need = ((int64_t)n + kDenseGroups - 1) / kDenseGroups;
+    return (unsigned)std::min<int64_t>(kDenseBlocks, (need + 7) / 8 * 8);
 }
 template <class Dom>
 void update_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, const float4 *rec, float4 *pos,
@@ -779,66 +1328,79 @@ void update_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, con
     if (c.n > 0) k_update<Dom><<<stream_blocks(c.n, ls), ls.sb, 0, st>>>(c, d, ls.ppt, rec, pos, vel, vel_sorted);
 }
 template <class Dom>
-void lambda_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, bool pad, const float4 *rec, const int32_t *nbr9,
-                 float *lambda, const PairList &pl, hipStream_t st) {
+void lambda_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, bool pad, const float4 *rec,
+                 const int32_t *nbr9, float *lambda, const PairList &pl, hipStream_t st,
+                 const DenseList *dl = nullptr) {
     if (c.n <= 0) return;
+    const DenseList dlv = dl ? *dl : DenseList();
+    const unsigned db = std::is_same<Dom, DomOne>::value ? dense_blocks_of(dl, c.n) : 0u;
     with_nb<Dom>(ls.nb, pad, [&](auto nbc, auto padc) {
         constexpr int NB = decltype(nbc)::value;
         constexpr bool PAD = decltype(padc)::value;
+        const unsigned grid = nb_blocks(c.n, NB) + db;
         if (c.h_is_one)
-            k_lambda<true, NB, PAD, Dom><<<nb_blocks(c.n, NB), NB, 0, st>>>(c, d, rec, nbr9, lambda, pl.entries, pl.count,
-                                                                       pl.cap);
+            k_lambda<true, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec, nbr9, lambda, pl.entries, pl.count, pl.cap,
+                                                              dlv, db);
         else
-            k_lambda<false, NB, PAD, Dom><<<nb_blocks(c.n, NB), NB, 0, st>>>(c, d, rec, nbr9, lambda, pl.entries,
-                                                                        pl.count, pl.cap);
+            k_lambda<false, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec, nbr9, lambda, pl.entries, pl.count, pl.cap,
+                                                               dlv, db);
     });
 }
 template <class Dom>
 void updatepos_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, bool pad, const float4 *rec_in,
-                    const float *lambda, const int32_t *nbr9, const PairList &pl, float4 *rec_out, hipStream_t st) {
+                    const float *lambda, const int32_t *nbr9, const PairList &pl, float4 *rec_out, hipStream_t st,
+                    const DenseList *dl = nullptr) {
     if (c.n <= 0) return;
+    const DenseList dlv = dl ? *dl : DenseList();
+    const unsigned db = std::is_same<Dom, DomOne>::value ? dense_blocks_of(dl, c.n) : 0u;
     with_nb<Dom>(ls.nb, pad, [&](auto nbc, auto padc) {
         constexpr int NB = decltype(nbc)::value;
         constexpr bool PAD = decltype(padc)::value;
+        const unsigned grid = nb_blocks(c.n, NB) + db;
         if (c.h_is_one)
-            k_updatepos<true, NB, PAD, Dom><<<nb_blocks(c.n, NB), NB, 0, st>>>(c, d, rec_in, lambda, nbr9, pl.entries,
-                                                                          pl.count, rec_out);
+            k_updatepos<true, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec_in, lambda, nbr9, pl.entries, pl.count,
+                                                                 rec_out, dlv, db);
         else
-            k_updatepos<false, NB, PAD, Dom><<<nb_blocks(c.n, NB), NB, 0, st>>>(c, d, rec_in, lambda, nbr9, pl.entries,
-                                                                           pl.count, rec_out);
+            k_updatepos<false, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec_in, lambda, nbr9, pl.entries, pl.count,
+                                                                  rec_out, dlv, db);
     });
 }
 template <class Dom>
 void vorticity_a_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, bool pad, const float4 *rec,
                       const int32_t *nbr9, const float4 *vel_sorted, const float4 *vel_id, float4 *vxsph,
-                      float4 *omega, float *vort, const PairList &pl, hipStream_t st) {
+                      float4 *omega, float *vort, const PairList &pl, hipStream_t st, const DenseList *dl = nullptr) {
     if (c.n <= 0) return;
+    const DenseList dlv = dl ? *dl : DenseList();
+    const unsigned db = std::is_same<Dom, DomOne>::value ? dense_blocks_of(dl, c.n) : 0u;
     with_nb<Dom>(ls.nb, pad, [&](auto nbc, auto padc) {
         constexpr int NB = decltype(nbc)::value;
         constexpr bool PAD = decltype(padc)::value;
+        const unsigned grid = nb_blocks(c.n, NB) + db;
         if (c.h_is_one)
-            k_vorticity_a<true, NB, PAD, Dom><<<nb_blocks(c.n, NB), NB, 0, st>>>(c, d, rec, nbr9, vel_sorted, vel_id, vxsph,
-                                                                            omega, vort, pl.entries, pl.count, pl.cap);
+            k_vorticity_a<true, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec, nbr9, vel_sorted, vel_id, vxsph, omega,
+                                                                   vort, pl.entries, pl.count, pl.cap, dlv, db);
         else
-            k_vorticity_a<false, NB, PAD, Dom><<<nb_blocks(c.n, NB), NB, 0, st>>>(c, d, rec, nbr9, vel_sorted, vel_id,
-                                                                             vxsph, omega, vort, pl.entries, pl.count,
-                                                                             pl.cap);
+            k_vorticity_a<false, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec, nbr9, vel_sorted, vel_id, vxsph, omega,
+                                                                    vort, pl.entries, pl.count, pl.cap, dlv, db);
     });
 }
 template <class Dom>
 void vorticity_b_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, bool pad, const float4 *rec,
                       const int32_t *nbr9, const PairList &pl, const float4 *vxsph, const float4 *omega,
-                      const float *vort, float4 *vel_id, hipStream_t st) {
+                      const float *vort, float4 *vel_id, hipStream_t st, const DenseList *dl = nullptr) {
     if (c.n <= 0) return;
+    const DenseList dlv = dl ? *dl : DenseList();
+    const unsigned db = std::is_same<Dom, DomOne>::value ? dense_blocks_of(dl, c.n) : 0u;
     with_nb<Dom>(ls.nb, pad, [&](auto nbc, auto padc) {
         constexpr int NB = decltype(nbc)::value;
         constexpr bool PAD = decltype(padc)::value;
+        const unsigned grid = nb_blocks(c.n, NB) + db;
         if (c.h_is_one)
-            k_vorticity_b<true, NB, PAD, Dom><<<nb_blocks(c.n, NB), NB, 0, st>>>(c, d, rec, nbr9, pl.entries, pl.count,
-                                                                            vxsph, omega, vort, vel_id);
+            k_vorticity_b<true, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec, nbr9, pl.entries, pl.count, vxsph, omega,
+                                                                   vort, vel_id, dlv, db);
         else
-            k_vorticity_b<false, NB, PAD, Dom><<<nb_blocks(c.n, NB), NB, 0, st>>>(c, d, rec, nbr9, pl.entries, pl.count,
-                                                                             vxsph, omega, vort, vel_id);
+            k_vorticity_b<false, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec, nbr9, pl.entries, pl.count, vxsph,
+                                                                    omega, vort, vel_id, dlv, db);
     });
 }
 }  // namespace
@@ -860,24 +1422,27 @@ void launch_gather_by_id(int32_t n, const float4 *rec, const float4 *src, float4
 void launch_fill_i32(int32_t *p, int32_t value, size_t count, hipStream_t st) {
     if (count > 0) k_fill_i32<<<grid_stride_blocks((count + 3) / 4), BLOCK, 0, st>>>(p, value, count);
 }
-void launch_begin_cells(int32_t *start, size_t cells, uint32_t *epoch, hipStream_t st) {
-    k_begin_cells<<<grid_stride_blocks((cells + 3) / 4), BLOCK, 0, st>>>(start, cells, epoch);
+void launch_begin_cells(int32_t *start, size_t cells, uint32_t *epoch, hipStream_t st, const DenseList *dl) {
+    k_begin_cells<<<grid_stride_blocks((cells + 3) / 4), BLOCK, 0, st>>>(start, cells, epoch, dl ? *dl : DenseList());
+}
+void launch_dense_select(const DenseList &dl, hipStream_t st) {
+    k_dense_select<<<1, kDenseSelectThreads, 0, st>>>(dl);
 }
 void launch_findcells(const SolverConsts &c, const float4 *sorted, int32_t *start, unsigned long long *end64,
                       const uint32_t *epoch, hipStream_t st) {
     findcells_impl(c, one(c), kDefaultShape, sorted, start, end64, epoch, st);
 }
 void launch_neighbourcells(const SolverConsts &c, const float4 *sorted, const int32_t *start,
-                           const unsigned long long *end64, int32_t *nbr9, hipStream_t st) {
-    neighbourcells_impl(c, one(c), kDefaultShape, sorted, start, end64, nbr9, st);
+                           const unsigned long long *end64, int32_t *nbr9, hipStream_t st, const DenseList *dl) {
+    neighbourcells_impl(c, one(c), kDefaultShape, sorted, start, end64, nbr9, st, dl);
 }
 void launch_lambda(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, float *lambda, const PairList &pl,
-                   hipStream_t st, bool padded) {
-    lambda_impl(c, one(c), kDefaultShape, padded, rec, nbr9, lambda, pl, st);
+                   hipStream_t st, bool padded, const DenseList *dl) {
+    lambda_impl(c, one(c), kDefaultShape, padded, rec, nbr9, lambda, pl, st, dl);
 }
 void launch_updatepos(const SolverConsts &c, const float4 *rec_in, const float *lambda, const int32_t *nbr9,
-                      const PairList &pl, float4 *rec_out, hipStream_t st, bool padded) {
-    updatepos_impl(c, one(c), kDefaultShape, padded, rec_in, lambda, nbr9, pl, rec_out, st);
+                      const PairList &pl, float4 *rec_out, hipStream_t st, bool padded, const DenseList *dl) {
+    updatepos_impl(c, one(c), kDefaultShape, padded, rec_in, lambda, nbr9, pl, rec_out, st, dl);
 }
 void launch_update(const SolverConsts &c, const float4 *rec, float4 *pos, float4 *vel, float4 *vel_sorted,
                    hipStream_t st) {
@@ -885,13 +1450,13 @@ void launch_update(const SolverConsts &c, const float4 *rec, float4 *pos, float4
 }
 void launch_vorticity_a(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, const float4 *vel_sorted,
                         const float4 *vel_id, float4 *vxsph, float4 *omega, float *vort, const PairList &pl,
-                        hipStream_t st, bool padded) {
-    vorticity_a_impl(c, one(c), kDefaultShape, padded, rec, nbr9, vel_sorted, vel_id, vxsph, omega, vort, pl, st);
+                        hipStream_t st, bool padded, const DenseList *dl) {
+    vorticity_a_impl(c, one(c), kDefaultShape, padded, rec, nbr9, vel_sorted, vel_id, vxsph, omega, vort, pl, st, dl);
 }
 void launch_vorticity_b(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, const PairList &pl,
                         const float4 *vxsph, const float4 *omega, const float *vort, float4 *vel_id, hipStream_t st,
-                        bool padded) {
-    vorticity_b_impl(c, one(c), kDefaultShape, padded, rec, nbr9, pl, vxsph, omega, vort, vel_id, st);
+                        bool padded, const DenseList *dl) {
+    vorticity_b_impl(c, one(c), kDefaultShape, padded, rec, nbr9, pl, vxsph, omega, vort, vel_id, st, dl);
 }
 
 // many scenes
