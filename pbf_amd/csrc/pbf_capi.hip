@@ -374,7 +374,7 @@ int pbf_sph_create(pbf_sph **out, uint32_t n, int32_t gx, int32_t gy, int32_t gz
     TRY(dalloc(&s->list_count, n));
     TRY(dalloc(&s->dense.cand, n));
     TRY(dalloc(&s->dense.levels, (size_t)kDenseLevels * kDenseCopies));
-    TRY(dalloc(&s->dense.threshold, 1));
+    TRY(dalloc(&s->dense.threshold, 2));
     TRY(hipMemset(s->dense.levels, 0, (size_t)kDenseLevels * kDenseCopies * sizeof(uint32_t)));
     TRY(dalloc(&s->vel_sorted, n));
     TRY(dalloc(&s->vxsph, n));

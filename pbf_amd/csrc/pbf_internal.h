@@ -113,7 +113,7 @@ constexpr uint32_t kDenseBudget = PBF_DENSE_BUDGET;
 struct DenseList {
     uint16_t *cand = nullptr;        // n entries
     uint32_t *levels = nullptr;      // kDenseLevels * kDenseCopies words
-    int32_t *threshold = nullptr;    // 1 word
+    int32_t *threshold = nullptr;    // 2 words: the threshold, the number of particles above it
 };
 void launch_dense_select(const DenseList &dl, hipStream_t st);
 // padded: rec, lambda and vort hold kPad zero entries after the n particles.

@@ -325,14 +325,15 @@ __global__ __launch_bounds__(kDenseSelectThreads) void k_dense_select(DenseList 
     }
     __syncthreads();
     if (t == 0) {
-        int32_t th = kDenseLevelValue[kDenseLevels - 1];
+        int level = kDenseLevels - 1;
         for (int l = 0; l < kDenseLevels; l++) {
             if (above[l] <= kDenseBudget) {
-                th = kDenseLevelValue[l];
+                level = l;
                 break;
             }
         }
-        dl.threshold[0] = th;
+        dl.threshold[0] = kDenseLevelValue[level];
+        dl.threshold[1] = (int32_t)above[level];  // particles above it
     }
 }
 
@@ -364,13 +365,20 @@ __device__ __forceinline__ float val_at(const float *__restrict__ a, const Seg &
 // the list. A column that fills up is evaluated early and reused. Sums keep the
 // reference's order, and candidates with fl(sqrt(sq)) > h add exact zeros
 // there, so skipping them changes no bit of the result.
-// LCAP entries per column: the LDS of a workgroup is (LCAP + 1) * 2 + 45 bytes
-// per thread, and LDS is what limits the resident waves per CU.
+// LC entries per column: the LDS of a workgroup is (LC + 1) * 2 + 45 bytes
+// per thread, and LDS is what limits the resident waves per CU. The scanning
+// passes (lambda, vorticity A) keep LCAP; the replaying passes (position
+// update, vorticity B) scan only for particles whose pair list overflowed and
+// keep kReplayLcap, so more of their waves fit on a CU.
 #ifndef PBF_LIST_LDS
 #define PBF_LIST_LDS 48
 #endif
+#ifndef PBF_REPLAY_LDS
+#define PBF_REPLAY_LDS 16
+#endif
 constexpr int LCAP = PBF_LIST_LDS;
-static_assert(LCAP % 8 == 0, "list column of whole batches");
+constexpr int kReplayLcap = PBF_REPLAY_LDS;
+static_assert(LCAP % 8 == 0 && kReplayLcap % 8 == 0, "list column of whole batches");
 // Candidates (and listed neighbours) whose loads are issued together: a
 // particle in a dense clump walks hundreds of them serially, and its wave
 // alone can set the kernel's duration, so memory-level parallelism per lane
@@ -379,12 +387,13 @@ static_assert(LCAP % 8 == 0, "list column of whole batches");
 #define PBF_BATCH 8
 #endif
 constexpr int kBatch = PBF_BATCH;
-static_assert(LCAP >= 2 * kBatch, "list column must hold at least two batches");
+static_assert(LCAP >= 2 * kBatch && kReplayLcap >= 2 * kBatch, "list column must hold at least two batches");
 static_assert(126 + kBatch <= kPad, "batches past a row stay inside the zero padding");
 
-template <int NB>
+template <int NB, int LC = LCAP>
 struct NbrLds {
-    uint16_t list[(LCAP + 1) * NB];  // row LCAP: sink of the branch-free appends
+    static constexpr int kCap = LC;
+    uint16_t list[(LC + 1) * NB];  // row LC: sink of the branch-free appends
     int32_t rowstart[9 * NB];
     int8_t rowcount[9 * NB];  // start + (count << 24) >> 24: in [-128, 127]
 };
@@ -404,9 +413,9 @@ __device__ __forceinline__ int32_t row_start(int32_t word) { return word & 0xFFF
 // The nine row words of slot i, loaded together (one round trip rather than
 // one per row ahead of that row's candidates) into the thread's LDS columns;
 // returns the number of candidates (the sum of the positive counts).
-template <int NB>
+template <int NB, class L>
 __device__ __forceinline__ int32_t load_rows(const SolverConsts &c, const int32_t *__restrict__ nbr9, int32_t i,
-                                             NbrLds<NB> &lds) {
+                                             L &lds) {
     const int t = threadIdx.x;
     int32_t d[9];
 #pragma unroll
@@ -429,10 +438,10 @@ __device__ __forceinline__ int32_t load_rows(const SolverConsts &c, const int32_
 // kBatch at a time with their loads issued together, so a particle with
 // hundreds of neighbours waits for one round trip per batch rather than per
 // neighbour.
-template <int NB, bool PAD, class Extra, class Body>
+template <int NB, bool PAD, class L, class Extra, class Body>
 __device__ __forceinline__ void scan_neighbours(const SolverConsts &c, const Seg &sg, const float4 *__restrict__ rec,
-                                                int32_t i, const float4 &pi, NbrLds<NB> &lds, Extra &&extra,
-                                                Body &&body) {
+                                                int32_t i, const float4 &pi, L &lds, Extra &&extra, Body &&body) {
+    constexpr int LC = L::kCap;
     const int t = threadIdx.x;
     const int32_t li = i - sg.off;
     const float4 *__restrict__ srec = rec + sg.off;
@@ -474,7 +483,7 @@ __device__ __forceinline__ void scan_neighbours(const SolverConsts &c, const Seg
             // fills up at a different candidate, which in a dense clump is
             // most of the wave's time. Each lane still evaluates its own list
             // in order.
-            if (wave_any(cnt > LCAP - kBatch)) flush();
+            if (wave_any(cnt > LC - kBatch)) flush();
             const int32_t j0 = s0 + q;
             float4 p[kBatch];
 #pragma unroll
@@ -521,11 +530,11 @@ struct PairListOut {
 // pass) or for candidates that may be within h (scanned); j is scene-local;
 // extra(j) gathers the kernel's per-neighbour operand so that, on the list
 // path, the loads of four entries are in flight together.
-template <int NB, bool PAD, class Extra, class Body>
+template <int NB, bool PAD, class L, class Extra, class Body>
 __device__ __forceinline__ void replay_neighbours(const SolverConsts &c, const Seg &sg, const float4 *__restrict__ rec,
                                                   const uint16_t *__restrict__ entries,
                                                   const uint16_t *__restrict__ count, int32_t i, const float4 &pi,
-                                                  NbrLds<NB> &lds, Extra &&extra, Body &&body) {
+                                                  L &lds, Extra &&extra, Body &&body) {
     const int t = threadIdx.x;
     const uint32_t cnt = entries ? count[i] : LIST_NONE;
     if (cnt == LIST_NONE) {
@@ -581,56 +590,71 @@ constexpr int kDenseGroup = 16;
 constexpr int kDenseBatch = 4;
 constexpr int kDenseSlots = kDenseGroup * kDenseBatch;
 constexpr int kDenseGroups = 64 / kDenseGroup;
-// Workgroups of a neighbour launch that work through the dense list (a
-// multiple of 8, so that the rest keep their XCD round robin).
-#ifndef PBF_DENSE_BLOCKS
-#define PBF_DENSE_BLOCKS 2048
+// Waves of a neighbour launch that work through the dense particles (in
+// whole workgroups, a multiple of 8 of them, so that the rest keep their XCD
+// round robin).
+#ifndef PBF_DENSE_WAVES
+#define PBF_DENSE_WAVES 2048
 #endif
-constexpr uint32_t kDenseBlocks = PBF_DENSE_BLOCKS;
-static_assert(kDenseBlocks % 8 == 0, "dense workgroups keep the XCD round robin of the rest");
+constexpr uint32_t kDenseWaves = PBF_DENSE_WAVES;
 
-// Terms are stored accumulator-major, so an accumulator's lane reads a
-// round's terms four at a time.
-template <int NP, int NM>
+// Per group (W waves of kDenseGroups groups). Terms are stored
+// accumulator-major, so an accumulator's lane reads a round's terms four at a
+// time.
+template <int NP, int NM, int W>
 struct DenseLds {
-    int32_t s0[kDenseGroups][9];
-    int32_t cum[kDenseGroups][10];
-    int32_t total[kDenseGroups];
-    float4 p[kDenseGroups][NP][kDenseSlots / 4];
-    float4 m[kDenseGroups][NM > 0 ? NM : 1][kDenseSlots / 4];
-    float acc[kDenseGroups][NP];
+    int32_t s0[W * kDenseGroups][9];
+    int32_t cum[W * kDenseGroups][10];
+    int32_t total[W * kDenseGroups];
+    float4 p[W * kDenseGroups][NP][kDenseSlots / 4];
+    float4 m[W * kDenseGroups][NM > 0 ? NM : 1][kDenseSlots / 4];
+    float acc[W * kDenseGroups][NP];
 };
+
+// Orders the LDS accesses of one wave's lanes (the dense groups of a wave
+// exchange data through LDS; waves of a workgroup do not wait for each other).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // Group-local bits of a wave ballot.
 __device__ __forceinline__ uint64_t group_bits(uint64_t ballot, int g) {
     return (ballot >> (kDenseGroup * g)) & ((1ull << kDenseGroup) - 1ull);
 }
 
-// Drives one pass over the dense list with workgroups block of blocks. Pass
-// supplies NP accumulators (NM of them of the form acc = (acc + p) - m, the
-// rest acc + p), RECORD (the pass scans and records the pair list) or replays
+// Drives one pass over the dense particles with workgroup wg of wgs, each of
+// W waves. Pass
+// supplies NP accumulators (the last NM of them of the form
+// acc = (acc + p) - m, the rest acc + p), RECORD (the pass scans and records the pair list) or replays
 // the recorded list, begin(i, sg), extra(sg, j), eval(pj, x, listed, P, M)
-// -> in range, and finish(i, sg, acc). The workgroup is one wave; every
-// barrier is reached by all of its lanes.
-template <class Pass, bool PAD, class Dom>
+// -> in range, and finish(i, sg, acc). Waves synchronize on their own, and
+// every synchronization is reached by all lanes of the wave.
+template <class Pass, bool PAD, int W, class Dom>
 __device__ __forceinline__ void dense_pass(const SolverConsts &c, const Dom &dom, const float4 *__restrict__ rec,
                                            const int32_t *__restrict__ nbr9, const DenseList &dl,
                                            uint16_t *__restrict__ list_entries, uint16_t *__restrict__ list_count,
-                                           int32_t list_cap, Pass &ps, DenseLds<Pass::NP, Pass::NM> &lds,
-                                           uint32_t block, uint32_t blocks) {
+                                           int32_t list_cap, Pass &ps, DenseLds<Pass::NP, Pass::NM, W> &lds,
+                                           uint32_t wg, uint32_t wgs) {
     constexpr int NP = Pass::NP, NM = Pass::NM;
-    const int lane = threadIdx.x;
-    const int g = lane / kDenseGroup, lg = lane % kDenseGroup;
+    const int lane = (int)(threadIdx.x & 63u);
+    const int wave = (int)(threadIdx.x >> 6);
+    const int lg = lane % kDenseGroup;
+    const int g = wave * kDenseGroups + lane / kDenseGroup;  // this lane's group in the workgroup
+    const int g0 = wave * kDenseGroups;                      // the wave's first group
+    const uint32_t block = wg * W + (uint32_t)wave, blocks = wgs * W;
     const int32_t th = dl.threshold[0];
-    // This workgroup's share: particles block, block + blocks, ... (clumps,
-    // contiguous in the sorted order, spread over all dense workgroups),
-    // examined 64 at a time; the groups take the dense ones four at a time.
+    if (dl.threshold[1] == 0) return;  // no particle above the threshold
+    // Each wave works alone. Its share: particles block, block + blocks, ...
+    // (clumps, contiguous in the sorted order, spread over all dense waves),
+    // examined 64 at a time; its groups take the dense ones four at a time.
     for (uint64_t m0 = 0; m0 * blocks < (uint64_t)c.n; m0 += 64) {
         const uint64_t ii = (uint64_t)block + (uint64_t)blocks * (m0 + (uint64_t)lane);
         uint64_t pending = __ballot(ii < (uint64_t)c.n && (int32_t)dl.cand[ii] > th);
         while (pending) {
             uint64_t mine = pending;
-            for (int q = 0; q < g; q++) mine &= mine - 1ull;
+            for (int q = 0; q < g - g0; q++) mine &= mine - 1ull;
             for (int q = 0; q < kDenseGroups; q++) pending &= pending - 1ull;
             const bool act = mine != 0ull;
             const uint64_t slot = m0 + (uint64_t)(act ? __builtin_ctzll(mine) : 0);
@@ -642,7 +666,7 @@ __device__ __forceinline__ void dense_pass(const SolverConsts &c, const Dom &dom
                 lds.s0[g][lg] = row_start(d);
                 lds.cum[g][lg + 1] = max(row_count(d), 0);
             }
-            __syncthreads();
+            wave_sync();
             if (lg == 0) {
                 int32_t run = 0;
                 lds.cum[g][0] = 0;
@@ -651,16 +675,17 @@ __device__ __forceinline__ void dense_pass(const SolverConsts &c, const Dom &dom
                     lds.cum[g][o + 1] = run;
                 }
             }
-            __syncthreads();
+            wave_sync();
             uint32_t lc = LIST_NONE;
             if (!Pass::RECORD && act && list_entries) lc = list_count[i];
             const bool listed = !Pass::RECORD && lc != LIST_NONE;
             if (lg == 0) lds.total[g] = !act ? 0 : (listed ? (int32_t)lc : lds.cum[g][9]);
-            __syncthreads();
+            wave_sync();
             const int32_t cn = lds.total[g];
             int32_t rounds = 0;
-    #pragma unroll
-            for (int q = 0; q < kDenseGroups; q++) rounds = max(rounds, (lds.total[q] + kDenseSlots - 1) / kDenseSlots);
+#pragma unroll
+            for (int q = 0; q < kDenseGroups; q++)
+                rounds = max(rounds, (lds.total[g0 + q] + kDenseSlots - 1) / kDenseSlots);
             ps.begin(i, sg);
             float acc = 0.0f;
             int32_t kept = 0;
@@ -670,7 +695,7 @@ __device__ __forceinline__ void dense_pass(const SolverConsts &c, const Dom &dom
                 int32_t j[kDenseBatch];
                 uint32_t code[kDenseBatch];
                 bool valid[kDenseBatch];
-    #pragma unroll
+#pragma unroll
                 for (int b = 0; b < kDenseBatch; b++) {
                     const int32_t k = r * kDenseSlots + b * kDenseGroup + lg;
                     valid[b] = k < cn;
@@ -688,41 +713,41 @@ __device__ __forceinline__ void dense_pass(const SolverConsts &c, const Dom &dom
                 }
                 float4 pj[kDenseBatch];
                 X x[kDenseBatch];
-    #pragma unroll
+#pragma unroll
                 for (int b = 0; b < kDenseBatch; b++) {
                     pj[b] = valid[b] ? rec_at<PAD>(rec, sg, j[b]) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                     x[b] = valid[b] ? ps.extra(sg, j[b]) : X();
                 }
                 uint64_t mask = 0;
                 bool in[kDenseBatch];
-    #pragma unroll
+#pragma unroll
                 for (int b = 0; b < kDenseBatch; b++) {
-                    float P[NP], M[NM > 0 ? NM : 1];
-    #pragma unroll
+                    float P[NP], M[NM > 0 ? NM : 1];  // M[a]: accumulator NP - NM + a
+#pragma unroll
                     for (int a = 0; a < NP; a++) P[a] = 0.0f;
-    #pragma unroll
+#pragma unroll
                     for (int a = 0; a < (NM > 0 ? NM : 1); a++) M[a] = 0.0f;
                     in[b] = valid[b] && ps.eval(pj[b], x[b], listed, P, M);
                     const int slot = b * kDenseGroup + lg;
-    #pragma unroll
+#pragma unroll
                     for (int a = 0; a < NP; a++)
                         reinterpret_cast<float *>(lds.p[g][a])[slot] = in[b] ? P[a] : 0.0f;
                     if (NM > 0) {
-    #pragma unroll
+#pragma unroll
                         for (int a = 0; a < NM; a++)
                             reinterpret_cast<float *>(lds.m[g][a])[slot] = in[b] ? M[a] : 0.0f;
                     }
-                    mask |= group_bits(__ballot(in[b]), g) << (kDenseGroup * b);
+                    mask |= group_bits(__ballot(in[b]), g - g0) << (kDenseGroup * b);
                 }
                 if (Pass::RECORD && list_entries) {
-    #pragma unroll
+#pragma unroll
                     for (int b = 0; b < kDenseBatch; b++) {
                         const int slot = b * kDenseGroup + lg;
                         const int32_t pos = kept + __popcll(mask & ((1ull << slot) - 1ull));
                         if (in[b] && pos < list_cap) list_entries[(size_t)pos * c.n + i] = (uint16_t)code[b];
                     }
                 }
-                __syncthreads();
+                wave_sync();
                 // Slots without an in-range pair hold +0.0, and an accumulator
                 // that starts at +0.0 is never -0.0 (round to nearest), so adding
                 // them changes nothing: every slot up to the round's last
@@ -732,7 +757,10 @@ __device__ __forceinline__ void dense_pass(const SolverConsts &c, const Dom &dom
                     for (int q = 0; q < (used + 3) / 4; q++) {
                         const float4 v = lds.p[g][lg][q];
                         if (NM > 0) {
-                            const float4 w = lds.m[g][NM > 0 ? lg : 0][q];
+                            // the first NP - NM accumulators subtract +0.0
+                            const float4 w0 = lds.m[g][max(lg - (NP - NM), 0)][q];
+                            const bool has_m = lg >= NP - NM;
+                            const float4 w = has_m ? w0 : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                             acc = (acc + v.x) - w.x;
                             acc = (acc + v.y) - w.y;
                             acc = (acc + v.z) - w.z;
@@ -746,15 +774,15 @@ __device__ __forceinline__ void dense_pass(const SolverConsts &c, const Dom &dom
                     }
                 }
                 kept += __popcll(mask);
-                __syncthreads();
+                wave_sync();
             }
             if (lg < NP) lds.acc[g][lg] = acc;
-            __syncthreads();
+            wave_sync();
             if (act && lg == 0) {
                 ps.finish(i, sg, lds.acc[g]);
                 if (Pass::RECORD && list_entries) list_count[i] = kept <= list_cap ? (uint16_t)kept : LIST_NONE;
             }
-            __syncthreads();
+            wave_sync();
         }
     }
 }
@@ -842,7 +870,7 @@ struct UpdatePosPass {
 // vorticity.glsl:34-64 for the dense particles (terms of k_vorticity_a).
 template <bool H1, bool PAD>
 struct VorticityAPass {
-    static constexpr int NP = 6, NM = 6;
+    static constexpr int NP = 6, NM = 3;
     static constexpr bool RECORD = true;
     const SolverConsts &c;
     const float4 *__restrict__ rec;
@@ -874,11 +902,11 @@ struct VorticityAPass {
         float gx, gy, gz;
         grad_spiky<H1>(c, px, py, pz, r, gx, gy, gz);
         P[3] = uy * gz;
-        M[3] = uz * gy;
+        M[0] = uz * gy;
         P[4] = uz * gx;
-        M[4] = ux * gz;
+        M[1] = ux * gz;
         P[5] = ux * gy;
-        M[5] = uy * gx;
+        M[2] = uy * gx;
         return true;
     }
     __device__ __forceinline__ void finish(int32_t i, const Seg &sg, const float *a) const {
@@ -948,12 +976,12 @@ __global__ __launch_bounds__(NB) void k_lambda(SolverConsts c, Dom dom, const fl
                                                      DenseList dl, uint32_t dense_blocks) {
     __shared__ union {
         NbrLds<NB> light;
-        DenseLds<5, 0> dense;
+        DenseLds<5, 0, NB / 64> dense;
     } u;
     if constexpr (std::is_same<Dom, DomOne>::value) {
         if (blockIdx.x < dense_blocks) {
             LambdaPass<H1, PAD> ps{c, rec, lambda, {}};
-            dense_pass<LambdaPass<H1, PAD>, PAD>(c, dom, rec, nbr9, dl, list_entries, list_count, list_cap, ps,
+            dense_pass<LambdaPass<H1, PAD>, PAD, NB / 64>(c, dom, rec, nbr9, dl, list_entries, list_count, list_cap, ps,
                                                  u.dense, blockIdx.x, dense_blocks);
             return;
         }
@@ -1003,19 +1031,19 @@ __global__ __launch_bounds__(NB) void k_updatepos(SolverConsts c, Dom dom, const
                                                         float4 *__restrict__ rec_out, DenseList dl,
                                                         uint32_t dense_blocks) {
     __shared__ union {
-        NbrLds<NB> light;
-        DenseLds<3, 0> dense;
+        NbrLds<NB, kReplayLcap> light;
+        DenseLds<3, 0, NB / 64> dense;
     } u;
     if constexpr (std::is_same<Dom, DomOne>::value) {
         if (blockIdx.x < dense_blocks) {
             UpdatePosPass<H1, PAD> ps{c, rec_in, lambda, rec_out, {}, 0.0f};
-            dense_pass<UpdatePosPass<H1, PAD>, PAD>(c, dom, rec_in, nbr9, dl, const_cast<uint16_t *>(list_entries),
+            dense_pass<UpdatePosPass<H1, PAD>, PAD, NB / 64>(c, dom, rec_in, nbr9, dl, const_cast<uint16_t *>(list_entries),
                                                     const_cast<uint16_t *>(list_count), 0, ps, u.dense, blockIdx.x,
                                                     dense_blocks);
             return;
         }
     }
-    NbrLds<NB> &lds = u.light;
+    NbrLds<NB, kReplayLcap> &lds = u.light;
     const int32_t i = xcd_block<NB>(blockIdx.x - dense_blocks, gridDim.x - dense_blocks) * NB + threadIdx.x;
     if (i >= c.n) return;
     if (leave_dense(load_rows<NB>(c, nbr9, i, lds), dl)) return;
@@ -1081,12 +1109,12 @@ __global__ __launch_bounds__(NB) void k_vorticity_a(SolverConsts c, Dom dom, con
                                                           DenseList dl, uint32_t dense_blocks) {
     __shared__ union {
         NbrLds<NB> light;
-        DenseLds<6, 6> dense;
+        DenseLds<6, 3, NB / 64> dense;
     } u;
     if constexpr (std::is_same<Dom, DomOne>::value) {
         if (blockIdx.x < dense_blocks) {
             VorticityAPass<H1, PAD> ps{c, rec, vel_sorted, vel_id, vxsph, omega, vort, {}, {}};
-            dense_pass<VorticityAPass<H1, PAD>, PAD>(c, dom, rec, nbr9, dl, list_entries, list_count, list_cap, ps,
+            dense_pass<VorticityAPass<H1, PAD>, PAD, NB / 64>(c, dom, rec, nbr9, dl, list_entries, list_count, list_cap, ps,
                                                      u.dense, blockIdx.x, dense_blocks);
             return;
         }
@@ -1141,19 +1169,19 @@ __global__ __launch_bounds__(NB) void k_vorticity_b(SolverConsts c, Dom dom, con
                                                           float4 *__restrict__ vel_id, DenseList dl,
                                                           uint32_t dense_blocks) {
     __shared__ union {
-        NbrLds<NB> light;
-        DenseLds<3, 0> dense;
+        NbrLds<NB, kReplayLcap> light;
+        DenseLds<3, 0, NB / 64> dense;
     } u;
     if constexpr (std::is_same<Dom, DomOne>::value) {
         if (blockIdx.x < dense_blocks) {
             VorticityBPass<H1, PAD> ps{c, rec, vxsph, omega, vort, vel_id, {}};
-            dense_pass<VorticityBPass<H1, PAD>, PAD>(c, dom, rec, nbr9, dl, const_cast<uint16_t *>(list_entries),
+            dense_pass<VorticityBPass<H1, PAD>, PAD, NB / 64>(c, dom, rec, nbr9, dl, const_cast<uint16_t *>(list_entries),
                                                      const_cast<uint16_t *>(list_count), 0, ps, u.dense, blockIdx.x,
                                                      dense_blocks);
             return;
         }
     }
-    NbrLds<NB> &lds = u.light;
+    NbrLds<NB, kReplayLcap> &lds = u.light;
     const int32_t i = xcd_block<NB>(blockIdx.x - dense_blocks, gridDim.x - dense_blocks) * NB + threadIdx.x;
     if (i >= c.n) return;
     if (leave_dense(load_rows<NB>(c, nbr9, i, lds), dl)) return;
@@ -1277,14 +1305,20 @@ unsigned nb_blocks(int32_t n, int nb) { return (unsigned)((n + nb - 1) / nb); }
 #ifndef PBF_NB_ONE
 #define PBF_NB_ONE 64
 #endif
+#ifndef PBF_NB_REPLAY
+#define PBF_NB_REPLAY 64
+#endif
 static_assert(PBF_NB_ONE == 64 || PBF_NB_ONE == 128 || PBF_NB_ONE == 256, "neighbour workgroup width");
-template <class Dom, class F>
+static_assert(PBF_NB_REPLAY == 64 || PBF_NB_REPLAY == 128 || PBF_NB_REPLAY == 256, "neighbour workgroup width");
+// REPLAY: the replaying passes' width for one scene (their LDS is smaller).
+template <class Dom, bool REPLAY = false, class F>
 void with_nb(int nb, bool pad, F &&f) {
     if constexpr (std::is_same<Dom, DomOne>::value) {
+        constexpr int W = REPLAY ? PBF_NB_REPLAY : PBF_NB_ONE;
         if (pad)
-            f(std::integral_constant<int, PBF_NB_ONE>(), std::true_type());
+            f(std::integral_constant<int, W>(), std::true_type());
         else
-            f(std::integral_constant<int, PBF_NB_ONE>(), std::false_type());
+            f(std::integral_constant<int, W>(), std::false_type());
     } else {
         switch (nb) {
             case 64: f(std::integral_constant<int, 64>(), std::false_type()); break;
@@ -1317,10 +1351,11 @@ void neighbourcells_impl(const SolverConsts &c, const Dom &d, const LaunchShape 
 }
 
 // Workgroups of a neighbour launch given to the dense list (none without one).
-unsigned dense_blocks_of(const DenseList *dl, int32_t n) {
+unsigned dense_blocks_of(const DenseList *dl, int32_t n, int nb) {
     if (!dl) return 0u;
-    const int64_t need = ((int64_t)n + kDenseGroups - 1) / kDenseGroups;
-    return (unsigned)std::min<int64_t>(kDenseBlocks, (need + 7) / 8 * 8);
+    const int64_t waves = std::min<int64_t>(kDenseWaves, ((int64_t)n + kDenseGroups - 1) / kDenseGroups);
+    const int64_t wgs = (waves + nb / 64 - 1) / (nb / 64);
+    return (unsigned)((wgs + 7) / 8 * 8);
 }
 template <class Dom>
 void update_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, const float4 *rec, float4 *pos,
@@ -1333,10 +1368,10 @@ void lambda_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, boo
                  const DenseList *dl = nullptr) {
     if (c.n <= 0) return;
     const DenseList dlv = dl ? *dl : DenseList();
-    const unsigned db = std::is_same<Dom, DomOne>::value ? dense_blocks_of(dl, c.n) : 0u;
     with_nb<Dom>(ls.nb, pad, [&](auto nbc, auto padc) {
         constexpr int NB = decltype(nbc)::value;
         constexpr bool PAD = decltype(padc)::value;
+        const unsigned db = std::is_same<Dom, DomOne>::value ? dense_blocks_of(dl, c.n, NB) : 0u;
         const unsigned grid = nb_blocks(c.n, NB) + db;
         if (c.h_is_one)
             k_lambda<true, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec, nbr9, lambda, pl.entries, pl.count, pl.cap,
@@ -1352,10 +1387,10 @@ void updatepos_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, 
                     const DenseList *dl = nullptr) {
     if (c.n <= 0) return;
     const DenseList dlv = dl ? *dl : DenseList();
-    const unsigned db = std::is_same<Dom, DomOne>::value ? dense_blocks_of(dl, c.n) : 0u;
-    with_nb<Dom>(ls.nb, pad, [&](auto nbc, auto padc) {
+    with_nb<Dom, true>(ls.nb, pad, [&](auto nbc, auto padc) {
         constexpr int NB = decltype(nbc)::value;
         constexpr bool PAD = decltype(padc)::value;
+        const unsigned db = std::is_same<Dom, DomOne>::value ? dense_blocks_of(dl, c.n, NB) : 0u;
         const unsigned grid = nb_blocks(c.n, NB) + db;
         if (c.h_is_one)
             k_updatepos<true, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec_in, lambda, nbr9, pl.entries, pl.count,
@@ -1371,10 +1406,10 @@ void vorticity_a_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls
                       float4 *omega, float *vort, const PairList &pl, hipStream_t st, const DenseList *dl = nullptr) {
     if (c.n <= 0) return;
     const DenseList dlv = dl ? *dl : DenseList();
-    const unsigned db = std::is_same<Dom, DomOne>::value ? dense_blocks_of(dl, c.n) : 0u;
     with_nb<Dom>(ls.nb, pad, [&](auto nbc, auto padc) {
         constexpr int NB = decltype(nbc)::value;
         constexpr bool PAD = decltype(padc)::value;
+        const unsigned db = std::is_same<Dom, DomOne>::value ? dense_blocks_of(dl, c.n, NB) : 0u;
         const unsigned grid = nb_blocks(c.n, NB) + db;
         if (c.h_is_one)
             k_vorticity_a<true, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec, nbr9, vel_sorted, vel_id, vxsph, omega,
@@ -1390,10 +1425,10 @@ void vorticity_b_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls
                       const float *vort, float4 *vel_id, hipStream_t st, const DenseList *dl = nullptr) {
     if (c.n <= 0) return;
     const DenseList dlv = dl ? *dl : DenseList();
-    const unsigned db = std::is_same<Dom, DomOne>::value ? dense_blocks_of(dl, c.n) : 0u;
-    with_nb<Dom>(ls.nb, pad, [&](auto nbc, auto padc) {
+    with_nb<Dom, true>(ls.nb, pad, [&](auto nbc, auto padc) {
         constexpr int NB = decltype(nbc)::value;
         constexpr bool PAD = decltype(padc)::value;
+        const unsigned db = std::is_same<Dom, DomOne>::value ? dense_blocks_of(dl, c.n, NB) : 0u;
         const unsigned grid = nb_blocks(c.n, NB) + db;
         if (c.h_is_one)
             k_vorticity_b<true, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec, nbr9, pl.entries, pl.count, vxsph, omega,
