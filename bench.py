@@ -259,6 +259,8 @@ def main():
     ap.add_argument("--workload", default="block_1m", choices=["block_1m", "dam_32k", "stress_4m"])
     ap.add_argument("--window", type=int, default=100, help="steps per reported window of the timed run")
     ap.add_argument("--no-kernel-timing", action="store_true", help="skip the per-kernel HIP-event run")
+    ap.add_argument("--eager", action="store_true",
+                    help="launch every kernel of a step from the host instead of replaying the step's hipGraph")
     ap.add_argument("--cpu-steps", type=int, default=20, help="oracle steps of the CPU baseline sample (0: skip)")
     ap.add_argument("--batched-steps", type=int, default=100, help="steps of the batched mix (0: skip)")
     ap.add_argument("--tune-steps", type=int, default=2, help="steps per autotuner trial of the batched mix")
@@ -282,6 +284,9 @@ def main():
         f"dt={scene.params.timestep}, {steps} timed steps from the initial state")
     sph = P.SPH(n, scene.grid)
     sph.set_params(P.scene_params(scene))
+    # each step one replay of a hipGraph of its launches (same kernels and
+    # work; the per-kernel timing run below launches eagerly)
+    sph.set_graph(not args.eager)
     torch.cuda.init()
     reset(sph, scene)
     for _ in range(args.warmup):
@@ -353,7 +358,8 @@ def main():
                                f"rollout from the initial state"
                    if scene.name == "block_1m" else scene.name,
                    "particles": n, "grid": list(scene.grid), "iterations": scene.params.iterations,
-                   "parallelism": "single GPU", "ms_per_step_windows": [round(w, 3) for w in windows]},
+                   "parallelism": "single GPU", "launch": "eager" if args.eager else "graph replay per step",
+                   "ms_per_step_windows": [round(w, 3) for w in windows]},
         "step_roofline": {"alg_bytes_per_particle_step": b_alg, "frac": round(step_frac, 5)},
         "roofline": roofline,
         "cpu_baseline": cpu,

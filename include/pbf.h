@@ -123,6 +123,13 @@ int pbf_sph_download_state(pbf_sph *s, float *pos4, float *vel4, int host);
 int pbf_sph_run(pbf_sph *s);
 /* nsteps consecutive SPH::Run calls. */
 int pbf_sph_step(pbf_sph *s, int32_t nsteps);
+/* No reference counterpart: enable != 0 makes every later step one replay of
+ * a hipGraph holding the step's launches (captured by the first such step on
+ * the handle's stream; recaptured after pbf_sph_set_params or a stream
+ * change), which removes most of the per-launch overhead of a step. Results
+ * are those of eager steps bit for bit. Phase timing (pbf_sph_timing) needs
+ * eager steps; per-kernel timing switches replay off while it is on. */
+int pbf_sph_set_graph(pbf_sph *s, int enable);
 /* SPH::GetPositionBuffer (src/SPH.h:49-51): float4 positions in id order,
  * stride 16 bytes, as PointSprite::SetPositionBuffer consumes them
  * (src/PointSprite.cpp:64-73). */

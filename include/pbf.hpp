@@ -88,6 +88,9 @@ public:
     // SPH::Run (src/SPH.cpp:246-334): one step, asynchronous on the stream.
     void Run() { detail::check(pbf_sph_run(h_), "SPH::Run"); }
 
+    // No reference counterpart: run each step as one hipGraph replay.
+    void SetGraphReplay(bool enable) { detail::check(pbf_sph_set_graph(h_, enable ? 1 : 0), "SPH::SetGraphReplay"); }
+
     // SPH::OutputTiming (src/SPH.cpp:218-240): the five phases of the last step.
     pbf_timing OutputTiming() {
         pbf_timing t;

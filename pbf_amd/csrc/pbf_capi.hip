@@ -218,9 +218,25 @@ struct pbf_sph {
     RadixScratch rs;
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     bool stepped = false;
+    bool graph_step = false;                     // the last step ran as a graph replay
     KernelTimer kt;
+    // Graph replay of one step (pbf_sph_set_graph): captured on first use on
+    // graph_stream, dropped when parameters or the stream change.
+    bool use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t graph_stream = nullptr;
+
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        graph_stream = nullptr;
+    }
 
     void release() {
+        drop_graph();
         kt.release();
         dfree(pos);
         dfree(vel);
@@ -412,6 +428,8 @@ int pbf_sph_set_params(pbf_sph *s, const pbf_params *p) {
     if (!s) return invalid("null SPH handle");
     int rc = check_params(p);
     if (rc) return rc;
+    if (s->exec) PBF_HIP(hipStreamSynchronize(s->stream));
+    s->drop_graph();  // the captured launches carry the old constants
     s->params = *p;
     s->consts = make_consts(s->n, s->grid, s->params, s->key_bits);
     return PBF_OK;
@@ -441,32 +459,37 @@ int pbf_sph_download_state(pbf_sph *s, float *pos4, float *vel4, int host) {
     return PBF_OK;
 }
 
-int pbf_sph_run(pbf_sph *s) {
-    if (!s) return invalid("null SPH handle");
-    hipStream_t st = s->stream;
+namespace {
+// One step's launches on st (SPH::Run, src/SPH.cpp:246-334); with events, the
+// five phase events and the kernel timer's brackets are recorded too.
+int sph_enqueue(pbf_sph *s, hipStream_t st, bool events) {
     const SolverConsts &c = s->consts;
     const int32_t n = (int32_t)s->n;
     KernelTimer &kt = s->kt;
 #define TIMED(id, launch)                    \
     do {                                     \
-        PBF_HIP(kt.begin(id, st));           \
+        if (events) PBF_HIP(kt.begin(id, st)); \
         launch;                              \
-        PBF_HIP(kt.end(st));                 \
+        if (events) PBF_HIP(kt.end(st));     \
     } while (0)
-    PBF_HIP(hipEventRecord(s->ev[0], st));
+#define PHASE(k)                                              \
+    do {                                                      \
+        if (events) PBF_HIP(hipEventRecord(s->ev[k], st));    \
+    } while (0)
+    PHASE(0);
     // predict positions (src/SPH.cpp:247-261) + cell keys
     TIMED(KID_PREDICT, launch_predict(c, s->pos, s->vel, s->xs, s->keys, st));
-    PBF_HIP(hipEventRecord(s->ev[1], st));
+    PHASE(1);
     // sort (src/SPH.cpp:263-268)
     TIMED(KID_SORT, radix_sort_perm(s->rs, s->keys, n, s->key_bits, s->perm, st));
     TIMED(KID_GATHER, launch_gather_records(n, s->perm, s->xs, s->rec0, st));
-    PBF_HIP(hipEventRecord(s->ev[2], st));
+    PHASE(2);
     // neighbour cells (src/SPH.cpp:270-275; src/NeighbourCellFinder.cpp:113-148)
     CellTables &t = s->cells_t;
     TIMED(KID_CLEAR_START, launch_begin_cells(t.start, t.cells, t.epoch, st, &s->dense));
     TIMED(KID_FINDCELLS, launch_findcells(c, s->rec0, t.start, t.end64, t.epoch, st));
     TIMED(KID_NEIGHBOURCELLS, launch_neighbourcells(c, s->rec0, t.start, t.end64, t.nbr9, st, &s->dense));
-    PBF_HIP(hipEventRecord(s->ev[3], st));
+    PHASE(3);
     // solver iterations (src/SPH.cpp:303-311), race-free: x* -> a -> b -> a ...
     PairList pl;
     pl.entries = s->list_entries;
@@ -480,7 +503,7 @@ int pbf_sph_run(pbf_sph *s) {
         cur = dst;
     }
     s->rec = const_cast<float4 *>(cur);
-    PBF_HIP(hipEventRecord(s->ev[4], st));
+    PHASE(4);
     // update + vorticity (src/SPH.cpp:315-333)
     TIMED(KID_UPDATE, launch_update(c, s->rec, s->pos, s->vel, s->params.vorticity ? s->vel_sorted : nullptr, st));
     if (s->params.vorticity) {
@@ -491,9 +514,51 @@ int pbf_sph_run(pbf_sph *s) {
               launch_vorticity_b(c, s->rec, t.nbr9, pl, s->vxsph, s->omega, s->vort, s->vel, st, true, &s->dense));
     }
 #undef TIMED
-    PBF_HIP(hipEventRecord(s->ev[5], st));
+    PHASE(5);
+#undef PHASE
     PBF_HIP(hipGetLastError());
+    return PBF_OK;
+}
+}  // namespace
+
+int pbf_sph_run(pbf_sph *s) {
+    if (!s) return invalid("null SPH handle");
+    if (!s->use_graph || s->kt.on) {
+        const int rc = sph_enqueue(s, s->stream, true);
+        if (rc) return rc;
+        s->stepped = true;
+        s->graph_step = false;
+        return PBF_OK;
+    }
+    if (!s->exec || s->graph_stream != s->stream) {
+        s->drop_graph();
+        PBF_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = sph_enqueue(s, s->stream, false);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(s->stream, &g);
+        if (rc || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc ? rc : hip_fail(e, "pbf_sph_run: capture");
+        }
+        s->graph = g;
+        const hipError_t ei = hipGraphInstantiate(&s->exec, g, nullptr, nullptr, 0);
+        if (ei != hipSuccess) {
+            s->drop_graph();
+            return hip_fail(ei, "pbf_sph_run: instantiate");
+        }
+        s->graph_stream = s->stream;
+    }
+    PBF_HIP(hipGraphLaunch(s->exec, s->stream));
     s->stepped = true;
+    s->graph_step = true;
+    return PBF_OK;
+}
+
+int pbf_sph_set_graph(pbf_sph *s, int enable) {
+    if (!s) return invalid("null SPH handle");
+    if (s->exec) PBF_HIP(hipStreamSynchronize(s->graph_stream));
+    s->drop_graph();
+    s->use_graph = enable != 0;
     return PBF_OK;
 }
 
@@ -525,6 +590,7 @@ int pbf_sph_timing(pbf_sph *s, pbf_timing *out) {
     if (!s || !out) return invalid("null argument");
     std::memset(out, 0, sizeof *out);
     if (!s->stepped) return PBF_OK;
+    if (s->graph_step) return invalid("phase timing needs an eager step (graph replay is on)");
     PBF_HIP(hipEventSynchronize(s->ev[5]));
     float ms[5];
     for (int k = 0; k < 5; k++) PBF_HIP(hipEventElapsedTime(&ms[k], s->ev[k], s->ev[k + 1]));

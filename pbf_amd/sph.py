@@ -214,6 +214,10 @@ class SPH:
         check(self._lib.pbf_sph_timing(self._h, C.byref(t)))
         return {k: getattr(t, k) for k, _ in PbfTiming._fields_}
 
+    def set_graph(self, enable: bool):
+        """Run each later step as one hipGraph replay (pbf_sph_set_graph)."""
+        check(self._lib.pbf_sph_set_graph(self._h, int(bool(enable))))
+
     def set_kernel_timing(self, enable: bool):
         check(self._lib.pbf_sph_set_kernel_timing(self._h, int(bool(enable))))
 
