@@ -48,12 +48,12 @@ ALG_BYTES = {
 # lambda writes its pair lists (2 B per in-range neighbour) and updatepos reads
 # them back; both also read the 9-row neighbour table (36 B per particle).
 MEASURED_TRAFFIC_BLOCK_1M = {
-    "lambda": 134.2,
-    "updatepos": 154.2,
-    "vorticity_a": 203.0,
-    "vorticity_b": 191.9,
-    "update": 86.2,
-    "neighbourcells": 60.6,
+    "lambda": 150.7,
+    "updatepos": 176.9,
+    "vorticity_a": 224.5,
+    "vorticity_b": 217.3,
+    "update": 86.4,
+    "neighbourcells": 63.5,
     "predict": 52.0,
 }
 
