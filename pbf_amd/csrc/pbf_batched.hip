@@ -44,6 +44,7 @@ struct pbf_batch {
     int32_t *start = nullptr, *nbr9 = nullptr;
     unsigned long long *end64 = nullptr;
     uint32_t *epoch = nullptr;
+    DenseList dense;  // the step's particles with the most candidates (all scenes)
     RadixScratch rs;
     bool use_graph = false;
     hipGraph_t graph = nullptr;
@@ -80,6 +81,9 @@ struct pbf_batch {
         dfree(nbr9);
         dfree(end64);
         dfree(epoch);
+        dfree(dense.cand);
+        dfree(dense.levels);
+        dfree(dense.threshold);
         radix_free(rs);
         if (own_stream) (void)hipStreamDestroy(own_stream);
     }
@@ -92,9 +96,9 @@ struct pbf_batch {
         launch_predict(c, d, shape, pos, vel, xs, keys, st);
         radix_sort_perm(rs, keys, nn, key_bits, perm, st);
         launch_gather_records(nn, perm, xs, rec0, st);
-        launch_begin_cells(start, cells, epoch, st);
+        launch_begin_cells(start, cells, epoch, st, &dense);
         launch_findcells(c, d, shape, rec0, start, end64, epoch, st);
-        launch_neighbourcells(c, d, shape, rec0, start, end64, nbr9, st);
+        launch_neighbourcells(c, d, shape, rec0, start, end64, nbr9, st, &dense);
         PairList pl;
         pl.entries = list_entries;
         pl.count = list_count;
@@ -102,14 +106,14 @@ struct pbf_batch {
         const float4 *cur = rec0;
         for (uint32_t it = 0; it < params.iterations; it++) {
             float4 *dst = (it & 1) ? rec_b : rec_a;
-            launch_lambda(c, d, shape, cur, nbr9, lambda, pl, st);
-            launch_updatepos(c, d, shape, cur, lambda, nbr9, pl, dst, st);
+            launch_lambda(c, d, shape, cur, nbr9, lambda, pl, st, &dense);
+            launch_updatepos(c, d, shape, cur, lambda, nbr9, pl, dst, st, &dense);
             cur = dst;
         }
         launch_update(c, d, shape, cur, pos, vel, params.vorticity ? vel_sorted : nullptr, st);
         if (params.vorticity) {
-            launch_vorticity_a(c, d, shape, cur, nbr9, vel_sorted, vel, vxsph, omega, vort, pl, st);
-            launch_vorticity_b(c, d, shape, cur, nbr9, pl, vxsph, omega, vort, vel, st);
+            launch_vorticity_a(c, d, shape, cur, nbr9, vel_sorted, vel, vxsph, omega, vort, pl, st, &dense);
+            launch_vorticity_b(c, d, shape, cur, nbr9, pl, vxsph, omega, vort, vel, st, &dense);
         }
     }
 };
@@ -227,6 +231,10 @@ int pbf_batch_create(pbf_batch **out, uint32_t nscenes, const pbf_scene_desc *sc
     TRY(dalloc(&b->nbr9, (size_t)9 * n));
     TRY(dalloc(&b->end64, b->cells));
     TRY(dalloc(&b->epoch, 1));
+    TRY(dalloc(&b->dense.cand, n));
+    TRY(dalloc(&b->dense.levels, (size_t)kDenseLevels * kDenseCopies));
+    TRY(dalloc(&b->dense.threshold, 2));
+    TRY(hipMemset(b->dense.levels, 0, (size_t)kDenseLevels * kDenseCopies * sizeof(uint32_t)));
     if (!radix_alloc(b->rs, (int32_t)n)) TRY(hipErrorOutOfMemory);
     TRY(hipMemcpy(b->segs, segs.data(), nscenes * sizeof(Seg), hipMemcpyHostToDevice));
     TRY(hipMemcpy(b->scene_of, scene_of.data(), n * sizeof(uint16_t), hipMemcpyHostToDevice));

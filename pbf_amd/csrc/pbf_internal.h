@@ -150,20 +150,23 @@ void launch_predict(const SolverConsts &c, const DomMany &d, const LaunchShape &
 void launch_findcells(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *sorted,
                       int32_t *start, unsigned long long *end64, const uint32_t *epoch, hipStream_t st);
 void launch_neighbourcells(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *sorted,
-                           const int32_t *start, const unsigned long long *end64, int32_t *nbr9, hipStream_t st);
+                           const int32_t *start, const unsigned long long *end64, int32_t *nbr9, hipStream_t st,
+                           const DenseList *dl = nullptr);
 void launch_lambda(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec,
-                   const int32_t *nbr9, float *lambda, const PairList &pl, hipStream_t st);
+                   const int32_t *nbr9, float *lambda, const PairList &pl, hipStream_t st,
+                   const DenseList *dl = nullptr);
 void launch_updatepos(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec_in,
                       const float *lambda, const int32_t *nbr9, const PairList &pl, float4 *rec_out,
-                      hipStream_t st);
+                      hipStream_t st, const DenseList *dl = nullptr);
 void launch_update(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec, float4 *pos,
                    float4 *vel, float4 *vel_sorted, hipStream_t st);
 void launch_vorticity_a(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec,
                         const int32_t *nbr9, const float4 *vel_sorted, const float4 *vel_id, float4 *vxsph,
-                        float4 *omega, float *vort, const PairList &pl, hipStream_t st);
+                        float4 *omega, float *vort, const PairList &pl, hipStream_t st,
+                        const DenseList *dl = nullptr);
 void launch_vorticity_b(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec,
                         const int32_t *nbr9, const PairList &pl, const float4 *vxsph, const float4 *omega,
-                        const float *vort, float4 *vel_id, hipStream_t st);
+                        const float *vort, float4 *vel_id, hipStream_t st, const DenseList *dl = nullptr);
 void launch_nbr9_to_12(int32_t n, const int32_t *nbr9, int32_t *nbr12, hipStream_t st);
 void launch_nbr12_to_9(int32_t n, const int32_t *nbr12, int32_t *nbr9, hipStream_t st);
 void launch_end64_to_i32(size_t cells, const unsigned long long *end64, int32_t *end, hipStream_t st);

@@ -978,7 +978,7 @@ __global__ __launch_bounds__(NB) void k_lambda(SolverConsts c, Dom dom, const fl
         NbrLds<NB> light;
         DenseLds<5, 0, NB / 64> dense;
     } u;
-    if constexpr (std::is_same<Dom, DomOne>::value) {
+    {
         if (blockIdx.x < dense_blocks) {
             LambdaPass<H1, PAD> ps{c, rec, lambda, {}};
             dense_pass<LambdaPass<H1, PAD>, PAD, NB / 64>(c, dom, rec, nbr9, dl, list_entries, list_count, list_cap, ps,
@@ -1034,7 +1034,7 @@ __global__ __launch_bounds__(NB) void k_updatepos(SolverConsts c, Dom dom, const
         NbrLds<NB, kReplayLcap> light;
         DenseLds<3, 0, NB / 64> dense;
     } u;
-    if constexpr (std::is_same<Dom, DomOne>::value) {
+    {
         if (blockIdx.x < dense_blocks) {
             UpdatePosPass<H1, PAD> ps{c, rec_in, lambda, rec_out, {}, 0.0f};
             dense_pass<UpdatePosPass<H1, PAD>, PAD, NB / 64>(c, dom, rec_in, nbr9, dl, const_cast<uint16_t *>(list_entries),
@@ -1111,7 +1111,7 @@ __global__ __launch_bounds__(NB) void k_vorticity_a(SolverConsts c, Dom dom, con
         NbrLds<NB> light;
         DenseLds<6, 3, NB / 64> dense;
     } u;
-    if constexpr (std::is_same<Dom, DomOne>::value) {
+    {
         if (blockIdx.x < dense_blocks) {
             VorticityAPass<H1, PAD> ps{c, rec, vel_sorted, vel_id, vxsph, omega, vort, {}, {}};
             dense_pass<VorticityAPass<H1, PAD>, PAD, NB / 64>(c, dom, rec, nbr9, dl, list_entries, list_count, list_cap, ps,
@@ -1172,7 +1172,7 @@ __global__ __launch_bounds__(NB) void k_vorticity_b(SolverConsts c, Dom dom, con
         NbrLds<NB, kReplayLcap> light;
         DenseLds<3, 0, NB / 64> dense;
     } u;
-    if constexpr (std::is_same<Dom, DomOne>::value) {
+    {
         if (blockIdx.x < dense_blocks) {
             VorticityBPass<H1, PAD> ps{c, rec, vxsph, omega, vort, vel_id, {}};
             dense_pass<VorticityBPass<H1, PAD>, PAD, NB / 64>(c, dom, rec, nbr9, dl, const_cast<uint16_t *>(list_entries),
@@ -1371,7 +1371,7 @@ void lambda_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, boo
     with_nb<Dom>(ls.nb, pad, [&](auto nbc, auto padc) {
         constexpr int NB = decltype(nbc)::value;
         constexpr bool PAD = decltype(padc)::value;
-        const unsigned db = std::is_same<Dom, DomOne>::value ? dense_blocks_of(dl, c.n, NB) : 0u;
+        const unsigned db = dense_blocks_of(dl, c.n, NB);
         const unsigned grid = nb_blocks(c.n, NB) + db;
         if (c.h_is_one)
             k_lambda<true, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec, nbr9, lambda, pl.entries, pl.count, pl.cap,
@@ -1390,7 +1390,7 @@ void updatepos_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, 
     with_nb<Dom, true>(ls.nb, pad, [&](auto nbc, auto padc) {
         constexpr int NB = decltype(nbc)::value;
         constexpr bool PAD = decltype(padc)::value;
-        const unsigned db = std::is_same<Dom, DomOne>::value ? dense_blocks_of(dl, c.n, NB) : 0u;
+        const unsigned db = dense_blocks_of(dl, c.n, NB);
         const unsigned grid = nb_blocks(c.n, NB) + db;
         if (c.h_is_one)
             k_updatepos<true, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec_in, lambda, nbr9, pl.entries, pl.count,
@@ -1409,7 +1409,7 @@ void vorticity_a_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls
     with_nb<Dom>(ls.nb, pad, [&](auto nbc, auto padc) {
         constexpr int NB = decltype(nbc)::value;
         constexpr bool PAD = decltype(padc)::value;
-        const unsigned db = std::is_same<Dom, DomOne>::value ? dense_blocks_of(dl, c.n, NB) : 0u;
+        const unsigned db = dense_blocks_of(dl, c.n, NB);
         const unsigned grid = nb_blocks(c.n, NB) + db;
         if (c.h_is_one)
             k_vorticity_a<true, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec, nbr9, vel_sorted, vel_id, vxsph, omega,
@@ -1428,7 +1428,7 @@ void vorticity_b_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls
     with_nb<Dom, true>(ls.nb, pad, [&](auto nbc, auto padc) {
         constexpr int NB = decltype(nbc)::value;
         constexpr bool PAD = decltype(padc)::value;
-        const unsigned db = std::is_same<Dom, DomOne>::value ? dense_blocks_of(dl, c.n, NB) : 0u;
+        const unsigned db = dense_blocks_of(dl, c.n, NB);
         const unsigned grid = nb_blocks(c.n, NB) + db;
         if (c.h_is_one)
             k_vorticity_b<true, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec, nbr9, pl.entries, pl.count, vxsph, omega,
@@ -1504,17 +1504,18 @@ void launch_findcells(const SolverConsts &c, const DomMany &d, const LaunchShape
     findcells_impl(c, d, ls, sorted, start, end64, epoch, st);
 }
 void launch_neighbourcells(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *sorted,
-                           const int32_t *start, const unsigned long long *end64, int32_t *nbr9, hipStream_t st) {
-    neighbourcells_impl(c, d, ls, sorted, start, end64, nbr9, st);
+                           const int32_t *start, const unsigned long long *end64, int32_t *nbr9, hipStream_t st,
+                           const DenseList *dl) {
+    neighbourcells_impl(c, d, ls, sorted, start, end64, nbr9, st, dl);
 }
 void launch_lambda(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec,
-                   const int32_t *nbr9, float *lambda, const PairList &pl, hipStream_t st) {
-    lambda_impl(c, d, ls, false, rec, nbr9, lambda, pl, st);
+                   const int32_t *nbr9, float *lambda, const PairList &pl, hipStream_t st, const DenseList *dl) {
+    lambda_impl(c, d, ls, false, rec, nbr9, lambda, pl, st, dl);
 }
 void launch_updatepos(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec_in,
                       const float *lambda, const int32_t *nbr9, const PairList &pl, float4 *rec_out,
-                      hipStream_t st) {
-    updatepos_impl(c, d, ls, false, rec_in, lambda, nbr9, pl, rec_out, st);
+                      hipStream_t st, const DenseList *dl) {
+    updatepos_impl(c, d, ls, false, rec_in, lambda, nbr9, pl, rec_out, st, dl);
 }
 void launch_update(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec, float4 *pos,
                    float4 *vel, float4 *vel_sorted, hipStream_t st) {
@@ -1522,13 +1523,13 @@ void launch_update(const SolverConsts &c, const DomMany &d, const LaunchShape &l
 }
 void launch_vorticity_a(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec,
                         const int32_t *nbr9, const float4 *vel_sorted, const float4 *vel_id, float4 *vxsph,
-                        float4 *omega, float *vort, const PairList &pl, hipStream_t st) {
-    vorticity_a_impl(c, d, ls, false, rec, nbr9, vel_sorted, vel_id, vxsph, omega, vort, pl, st);
+                        float4 *omega, float *vort, const PairList &pl, hipStream_t st, const DenseList *dl) {
+    vorticity_a_impl(c, d, ls, false, rec, nbr9, vel_sorted, vel_id, vxsph, omega, vort, pl, st, dl);
 }
 void launch_vorticity_b(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec,
                         const int32_t *nbr9, const PairList &pl, const float4 *vxsph, const float4 *omega,
-                        const float *vort, float4 *vel_id, hipStream_t st) {
-    vorticity_b_impl(c, d, ls, false, rec, nbr9, pl, vxsph, omega, vort, vel_id, st);
+                        const float *vort, float4 *vel_id, hipStream_t st, const DenseList *dl) {
+    vorticity_b_impl(c, d, ls, false, rec, nbr9, pl, vxsph, omega, vort, vel_id, st, dl);
 }
 void launch_nbr9_to_12(int32_t n, const int32_t *nbr9, int32_t *nbr12, hipStream_t st) {
     if (n > 0) k_nbr9_to_12<<<blocks_for(n), BLOCK, 0, st>>>(n, nbr9, nbr12);
