@@ -468,35 +468,55 @@ __device__ __forceinline__ void scan_neighbours(const SolverConsts &c, const Seg
         }
         cnt = 0;
     };
-    for (int o = 0; o < 9; o++) {
-        const int32_t s0 = lds.rowstart[o * NB + t], e = (int32_t)lds.rowcount[o * NB + t];
+    // Appends go through slot, the LDS index of the column's next entry
+    // (cnt * NB + t), advanced by NB per kept candidate.
+    auto scan_row = [&](int o, int32_t s0, int32_t e, int32_t selfq, auto check_self) {
+        constexpr bool SELF = decltype(check_self)::value;
         // Loads stay in bounds: with PAD, a row's batches end at most
         // 126 + kBatch entries past n (start < n, count < 128), inside the
-        // zero padding;
-        // without, indices past the row or past the scene repeat the last valid
-        // one, and candidates past the scene are the zero record.
+        // zero padding; without, indices past the row or past the scene
+        // repeat the last valid one, and candidates past the scene are the
+        // zero record.
         const int32_t last = max(min(s0 + e, sg.n) - 1, 0);
         const float4 *__restrict__ row = srec + s0;
+        int slot = cnt * NB + t;
         for (int32_t q = 0; q < e; q += kBatch) {
             // All lanes flush together when one column is full: a per-lane
             // branch would run the flush once for every subset of lanes that
             // fills up at a different candidate, which in a dense clump is
             // most of the wave's time. Each lane still evaluates its own list
             // in order.
-            if (wave_any(cnt > LC - kBatch)) flush();
+            if (wave_any(slot > (LC - kBatch) * NB + t)) {
+                cnt = (slot - t) / NB;
+                flush();
+                slot = t;
+            }
             const int32_t j0 = s0 + q;
             float4 p[kBatch];
 #pragma unroll
             for (int u = 0; u < kBatch; u++) p[u] = PAD ? row[q + u] : srec[min(j0 + u, last)];
 #pragma unroll
             for (int u = 0; u < kBatch; u++) {
-                const int32_t j = j0 + u;
-                const float4 pj = (PAD || j < sg.n) ? p[u] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                const bool f = (q + u < e) & (j != li) & maybe_near(c, pi, pj);
-                lds.list[cnt * NB + t] = (uint16_t)((o << 8) | (q + u));
-                cnt += f;
+                const float4 pj = (PAD || j0 + u < sg.n) ? p[u] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                bool f = (q + u < e) & maybe_near(c, pi, pj);
+                if (SELF) f = f & (q + u != selfq);
+                lds.list[slot] = (uint16_t)((o << 8) | (q + u));
+                slot += f ? NB : 0;
             }
         }
+        cnt = (slot - t) / NB;
+    };
+    for (int o = 0; o < 9; o++) {
+        const int32_t s0 = lds.rowstart[o * NB + t], e = (int32_t)lds.rowcount[o * NB + t];
+        // The particle's own slot, when this row holds it (the centre row,
+        // or any row in the reference's boundary cases): the one candidate
+        // skipped besides the out-of-range ones. Rows without it in any lane
+        // of the wave skip the test.
+        const int32_t selfq = (li >= s0 && li - s0 < e) ? li - s0 : -1;
+        if (wave_any(selfq >= 0))
+            scan_row(o, s0, e, selfq, std::true_type());
+        else
+            scan_row(o, s0, e, selfq, std::false_type());
     }
     flush();
 }
