@@ -606,7 +606,11 @@ __device__ __forceinline__ void replay_neighbours(const SolverConsts &c, const S
 // adds the round's in-range terms in visiting order. Every sum is the same
 // sequence of the same operations, so results equal the one-thread passes bit
 // for bit. The pair lists are recorded and replayed in the same format.
-constexpr int kDenseGroup = 16;
+#ifndef PBF_DENSE_GROUP
+#define PBF_DENSE_GROUP 16
+#endif
+constexpr int kDenseGroup = PBF_DENSE_GROUP;
+static_assert(kDenseGroup == 8 || kDenseGroup == 16, "dense group width (a round's slots fit a 64-bit mask)");
 constexpr int kDenseBatch = 4;
 constexpr int kDenseSlots = kDenseGroup * kDenseBatch;
 constexpr int kDenseGroups = 64 / kDenseGroup;
@@ -681,10 +685,10 @@ __device__ __forceinline__ void dense_pass(const SolverConsts &c, const Dom &dom
             const int32_t i = act ? (int32_t)((uint64_t)block + (uint64_t)blocks * slot) : 0;
             const Seg sg = dom.at(i);
             const int32_t li = i - sg.off;
-            if (lg < 9) {
-                const int32_t d = act ? nbr9[(size_t)lg * c.n + i] : -1;  // -1: no candidates
-                lds.s0[g][lg] = row_start(d);
-                lds.cum[g][lg + 1] = max(row_count(d), 0);
+            for (int o = lg; o < 9; o += kDenseGroup) {
+                const int32_t d = act ? nbr9[(size_t)o * c.n + i] : -1;  // -1: no candidates
+                lds.s0[g][o] = row_start(d);
+                lds.cum[g][o + 1] = max(row_count(d), 0);
             }
             wave_sync();
             if (lg == 0) {
