@@ -130,6 +130,10 @@ int pbf_sph_step(pbf_sph *s, int32_t nsteps);
  * are those of eager steps bit for bit. Phase timing (pbf_sph_timing) needs
  * eager steps; per-kernel timing switches replay off while it is on. */
 int pbf_sph_set_graph(pbf_sph *s, int enable);
+/* No reference counterpart: the candidate threshold of the last step above
+ * which particles took the dense-particle workgroups, and how many did
+ * (synchronizes the handle's stream). Results do not depend on either. */
+int pbf_sph_dense_info(pbf_sph *s, int32_t *threshold, int32_t *count);
 /* SPH::GetPositionBuffer (src/SPH.h:49-51): float4 positions in id order,
  * stride 16 bytes, as PointSprite::SetPositionBuffer consumes them
  * (src/PointSprite.cpp:64-73). */

@@ -70,6 +70,7 @@ SIGNATURES = {
     "pbf_sph_run": ([vp], C.c_int),
     "pbf_sph_step": ([vp, i32], C.c_int),
     "pbf_sph_set_graph": ([vp, C.c_int], C.c_int),
+    "pbf_sph_dense_info": ([vp, vp, vp], C.c_int),
     "pbf_sph_positions_device": ([vp, C.POINTER(vp), C.POINTER(C.c_size_t)], C.c_int),
     "pbf_sph_velocities_device": ([vp, C.POINTER(vp), C.POINTER(C.c_size_t)], C.c_int),
     "pbf_sph_timing": ([vp, C.POINTER(PbfTiming)], C.c_int),

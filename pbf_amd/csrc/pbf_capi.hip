@@ -554,6 +554,18 @@ int pbf_sph_run(pbf_sph *s) {
     return PBF_OK;
 }
 
+int pbf_sph_dense_info(pbf_sph *s, int32_t *threshold, int32_t *count) {
+    if (!s || !threshold || !count) return invalid("null argument");
+    int32_t v[2] = {0, 0};
+    if (s->stepped) {
+        PBF_HIP(hipStreamSynchronize(s->stream));
+        PBF_HIP(hipMemcpy(v, s->dense.threshold, sizeof v, hipMemcpyDeviceToHost));
+    }
+    *threshold = v[0];
+    *count = v[1];
+    return PBF_OK;
+}
+
 int pbf_sph_set_graph(pbf_sph *s, int enable) {
     if (!s) return invalid("null SPH handle");
     if (s->exec) PBF_HIP(hipStreamSynchronize(s->graph_stream));

@@ -218,6 +218,12 @@ class SPH:
         """Run each later step as one hipGraph replay (pbf_sph_set_graph)."""
         check(self._lib.pbf_sph_set_graph(self._h, int(bool(enable))))
 
+    def dense_info(self):
+        """(threshold, count) of the last step's dense-particle split (pbf_sph_dense_info)."""
+        th, cnt = C.c_int32(), C.c_int32()
+        check(self._lib.pbf_sph_dense_info(self._h, C.byref(th), C.byref(cnt)))
+        return th.value, cnt.value
+
     def set_kernel_timing(self, enable: bool):
         check(self._lib.pbf_sph_set_kernel_timing(self._h, int(bool(enable))))
 
