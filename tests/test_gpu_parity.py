@@ -162,6 +162,28 @@ def test_block_1m_one_step(gpu_lib):
     assert np.array_equal(np.sort(ids), np.arange(sc.n)), "sorted ids are a permutation"
 
 
+def test_stress_4m_one_step(gpu_lib):
+    """The 4,194,304-particle stress scene (grid 384x192x384: 26-bit keys, four
+    sort passes, keys above 2^24 that the reference's FP32 hash rounds; a dense
+    block under uniform-random particles, 0 to 216 candidates per particle): one
+    step bit-exact against the oracle, with its sorted records and neighbour table."""
+    sc = S.cfg_stress_4m()
+    say(f"[stress_4m] n={sc.n}")
+    sph, pos, vel = run_sph(sc, 1)
+    cfg = O.cfg_for(sc)
+    wpos, wvel, _, _, st = O.step(cfg, sc.pos, sc.vel, None, None, O.ORDER_JACOBI, stages=True)
+    # records after the solver iterations, in sorted order: the sort
+    # permutation (ids in w) and the last iteration's positions
+    assert_bitexact(sph.sorted_records(), st["solved"][-1], "stress_4m solved records in sorted order")
+    assert_bitexact(sph.sorted_records()[:, 3], st["sorted"][:, 3], "stress_4m sort permutation")
+    assert_bitexact(sph.neighbour_table()[:, :, :3], st["nbr"][:, :, :3], "stress_4m neighbour table")
+    assert_bitexact(pos, wpos, "stress_4m positions")
+    assert_bitexact(vel, wvel, "stress_4m velocities")
+    th, cnt = sph.dense_info()
+    say(f"[stress_4m] dense threshold {th}, {cnt} dense particles")
+    sph.close()
+
+
 def test_error_paths(gpu_lib):
     """C-ABI errors: bad sizes and odd sort blocks fail with the reference's error kinds."""
     with pytest.raises(P._lib.PbfError):
