@@ -388,6 +388,7 @@ static_assert(LCAP % 8 == 0 && kReplayLcap % 8 == 0, "list column of whole batch
 #endif
 constexpr int kBatch = PBF_BATCH;
 static_assert(LCAP >= 2 * kBatch && kReplayLcap >= 2 * kBatch, "list column must hold at least two batches");
+static_assert(kBatch % 2 == 0, "codes are written in pairs");
 static_assert(126 + kBatch <= kPad, "batches past a row stay inside the zero padding");
 
 template <int NB, int LC = LCAP>
@@ -468,8 +469,11 @@ __device__ __forceinline__ void scan_neighbours(const SolverConsts &c, const Seg
         }
         cnt = 0;
     };
-    // Appends go through slot, the LDS index of the column's next entry
-    // (cnt * NB + t), advanced by NB per kept candidate.
+    // Appends go through boff, the byte offset in the list of the column's
+    // next entry (2 (cnt NB + t)), advanced by 2 NB per kept candidate; the
+    // codes of two consecutive candidates share one register (low and high
+    // halves), since the code is the same in every lane.
+    char *const lbase = reinterpret_cast<char *>(lds.list);
     auto scan_row = [&](int o, int32_t s0, int32_t e, int32_t selfq, auto check_self) {
         constexpr bool SELF = decltype(check_self)::value;
         // Loads stay in bounds: with PAD, a row's batches end at most
@@ -479,32 +483,43 @@ __device__ __forceinline__ void scan_neighbours(const SolverConsts &c, const Seg
         // zero record.
         const int32_t last = max(min(s0 + e, sg.n) - 1, 0);
         const float4 *__restrict__ row = srec + s0;
-        int slot = cnt * NB + t;
+        uint32_t boff = 2u * (uint32_t)(cnt * NB + t);
+        const uint32_t bfull = 2u * (uint32_t)((LC - kBatch) * NB + t);
         for (int32_t q = 0; q < e; q += kBatch) {
             // All lanes flush together when one column is full: a per-lane
             // branch would run the flush once for every subset of lanes that
             // fills up at a different candidate, which in a dense clump is
             // most of the wave's time. Each lane still evaluates its own list
             // in order.
-            if (wave_any(slot > (LC - kBatch) * NB + t)) {
-                cnt = (slot - t) / NB;
+            if (wave_any(boff > bfull)) {
+                cnt = (int)(boff / 2u - (uint32_t)t) / NB;
                 flush();
-                slot = t;
+                boff = 2u * (uint32_t)t;
             }
             const int32_t j0 = s0 + q;
             float4 p[kBatch];
 #pragma unroll
             for (int u = 0; u < kBatch; u++) p[u] = PAD ? row[q + u] : srec[min(j0 + u, last)];
+            const uint32_t code0 = (uint32_t)((o << 8) | q);
 #pragma unroll
-            for (int u = 0; u < kBatch; u++) {
-                const float4 pj = (PAD || j0 + u < sg.n) ? p[u] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                bool f = (q + u < e) & maybe_near(c, pi, pj);
-                if (SELF) f = f & (q + u != selfq);
-                lds.list[slot] = (uint16_t)((o << 8) | (q + u));
-                slot += f ? NB : 0;
+            for (int u = 0; u < kBatch; u += 2) {
+                const uint32_t two_s = (code0 + (uint32_t)u) | ((code0 + (uint32_t)u + 1u) << 16);
+                // one register move for both codes; the high half is stored
+                // from the same register (ds_write_b16_d16_hi)
+                uint32_t two;
+                asm("v_mov_b32 %0, %1" : "=v"(two) : "s"(two_s));
+#pragma unroll
+                for (int v = 0; v < 2; v++) {
+                    const float4 pj =
+                        (PAD || j0 + u + v < sg.n) ? p[u + v] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    bool f = (q + u + v < e) & maybe_near(c, pi, pj);
+                    if (SELF) f = f & (q + u + v != selfq);
+                    *reinterpret_cast<uint16_t *>(lbase + boff) = (uint16_t)(v ? two >> 16 : two);
+                    boff += f ? 2u * NB : 0u;
+                }
             }
         }
-        cnt = (slot - t) / NB;
+        cnt = (int)(boff / 2u - (uint32_t)t) / NB;
     };
     for (int o = 0; o < 9; o++) {
         const int32_t s0 = lds.rowstart[o * NB + t], e = (int32_t)lds.rowcount[o * NB + t];
