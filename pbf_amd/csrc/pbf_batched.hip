@@ -103,14 +103,20 @@ struct pbf_batch {
         pl.entries = list_entries;
         pl.count = list_count;
         pl.cap = kListCap;
+        // the last position update also performs the closing update
+        StepUpdate up;
+        up.pos = pos;
+        up.vel = vel;
+        up.vel_sorted = params.vorticity ? vel_sorted : nullptr;
         const float4 *cur = rec0;
         for (uint32_t it = 0; it < params.iterations; it++) {
             float4 *dst = (it & 1) ? rec_b : rec_a;
             launch_lambda(c, d, shape, cur, nbr9, lambda, pl, st, &dense);
-            launch_updatepos(c, d, shape, cur, lambda, nbr9, pl, dst, st, &dense);
+            launch_updatepos(c, d, shape, cur, lambda, nbr9, pl, dst, st, &dense,
+                             it + 1 == params.iterations ? &up : nullptr);
             cur = dst;
         }
-        launch_update(c, d, shape, cur, pos, vel, params.vorticity ? vel_sorted : nullptr, st);
+        if (params.iterations == 0) launch_update(c, d, shape, cur, pos, vel, up.vel_sorted, st);
         if (params.vorticity) {
             launch_vorticity_a(c, d, shape, cur, nbr9, vel_sorted, vel, vxsph, omega, vort, pl, st, &dense);
             launch_vorticity_b(c, d, shape, cur, nbr9, pl, vxsph, omega, vort, vel, st, &dense);

@@ -495,17 +495,25 @@ int sph_enqueue(pbf_sph *s, hipStream_t st, bool events) {
     pl.entries = s->list_entries;
     pl.count = s->list_count;
     pl.cap = kListCap;
+    // The last position update also performs the step's closing update
+    // (src/SPH.cpp:315-318) for each particle as its final position is known.
+    StepUpdate up;
+    up.pos = s->pos;
+    up.vel = s->vel;
+    up.vel_sorted = s->params.vorticity ? s->vel_sorted : nullptr;
     const float4 *cur = s->rec0;
-    for (uint32_t it = 0; it < s->params.iterations; it++) {
+    const uint32_t iters = s->params.iterations;
+    for (uint32_t it = 0; it < iters; it++) {
         float4 *dst = (it & 1) ? s->rec_b : s->rec_a;
         TIMED(KID_LAMBDA, launch_lambda(c, cur, t.nbr9, s->lambda, pl, st, true, &s->dense));
-        TIMED(KID_UPDATEPOS, launch_updatepos(c, cur, s->lambda, t.nbr9, pl, dst, st, true, &s->dense));
+        TIMED(KID_UPDATEPOS, launch_updatepos(c, cur, s->lambda, t.nbr9, pl, dst, st, true, &s->dense,
+                                              it + 1 == iters ? &up : nullptr));
         cur = dst;
     }
     s->rec = const_cast<float4 *>(cur);
     PHASE(4);
-    // update + vorticity (src/SPH.cpp:315-333)
-    TIMED(KID_UPDATE, launch_update(c, s->rec, s->pos, s->vel, s->params.vorticity ? s->vel_sorted : nullptr, st));
+    // update (when no iteration ran) + vorticity (src/SPH.cpp:315-333)
+    if (iters == 0) TIMED(KID_UPDATE, launch_update(c, s->rec, s->pos, s->vel, up.vel_sorted, st));
     if (s->params.vorticity) {
         TIMED(KID_VORTICITY_A,
               launch_vorticity_a(c, s->rec, t.nbr9, s->vel_sorted, s->vel, s->vxsph, s->omega, s->vort, pl, st, true,

@@ -122,9 +122,19 @@ void launch_dense_select(const DenseList &dl, hipStream_t st);
 // one-thread-per-particle path.
 void launch_lambda(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, float *lambda, const PairList &pl,
                    hipStream_t st, bool padded = false, const DenseList *dl = nullptr);
+// The step's closing update (update.glsl:16-28: velocities from the
+// positions before and after the step, both stored by particle id; vel_sorted,
+// if set, also keeps the velocities in sorted order). Given to the last
+// position update of a step, which performs it for each particle as soon as
+// that particle's final position is known, in place of launch_update.
+struct StepUpdate {
+    float4 *pos = nullptr;
+    float4 *vel = nullptr;
+    float4 *vel_sorted = nullptr;
+};
 void launch_updatepos(const SolverConsts &c, const float4 *rec_in, const float *lambda, const int32_t *nbr9,
                       const PairList &pl, float4 *rec_out, hipStream_t st, bool padded = false,
-                      const DenseList *dl = nullptr);
+                      const DenseList *dl = nullptr, const StepUpdate *up = nullptr);
 void launch_update(const SolverConsts &c, const float4 *rec, float4 *pos, float4 *vel, float4 *vel_sorted,
                    hipStream_t st);
 void launch_vorticity_a(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, const float4 *vel_sorted,
@@ -157,7 +167,7 @@ void launch_lambda(const SolverConsts &c, const DomMany &d, const LaunchShape &l
                    const DenseList *dl = nullptr);
 void launch_updatepos(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec_in,
                       const float *lambda, const int32_t *nbr9, const PairList &pl, float4 *rec_out,
-                      hipStream_t st, const DenseList *dl = nullptr);
+                      hipStream_t st, const DenseList *dl = nullptr, const StepUpdate *up = nullptr);
 void launch_update(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec, float4 *pos,
                    float4 *vel, float4 *vel_sorted, hipStream_t st);
 void launch_vorticity_a(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec,

@@ -864,6 +864,17 @@ struct LambdaPass {
 };
 
 // updatepos.glsl:42-105 for the dense particles (terms of k_updatepos).
+// update.glsl:16-28 for sorted slot i with final record r (scene sg).
+__device__ __forceinline__ void update_one(const SolverConsts &c, const Seg &sg, int32_t i, const float4 &r,
+                                           const StepUpdate &up) {
+    const int32_t id = sg.off + id_of(r);
+    const float4 old = up.pos[id];
+    const float4 v = make_float4((r.x - old.x) / c.dt, (r.y - old.y) / c.dt, (r.z - old.z) / c.dt, 0.0f);
+    up.pos[id] = make_float4(r.x, r.y, r.z, 0.0f);
+    up.vel[id] = v;
+    if (up.vel_sorted) up.vel_sorted[i] = v;
+}
+
 template <bool H1, bool PAD>
 struct UpdatePosPass {
     static constexpr int NP = 3, NM = 0;
@@ -872,6 +883,7 @@ struct UpdatePosPass {
     const float4 *__restrict__ rec_in;
     const float *__restrict__ lambda;
     float4 *__restrict__ rec_out;
+    StepUpdate up;  // the step's closing update (last iteration), or none
     float4 pi;
     float li;
     __device__ __forceinline__ void begin(int32_t i, const Seg &) {
@@ -902,7 +914,9 @@ struct UpdatePosPass {
         x = clamp_glsl(x, sg.lo[0], sg.hi[0]);
         y = clamp_glsl(y, sg.lo[1], sg.hi[1]);
         z = clamp_glsl(z, sg.lo[2], sg.hi[2]);
-        rec_out[i] = make_float4(x, y, z, pi.w);
+        const float4 r = make_float4(x, y, z, pi.w);
+        rec_out[i] = r;
+        if (up.pos) update_one(c, sg, i, r, up);
     }
 };
 
@@ -1068,14 +1082,14 @@ __global__ __launch_bounds__(NB) void k_updatepos(SolverConsts c, Dom dom, const
                                                         const uint16_t *__restrict__ list_entries,
                                                         const uint16_t *__restrict__ list_count,
                                                         float4 *__restrict__ rec_out, DenseList dl,
-                                                        uint32_t dense_blocks) {
+                                                        uint32_t dense_blocks, StepUpdate up) {
     __shared__ union {
         NbrLds<NB, kReplayLcap> light;
         DenseLds<3, 0, NB / 64> dense;
     } u;
     {
         if (blockIdx.x < dense_blocks) {
-            UpdatePosPass<H1, PAD> ps{c, rec_in, lambda, rec_out, {}, 0.0f};
+            UpdatePosPass<H1, PAD> ps{c, rec_in, lambda, rec_out, up, {}, 0.0f};
             dense_pass<UpdatePosPass<H1, PAD>, PAD, NB / 64>(c, dom, rec_in, nbr9, dl, const_cast<uint16_t *>(list_entries),
                                                     const_cast<uint16_t *>(list_count), 0, ps, u.dense, blockIdx.x,
                                                     dense_blocks);
@@ -1113,24 +1127,18 @@ __global__ __launch_bounds__(NB) void k_updatepos(SolverConsts c, Dom dom, const
     x = clamp_glsl(x, sg.lo[0], sg.hi[0]);
     y = clamp_glsl(y, sg.lo[1], sg.hi[1]);
     z = clamp_glsl(z, sg.lo[2], sg.hi[2]);
-    rec_out[i] = make_float4(x, y, z, pi.w);
+    const float4 r = make_float4(x, y, z, pi.w);
+    rec_out[i] = r;
+    if (up.pos) update_one(c, sg, i, r, up);
 }
 
 // update.glsl:16-28; also keeps the velocities in sorted order for the
 // vorticity passes.
 template <class Dom>
 __global__ __launch_bounds__(1024) void k_update(SolverConsts c, Dom dom, int32_t ppt, const float4 *__restrict__ rec,
-                                                 float4 *__restrict__ pos, float4 *__restrict__ vel,
-                                                 float4 *__restrict__ vel_sorted) {
+                                                 StepUpdate up) {
     PBF_STREAM_FOR(i) {
-        const Seg sg = dom.at(i);
-        const float4 r = rec[i];
-        const int32_t id = sg.off + id_of(r);
-        const float4 old = pos[id];
-        const float4 v = make_float4((r.x - old.x) / c.dt, (r.y - old.y) / c.dt, (r.z - old.z) / c.dt, 0.0f);
-        pos[id] = make_float4(r.x, r.y, r.z, 0.0f);
-        vel[id] = v;
-        if (vel_sorted) vel_sorted[i] = v;
+        update_one(c, dom.at(i), i, rec[i], up);
     }
 }
 
@@ -1399,7 +1407,11 @@ unsigned dense_blocks_of(const DenseList *dl, int32_t n, int nb) {
 template <class Dom>
 void update_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, const float4 *rec, float4 *pos,
                  float4 *vel, float4 *vel_sorted, hipStream_t st) {
-    if (c.n > 0) k_update<Dom><<<stream_blocks(c.n, ls), ls.sb, 0, st>>>(c, d, ls.ppt, rec, pos, vel, vel_sorted);
+    StepUpdate up;
+    up.pos = pos;
+    up.vel = vel;
+    up.vel_sorted = vel_sorted;
+    if (c.n > 0) k_update<Dom><<<stream_blocks(c.n, ls), ls.sb, 0, st>>>(c, d, ls.ppt, rec, up);
 }
 template <class Dom>
 void lambda_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, bool pad, const float4 *rec,
@@ -1423,9 +1435,10 @@ void lambda_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, boo
 template <class Dom>
 void updatepos_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, bool pad, const float4 *rec_in,
                     const float *lambda, const int32_t *nbr9, const PairList &pl, float4 *rec_out, hipStream_t st,
-                    const DenseList *dl = nullptr) {
+                    const DenseList *dl = nullptr, const StepUpdate *up = nullptr) {
     if (c.n <= 0) return;
     const DenseList dlv = dl ? *dl : DenseList();
+    const StepUpdate upv = up ? *up : StepUpdate();
     with_nb<Dom, true>(ls.nb, pad, [&](auto nbc, auto padc) {
         constexpr int NB = decltype(nbc)::value;
         constexpr bool PAD = decltype(padc)::value;
@@ -1433,10 +1446,10 @@ void updatepos_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, 
         const unsigned grid = nb_blocks(c.n, NB) + db;
         if (c.h_is_one)
             k_updatepos<true, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec_in, lambda, nbr9, pl.entries, pl.count,
-                                                                 rec_out, dlv, db);
+                                                                 rec_out, dlv, db, upv);
         else
             k_updatepos<false, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec_in, lambda, nbr9, pl.entries, pl.count,
-                                                                  rec_out, dlv, db);
+                                                                  rec_out, dlv, db, upv);
     });
 }
 template <class Dom>
@@ -1515,8 +1528,9 @@ void launch_lambda(const SolverConsts &c, const float4 *rec, const int32_t *nbr9
     lambda_impl(c, one(c), kDefaultShape, padded, rec, nbr9, lambda, pl, st, dl);
 }
 void launch_updatepos(const SolverConsts &c, const float4 *rec_in, const float *lambda, const int32_t *nbr9,
-                      const PairList &pl, float4 *rec_out, hipStream_t st, bool padded, const DenseList *dl) {
-    updatepos_impl(c, one(c), kDefaultShape, padded, rec_in, lambda, nbr9, pl, rec_out, st, dl);
+                      const PairList &pl, float4 *rec_out, hipStream_t st, bool padded, const DenseList *dl,
+                      const StepUpdate *up) {
+    updatepos_impl(c, one(c), kDefaultShape, padded, rec_in, lambda, nbr9, pl, rec_out, st, dl, up);
 }
 void launch_update(const SolverConsts &c, const float4 *rec, float4 *pos, float4 *vel, float4 *vel_sorted,
                    hipStream_t st) {
@@ -1553,8 +1567,8 @@ void launch_lambda(const SolverConsts &c, const DomMany &d, const LaunchShape &l
 }
 void launch_updatepos(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec_in,
                       const float *lambda, const int32_t *nbr9, const PairList &pl, float4 *rec_out,
-                      hipStream_t st, const DenseList *dl) {
-    updatepos_impl(c, d, ls, false, rec_in, lambda, nbr9, pl, rec_out, st, dl);
+                      hipStream_t st, const DenseList *dl, const StepUpdate *up) {
+    updatepos_impl(c, d, ls, false, rec_in, lambda, nbr9, pl, rec_out, st, dl, up);
 }
 void launch_update(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec, float4 *pos,
                    float4 *vel, float4 *vel_sorted, hipStream_t st) {
