@@ -46,13 +46,14 @@ ALG_BYTES = {
 # rocprofv3 FETCH_SIZE and WRITE_SIZE counters over steps 1-2 of the rollout,
 # scaled by the counters of a plain 256 MiB copy (profiles/hbm_traffic_block_1m.md).
 # lambda writes its pair lists (2 B per in-range neighbour) and updatepos reads
-# them back; both also read the 9-row neighbour table (36 B per particle).
+# them back; both also read the 9-row neighbour table (36 B per particle). The
+# last updatepos launch of each step also performs the closing update (pos and
+# vel by particle id), so updatepos is the mean over launches with and without.
 MEASURED_TRAFFIC_BLOCK_1M = {
-    "lambda": 150.7,
-    "updatepos": 176.9,
-    "vorticity_a": 224.5,
-    "vorticity_b": 217.3,
-    "update": 86.4,
+    "lambda": 150.8,
+    "updatepos": 196.2,
+    "vorticity_a": 224.4,
+    "vorticity_b": 217.0,
     "neighbourcells": 63.5,
     "predict": 52.0,
 }
