@@ -40,15 +40,23 @@ __device__ __forceinline__ float sqrt_rn_fast(float x) {
 // condition, so that rare fallbacks cost no per-lane mask bookkeeping.
 __device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
 
+// Lane mask of a single comparison: the mask the comparison itself produces.
+// Combining such masks with scalar operations, rather than taking the ballot
+// of a combined predicate, saves the two vector instructions the compiler
+// otherwise spends turning the predicate into a register and back.
+__device__ __forceinline__ uint64_t lanes(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+__device__ __forceinline__ uint64_t active_lanes() { return __builtin_amdgcn_read_exec(); }
+
 // The empty volatile asm keeps the compiler from speculating the general
 // sequence into every call (it would otherwise if-convert the rare branch and
 // run both square roots for every operand).
 __device__ __forceinline__ float sqrt_rn(float x) {
     float r = sqrt_rn_fast(x);
-    const bool slow = !(x >= kSqrtLo) | !(x < __builtin_inff());
-    if (wave_any(slow)) {
+    // some lane has x outside [kSqrtLo, inf): the fast lanes' masks do not
+    // cover every active lane
+    if ((lanes(x >= kSqrtLo) & lanes(x < __builtin_inff())) != active_lanes()) {
         asm volatile("" ::: "memory");
-        if (slow) r = sqrtf(x);
+        if (!(x >= kSqrtLo) | !(x < __builtin_inff())) r = sqrtf(x);
     }
     return r;
 }
@@ -99,7 +107,20 @@ __device__ __forceinline__ void div3_rn(float a0, float a1, float a2, float b, f
     const bool fast = BOUNDED ? (b >= kDenLo) & fast_numerator_lo(a0) & fast_numerator_lo(a1) & fast_numerator_lo(a2)
                               : (b >= kDenLo) & (b <= kDenHi) & fast_numerator(a0) & fast_numerator(a1) &
                                     fast_numerator(a2);
-    if (wave_any(!fast)) {
+    // the same test as wave_any(!fast), from the lane masks of the single
+    // comparisons (zero or >= kNumLo, per numerator)
+    uint64_t ok = lanes(b >= kDenLo);
+    if (BOUNDED) {
+        ok &= (lanes(fabsf(a0) == 0.0f) | lanes(fabsf(a0) >= kNumLo)) &
+              (lanes(fabsf(a1) == 0.0f) | lanes(fabsf(a1) >= kNumLo)) &
+              (lanes(fabsf(a2) == 0.0f) | lanes(fabsf(a2) >= kNumLo));
+    } else {
+        ok &= lanes(b <= kDenHi);
+        ok &= (lanes(fabsf(a0) == 0.0f) | (lanes(fabsf(a0) >= kNumLo) & lanes(fabsf(a0) <= kNumHi))) &
+              (lanes(fabsf(a1) == 0.0f) | (lanes(fabsf(a1) >= kNumLo) & lanes(fabsf(a1) <= kNumHi))) &
+              (lanes(fabsf(a2) == 0.0f) | (lanes(fabsf(a2) >= kNumLo) & lanes(fabsf(a2) <= kNumHi)));
+    }
+    if (ok != active_lanes()) {
         asm volatile("" ::: "memory");
         if (!fast) {
             q0 = a0 / b;
