@@ -343,15 +343,20 @@ __global__ __launch_bounds__(kDenseSelectThreads) void k_dense_select(DenseList 
 // buffer access does.
 // PAD: the arrays hold at least kPad zero entries after the last particle
 // (one scene), so indices past n read the zero record without a test.
+// Without padding the load is of a clamped index and the zero selected after
+// it: a load under the condition would be a branch, which keeps the loads of a
+// batch from being in flight together.
 template <bool PAD>
 __device__ __forceinline__ float4 rec_at(const float4 *__restrict__ rec, const Seg &sg, int32_t j) {
     if (PAD) return rec[j];
-    return j < sg.n ? rec[sg.off + j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4 r = rec[sg.off + min(j, sg.n - 1)];
+    return j < sg.n ? r : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 template <bool PAD>
 __device__ __forceinline__ float val_at(const float *__restrict__ a, const Seg &sg, int32_t j) {
     if (PAD) return a[j];
-    return j < sg.n ? a[sg.off + j] : 0.0f;
+    const float v = a[sg.off + min(j, sg.n - 1)];
+    return j < sg.n ? v : 0.0f;
 }
 
 // Neighbour iteration in two phases. FOR_EACH_NEIGHBOUR (foreachneighbour.glsl:
@@ -580,9 +585,15 @@ __device__ __forceinline__ void replay_neighbours(const SolverConsts &c, const S
     using X = decltype(extra(0));
     // The entries of the next batch are requested before this batch's
     // neighbours, so each batch waits for one round trip rather than two.
+    // Every load is unconditional, so that a batch's loads are in flight
+    // together (a load under a per-lane condition is a branch, and the
+    // compiler waits for each before the next): entry indices past the list
+    // repeat its last entry (index 0 for an empty list, whose loop does not
+    // run), so every code and neighbour index is a valid one.
+    const uint32_t lastk = cnt > 0 ? cnt - 1u : 0u;
     uint32_t next[kBatch];
 #pragma unroll
-    for (int u = 0; u < kBatch; u++) next[u] = (uint32_t)u < cnt ? entries[(size_t)u * c.n + i] : 0u;
+    for (int u = 0; u < kBatch; u++) next[u] = entries[(size_t)min((uint32_t)u, lastk) * c.n + i];
     for (uint32_t k = 0; k < cnt; k += kBatch) {
         uint32_t e[kBatch];
         int32_t j[kBatch];
@@ -591,15 +602,13 @@ __device__ __forceinline__ void replay_neighbours(const SolverConsts &c, const S
 #pragma unroll
         for (int u = 0; u < kBatch; u++) e[u] = next[u];
 #pragma unroll
-        for (int u = 0; u < kBatch; u++)
-            next[u] = k + kBatch + u < cnt ? entries[(size_t)(k + kBatch + u) * c.n + i] : 0u;
+        for (int u = 0; u < kBatch; u++) next[u] = entries[(size_t)min(k + kBatch + u, lastk) * c.n + i];
 #pragma unroll
         for (int u = 0; u < kBatch; u++) j[u] = lds.rowstart[(e[u] >> 8) * NB + t] + (int32_t)(e[u] & 255u);
 #pragma unroll
         for (int u = 0; u < kBatch; u++) {
-            const bool v = k + u < cnt;
-            pj[u] = v ? rec_at<PAD>(rec, sg, j[u]) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            x[u] = v ? extra(j[u]) : X();
+            pj[u] = rec_at<PAD>(rec, sg, j[u]);
+            x[u] = extra(j[u]);
         }
 #pragma unroll
         for (int u = 0; u < kBatch; u++)
@@ -739,7 +748,10 @@ __device__ __forceinline__ void dense_pass(const SolverConsts &c, const Dom &dom
                     const int32_t k = r * kDenseSlots + b * kDenseGroup + lg;
                     valid[b] = k < cn;
                     if (listed) {
-                        code[b] = valid[b] ? list_entries[(size_t)k * c.n + i] : 0u;
+                        // unconditional load (a listed group's particle is real;
+                        // the index stays inside its list), selected after
+                        const uint32_t raw = list_entries[(size_t)max(min(k, cn - 1), 0) * c.n + i];
+                        code[b] = valid[b] ? raw : 0u;
                         j[b] = lds.s0[g][code[b] >> 8] + (int32_t)(code[b] & 255u);
                     } else {
                         if (valid[b])
@@ -754,8 +766,11 @@ __device__ __forceinline__ void dense_pass(const SolverConsts &c, const Dom &dom
                 X x[kDenseBatch];
 #pragma unroll
                 for (int b = 0; b < kDenseBatch; b++) {
-                    pj[b] = valid[b] ? rec_at<PAD>(rec, sg, j[b]) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                    x[b] = valid[b] ? ps.extra(sg, j[b]) : X();
+                    // invalid slots load the particle itself (a valid index), so
+                    // that the loads are unconditional; only valid slots are used
+                    const int32_t jl = valid[b] ? j[b] : li;
+                    pj[b] = rec_at<PAD>(rec, sg, jl);
+                    x[b] = ps.extra(sg, jl);
                 }
                 uint64_t mask = 0;
                 bool in[kDenseBatch];
@@ -938,7 +953,7 @@ struct VorticityAPass {
         vi = vel_sorted[i];
     }
     __device__ __forceinline__ float4 extra(const Seg &sg, int32_t j) const {
-        return j < sg.n ? vel_sorted[sg.off + j] : vel_id[sg.off];
+        return *(j < sg.n ? vel_sorted + sg.off + j : vel_id + sg.off);  // one load of either
     }
     __device__ __forceinline__ bool eval(const float4 &pj, const float4 &vj, bool, float *P, float *M) const {
         const float px = pi.x - pj.x, py = pi.y - pj.y, pz = pi.z - pj.z;
@@ -1177,7 +1192,7 @@ __global__ __launch_bounds__(NB) void k_vorticity_a(SolverConsts c, Dom dom, con
     PairListOut out = {list_entries, list_count, list_cap, 0};
     // a zero record past the scene has id 0: imageLoad (velocitytexture, 0)
     scan_neighbours<NB, PAD>(c, sg, rec, i, pi, lds,
-                             [&](int32_t j) { return j < sg.n ? vel_sorted[sg.off + j] : vel_id[sg.off]; },
+                             [&](int32_t j) { return *(j < sg.n ? vel_sorted + sg.off + j : vel_id + sg.off); },
                              [&](int32_t, uint32_t code, const float4 &pj, const float4 &vj) {
         const float px = pi.x - pj.x, py = pi.y - pj.y, pz = pi.z - pj.z;
         const float sq = sq_len(px, py, pz);
