@@ -468,6 +468,8 @@ __device__ __forceinline__ void scan_neighbours(const SolverConsts &c, const Seg
                 pj[u] = rec_at<PAD>(rec, sg, j[u]);
                 x[u] = extra(j[u]);
             }
+            // every load of the batch is issued before any arithmetic on them
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int u = 0; u < kBatch; u++)
                 if (k + u < cnt) body(j[u], e[u], pj[u], x[u]);
