@@ -94,8 +94,7 @@ struct pbf_batch {
         const DomMany d = dom();
         const int32_t nn = (int32_t)n;
         launch_predict(c, d, shape, pos, vel, xs, keys, st);
-        radix_sort_perm(rs, keys, nn, key_bits, perm, st);
-        launch_gather_records(nn, perm, xs, rec0, st);
+        radix_sort_perm(rs, keys, nn, key_bits, nullptr, st, xs, rec0);  // last pass writes the records
         launch_begin_cells(start, cells, epoch, st, &dense);
         launch_findcells(c, d, shape, rec0, start, end64, epoch, st);
         launch_neighbourcells(c, d, shape, rec0, start, end64, nbr9, st, &dense);

@@ -481,8 +481,8 @@ int sph_enqueue(pbf_sph *s, hipStream_t st, bool events) {
     TIMED(KID_PREDICT, launch_predict(c, s->pos, s->vel, s->xs, s->keys, st));
     PHASE(1);
     // sort (src/SPH.cpp:263-268)
-    TIMED(KID_SORT, radix_sort_perm(s->rs, s->keys, n, s->key_bits, s->perm, st));
-    TIMED(KID_GATHER, launch_gather_records(n, s->perm, s->xs, s->rec0, st));
+    // the sort's last pass writes the sorted records (globalsort.glsl:64 as a gather)
+    TIMED(KID_SORT, radix_sort_perm(s->rs, s->keys, n, s->key_bits, nullptr, st, s->xs, s->rec0));
     PHASE(2);
     // neighbour cells (src/SPH.cpp:270-275; src/NeighbourCellFinder.cpp:113-148)
     CellTables &t = s->cells_t;

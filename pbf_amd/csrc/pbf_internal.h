@@ -196,9 +196,12 @@ struct RadixScratch {
 };
 bool radix_alloc(RadixScratch &s, int32_t n);
 void radix_free(RadixScratch &s);
-// keys is clobbered. The permutation is written to perm, which must not alias
-// the scratch buffers.
-void radix_sort_perm(RadixScratch &s, uint32_t *keys, int32_t n, int key_bits, uint32_t *perm, hipStream_t st);
+// keys is clobbered. The permutation is written to perm (unless null), which
+// must not alias the scratch buffers. With gather_dst, the last pass also
+// writes gather_dst[k] = gather_src[perm[k]] (the sorted records), in place of
+// a separate gather.
+void radix_sort_perm(RadixScratch &s, uint32_t *keys, int32_t n, int key_bits, uint32_t *perm, hipStream_t st,
+                     const float4 *gather_src = nullptr, float4 *gather_dst = nullptr);
 
 // ------------------------------------------------------------ errors and helpers
 void set_error(const std::string &msg);

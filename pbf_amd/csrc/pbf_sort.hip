@@ -104,7 +104,9 @@ __global__ __launch_bounds__(RS_THREADS) void k_downsweep(const uint32_t *__rest
                                                           int32_t tiles, const uint32_t *__restrict__ offsets,
                                                           const uint32_t *__restrict__ totals,
                                                           uint32_t *__restrict__ keys_out,
-                                                          uint32_t *__restrict__ vals_out) {
+                                                          uint32_t *__restrict__ vals_out,
+                                                          const float4 *__restrict__ gather_src,
+                                                          float4 *__restrict__ gather_dst) {
     __shared__ uint32_t cnt[RS_ITEMS * RS_WAVES][256];
     __shared__ uint32_t wsum[RS_WAVES];
     for (int s = 0; s < RS_ITEMS * RS_WAVES; s++) cnt[s][threadIdx.x] = 0;
@@ -146,7 +148,8 @@ __global__ __launch_bounds__(RS_THREADS) void k_downsweep(const uint32_t *__rest
         const uint32_t d = (key[k] >> shift) & 255u;
         const uint32_t dst = cnt[k * RS_WAVES + w][d] + rank[k];
         if (keys_out) keys_out[dst] = key[k];
-        vals_out[dst] = val[k];
+        if (vals_out) vals_out[dst] = val[k];
+        if (gather_dst) gather_dst[dst] = gather_src[val[k]];  // the record itself (last pass)
     }
 }
 
@@ -175,7 +178,8 @@ void radix_free(RadixScratch &s) {
     s = RadixScratch();
 }
 
-void radix_sort_perm(RadixScratch &s, uint32_t *keys, int32_t n, int key_bits, uint32_t *perm, hipStream_t st) {
+void radix_sort_perm(RadixScratch &s, uint32_t *keys, int32_t n, int key_bits, uint32_t *perm, hipStream_t st,
+                     const float4 *gather_src, float4 *gather_dst) {
     if (n <= 0) return;
     const int32_t tiles = (n + RS_TILE - 1) / RS_TILE;
     const int passes = key_bits <= 8 ? 1 : (key_bits + 7) / 8;
@@ -188,7 +192,8 @@ void radix_sort_perm(RadixScratch &s, uint32_t *keys, int32_t n, int key_bits, u
         hipLaunchKernelGGL(k_upsweep, dim3(tiles), dim3(RS_THREADS), 0, st, kin, n, 8 * p, tiles, s.counts);
         hipLaunchKernelGGL(k_scan_rows, dim3(256), dim3(RS_THREADS), 0, st, s.counts, tiles, s.totals);
         hipLaunchKernelGGL(k_downsweep, dim3(tiles), dim3(RS_THREADS), 0, st, kin, vin, n, 8 * p, tiles,
-                           s.counts, s.totals, last ? nullptr : kout, vout);
+                           s.counts, s.totals, last ? nullptr : kout, vout, last ? gather_src : nullptr,
+                           last ? gather_dst : nullptr);
         vin = vout;
         uint32_t *t = kin;
         kin = kout;
