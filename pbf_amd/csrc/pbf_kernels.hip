@@ -245,22 +245,37 @@ __device__ __forceinline__ int32_t neighbourcells_one(const SolverConsts &c, con
     const Seg sg = dom.at(i);
     const float4 p = sorted[i];
     const int32_t px = (int32_t)p.x, py = (int32_t)p.y, pz = (int32_t)p.z;  // ivec3 (pos): truncation
+    // The 27 cells' table words in two rounds of loads in flight together
+    // (a load under a per-lane condition is waited for before the next):
+    // every start word (index 0 outside the grid), then the end words of the
+    // cells with a start, the others reading one shared word.
+    uint32_t idx[27];
+    int32_t s[27];
+#pragma unroll
+    for (int q = 0; q < 27; q++) {
+        const int32_t x = px + (q % 3 - 1), y = py + (q / 9 - 1), z = pz + ((q / 3) % 3 - 1);
+        const bool inb = in_grid(sg, x, y, z);
+        idx[q] = inb ? (uint32_t)sg.cell_base + (uint32_t)x + (uint32_t)sg.gx * ((uint32_t)y + (uint32_t)sg.gy * (uint32_t)z)
+                     : 0xFFFFFFFFu;
+        s[q] = start[idx[q] == 0xFFFFFFFFu ? 0u : idx[q]];
+    }
+    int32_t e[27];
+#pragma unroll
+    for (int q = 0; q < 27; q++) {
+        const bool inb = idx[q] != 0xFFFFFFFFu;
+        s[q] = inb ? s[q] : 0;
+        const bool used = inb && s[q] != -1;
+        e[q] = (int32_t)(uint32_t)(end64[used ? idx[q] : 0u] & 0xFFFFFFFFull);
+        e[q] = used ? e[q] : 0;
+    }
 #pragma unroll
     for (int o = 0; o < 9; o++) {
-        const int32_t y = py + (o / 3 - 1), z = pz + (o % 3 - 1);
         int32_t cell = -1, entries = 0;
 #pragma unroll
-        for (int dx = -1; dx <= 1; dx++) {
-            const int32_t x = px + dx;
-            const bool inb = in_grid(sg, x, y, z);
-            const size_t idx =
-                inb ? (size_t)sg.cell_base + (size_t)x + (size_t)sg.gx * ((size_t)y + (size_t)sg.gy * (size_t)z) : 0;
-            const int32_t s = inb ? start[idx] : 0;
-            if (cell == -1) cell = s;
-            if (s != -1) {
-                const int32_t e = inb ? (int32_t)(uint32_t)(end64[idx] & 0xFFFFFFFFull) : 0;
-                entries += e - s;
-            }
+        for (int k = 0; k < 3; k++) {
+            const int q = o * 3 + k;  // (dy, dz) row o = (o / 3 - 1, o % 3 - 1), dx = k - 1
+            if (cell == -1) cell = s[q];
+            entries += s[q] != -1 ? e[q] - s[q] : 0;
         }
         const int32_t word = (int32_t)((uint32_t)cell + ((uint32_t)entries << 24));
         nbr9[(size_t)o * c.n + i] = word;
