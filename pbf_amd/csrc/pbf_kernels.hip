@@ -358,20 +358,23 @@ __global__ __launch_bounds__(kDenseSelectThreads) void k_dense_select(DenseList 
 // buffer access does.
 // PAD: the arrays hold at least kPad zero entries after the last particle
 // (one scene), so indices past n read the zero record without a test.
-// Without padding the load is of a clamped index and the zero selected after
-// it: a load under the condition would be a branch, which keeps the loads of a
-// batch from being in flight together.
+// Without padding the load is of a clamped index and the zero made after it
+// by masking the bits (+0.0): a load under the condition would be a branch,
+// which keeps the loads of a batch from being in flight together (and the
+// compiler turns a select of the loaded value back into such a branch).
+__device__ __forceinline__ float mask_bits(float v, int32_t m) { return __int_as_float(__float_as_int(v) & m); }
 template <bool PAD>
 __device__ __forceinline__ float4 rec_at(const float4 *__restrict__ rec, const Seg &sg, int32_t j) {
     if (PAD) return rec[j];
     const float4 r = rec[sg.off + min(j, sg.n - 1)];
-    return j < sg.n ? r : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const int32_t m = j < sg.n ? -1 : 0;
+    return make_float4(mask_bits(r.x, m), mask_bits(r.y, m), mask_bits(r.z, m), mask_bits(r.w, m));
 }
 template <bool PAD>
 __device__ __forceinline__ float val_at(const float *__restrict__ a, const Seg &sg, int32_t j) {
     if (PAD) return a[j];
     const float v = a[sg.off + min(j, sg.n - 1)];
-    return j < sg.n ? v : 0.0f;
+    return mask_bits(v, j < sg.n ? -1 : 0);
 }
 
 // Neighbour iteration in two phases. FOR_EACH_NEIGHBOUR (foreachneighbour.glsl:
