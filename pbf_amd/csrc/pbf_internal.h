@@ -107,7 +107,7 @@ struct PairList {
 constexpr int kDenseLevels = 8;
 constexpr int kDenseCopies = 256;
 #ifndef PBF_DENSE_BUDGET
-#define PBF_DENSE_BUDGET 16384
+#define PBF_DENSE_BUDGET 32768
 #endif
 constexpr uint32_t kDenseBudget = PBF_DENSE_BUDGET;
 struct DenseList {
