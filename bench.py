@@ -50,11 +50,11 @@ ALG_BYTES = {
 # last updatepos launch of each step also performs the closing update (pos and
 # vel by particle id), so updatepos is the mean over launches with and without.
 MEASURED_TRAFFIC_BLOCK_1M = {
-    "lambda": 150.5,
-    "updatepos": 198.8,
-    "vorticity_a": 224.3,
-    "vorticity_b": 219.7,
-    "neighbourcells": 63.5,
+    "lambda": 150.0,
+    "updatepos": 198.7,
+    "vorticity_a": 223.1,
+    "vorticity_b": 219.9,
+    "neighbourcells": 63.7,
     "predict": 52.0,
 }
 

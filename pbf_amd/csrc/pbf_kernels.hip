@@ -394,7 +394,7 @@ __device__ __forceinline__ float val_at(const float *__restrict__ a, const Seg &
 // update, vorticity B) scan only for particles whose pair list overflowed and
 // keep kReplayLcap, so more of their waves fit on a CU.
 #ifndef PBF_LIST_LDS
-#define PBF_LIST_LDS 48
+#define PBF_LIST_LDS 56
 #endif
 #ifndef PBF_REPLAY_LDS
 #define PBF_REPLAY_LDS 16
