@@ -569,14 +569,18 @@ __device__ __forceinline__ void scan_neighbours(const SolverConsts &c, const Seg
 // the replaying pass scans.
 constexpr uint16_t LIST_NONE = 0xFFFF;
 
+// The next entry's address is kept and advanced by n per entry (an add, not
+// a 64-bit multiply per entry).
 struct PairListOut {
     uint16_t *entries;
     uint16_t *count;
     int32_t cap;
     int32_t cnt;
-    __device__ __forceinline__ void add(int32_t i, int32_t n, uint32_t code) {
+    uint16_t *next;  // &entries[cnt * n + i]: entries + i to begin with
+    __device__ __forceinline__ void add(int32_t, int32_t n, uint32_t code) {
         if (entries) {
-            if (cnt < cap) entries[(size_t)cnt * n + i] = (uint16_t)code;
+            if (cnt < cap) *next = (uint16_t)code;
+            next += n;
             cnt++;
         }
     }
@@ -1081,7 +1085,7 @@ __global__ __launch_bounds__(NB) void k_lambda(SolverConsts c, Dom dom, const fl
     const Seg sg = dom.at(i);
     const float4 pi = rec[i];
     float rho = 0.0f, sk = 0.0f, gix = 0.0f, giy = 0.0f, giz = 0.0f;
-    PairListOut out = {list_entries, list_count, list_cap, 0};
+    PairListOut out = {list_entries, list_count, list_cap, 0, list_entries ? list_entries + i : nullptr};
     scan_neighbours<NB, PAD>(c, sg, rec, i, pi, lds, [](int32_t) { return 0; },
                              [&](int32_t, uint32_t code, const float4 &pj, int) {
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
@@ -1209,7 +1213,7 @@ __global__ __launch_bounds__(NB) void k_vorticity_a(SolverConsts c, Dom dom, con
     const float4 pi = rec[i];
     const float4 vi = vel_sorted[i];
     float sx = 0.0f, sy = 0.0f, sz = 0.0f, wx = 0.0f, wy = 0.0f, wz = 0.0f;
-    PairListOut out = {list_entries, list_count, list_cap, 0};
+    PairListOut out = {list_entries, list_count, list_cap, 0, list_entries ? list_entries + i : nullptr};
     // a zero record past the scene has id 0: imageLoad (velocitytexture, 0)
     scan_neighbours<NB, PAD>(c, sg, rec, i, pi, lds,
                              [&](int32_t j) { return *(j < sg.n ? vel_sorted + sg.off + j : vel_id + sg.off); },
