@@ -230,7 +230,7 @@ int pbf_batch_create(pbf_batch **out, uint32_t nscenes, const pbf_scene_desc *sc
     TRY(dalloc(&b->vort, n));
     TRY(dalloc(&b->keys, n));
     TRY(dalloc(&b->perm, n));
-    TRY(dalloc(&b->list_entries, (size_t)kListCap * n));
+    TRY(dalloc(&b->list_entries, (size_t)kListRows * n));
     TRY(dalloc(&b->list_count, n));
     TRY(dalloc(&b->start, b->cells));
     TRY(dalloc(&b->nbr9, (size_t)9 * n));

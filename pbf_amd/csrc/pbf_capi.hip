@@ -386,7 +386,7 @@ int pbf_sph_create(pbf_sph **out, uint32_t n, int32_t gx, int32_t gy, int32_t gz
     TRY(dalloc(&s->rec_a, (size_t)n + kPad));
     TRY(dalloc(&s->rec_b, (size_t)n + kPad));
     s->rec = s->rec0;
-    TRY(dalloc(&s->list_entries, (size_t)kListCap * n));
+    TRY(dalloc(&s->list_entries, (size_t)kListRows * n));
     TRY(dalloc(&s->list_count, n));
     TRY(dalloc(&s->dense.cand, n));
     TRY(dalloc(&s->dense.levels, (size_t)kDenseLevels * kDenseCopies));

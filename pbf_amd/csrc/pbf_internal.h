@@ -84,6 +84,9 @@ void launch_neighbourcells(const SolverConsts &c, const float4 *sorted, const in
 // Capacity of the per-particle neighbour list that the lambda pass records for
 // the position update (particles with more neighbours within h scan instead).
 constexpr int32_t kListCap = 128;
+// Rows allocated for the lists: a replay requests the rows of its next batch
+// (up to 16 past the list's end) before it knows which of them are entries.
+constexpr int32_t kListRows = kListCap + 16;
 
 // Zero entries kept after the n particles of the one-scene solver's sorted
 // records, lambda and |omega| arrays: a neighbour row starts below n and holds

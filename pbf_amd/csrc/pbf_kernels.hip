@@ -412,6 +412,7 @@ static_assert(LCAP % 8 == 0 && kReplayLcap % 8 == 0, "list column of whole batch
 constexpr int kBatch = PBF_BATCH;
 static_assert(LCAP >= 2 * kBatch && kReplayLcap >= 2 * kBatch, "list column must hold at least two batches");
 static_assert(kBatch % 2 == 0, "codes are written in pairs");
+static_assert(kListCap + kBatch <= kListRows, "a replay's next-batch rows stay inside the list");
 static_assert(126 + kBatch <= kPad, "batches past a row stay inside the zero padding");
 
 template <int NB, int LC = LCAP>
@@ -611,22 +612,26 @@ __device__ __forceinline__ void replay_neighbours(const SolverConsts &c, const S
     // neighbours, so each batch waits for one round trip rather than two.
     // Every load is unconditional, so that a batch's loads are in flight
     // together (a load under a per-lane condition is a branch, and the
-    // compiler waits for each before the next): entry indices past the list
-    // repeat its last entry (index 0 for an empty list, whose loop does not
-    // run), so every code and neighbour index is a valid one.
-    const uint32_t lastk = cnt > 0 ? cnt - 1u : 0u;
+    // compiler waits for each before the next): the rows of the next batch
+    // are requested whether they hold entries or not (the lists have
+    // kListRows >= kListCap + kBatch rows), and codes past the list are
+    // replaced by the batch's first, so every code and neighbour index is a
+    // valid one. The row address advances by n per row (no multiply).
+    const size_t stride = (size_t)c.n;
+    const uint16_t *__restrict__ rp = entries + i;  // row k of the list
     uint32_t next[kBatch];
 #pragma unroll
-    for (int u = 0; u < kBatch; u++) next[u] = entries[(size_t)min((uint32_t)u, lastk) * c.n + i];
+    for (int u = 0; u < kBatch; u++) next[u] = rp[u * stride];
     for (uint32_t k = 0; k < cnt; k += kBatch) {
         uint32_t e[kBatch];
         int32_t j[kBatch];
         float4 pj[kBatch];
         X x[kBatch];
 #pragma unroll
-        for (int u = 0; u < kBatch; u++) e[u] = next[u];
+        for (int u = 0; u < kBatch; u++) e[u] = k + u < cnt ? next[u] : next[0];
+        rp += kBatch * stride;
 #pragma unroll
-        for (int u = 0; u < kBatch; u++) next[u] = entries[(size_t)min(k + kBatch + u, lastk) * c.n + i];
+        for (int u = 0; u < kBatch; u++) next[u] = rp[u * stride];
 #pragma unroll
         for (int u = 0; u < kBatch; u++) j[u] = lds.rowstart[(e[u] >> 8) * NB + t] + (int32_t)(e[u] & 255u);
 #pragma unroll
