@@ -51,9 +51,9 @@ ALG_BYTES = {
 # vel by particle id), so updatepos is the mean over launches with and without.
 MEASURED_TRAFFIC_BLOCK_1M = {
     "lambda": 150.0,
-    "updatepos": 198.7,
-    "vorticity_a": 223.1,
-    "vorticity_b": 219.9,
+    "updatepos": 218.1,
+    "vorticity_a": 223.3,
+    "vorticity_b": 239.6,
     "neighbourcells": 63.7,
     "predict": 52.0,
 }
