@@ -147,7 +147,10 @@ int pbf_sph_timing(pbf_sph *s, pbf_timing *out);
  * counterpart; finer than OutputTiming's five phases). While enabled, every
  * kernel launch of pbf_sph_run is bracketed by two events on the handle's
  * stream; pbf_sph_kernel_times synchronizes, and returns and resets per-kernel
- * totals. Kernel ids: see pbf_kernel_name. */
+ * totals. Kernel ids: see pbf_kernel_name. The sort's time includes the
+ * sorted records (its last pass writes them; "gather" has no launches), and
+ * the last position update of a step includes the closing update ("update"
+ * has launches only in steps without solver iterations). */
 #define PBF_NUM_KERNEL_IDS 11
 typedef struct pbf_kernel_time {
     double total_ms;
