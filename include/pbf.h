@@ -11,7 +11,8 @@
  *   - "dev" pointers are HIP device pointers, "host" pointers are host memory;
  *   - work is enqueued on the handle's stream (pbf_*_set_stream; default: a
  *     stream the handle creates), in order, asynchronously to the host unless
- *     a function says it synchronizes;
+ *     a function says it synchronizes; pbf_*_set_stream waits for the work
+ *     already queued on the old stream, so the order holds across a switch;
  *   - one host thread per handle at a time; distinct handles are independent.
  *
  * Arithmetic: FP32 with the reference shaders' operation order and IEEE
@@ -69,7 +70,10 @@ typedef struct pbf_timing {
     double sort_ms;
     double neighbour_cells_ms;
     double solver_ms;
-    double vorticity_ms; /* update + XSPH/vorticity, as the reference's last query */
+    double vorticity_ms; /* XSPH/vorticity passes. The reference's last query also
+                          * holds update.glsl; here the last position update of a
+                          * step performs the update per particle, so its time is in
+                          * solver_ms (vorticity_ms holds it only with 0 iterations) */
 } pbf_timing;
 
 typedef struct pbf_sph pbf_sph;
@@ -100,6 +104,11 @@ int pbf_key_bits(int32_t gx, int32_t gy, int32_t gz);
  * square root over every float of [2^-96, 2^8), division over div_samples
  * pseudo-random operand pairs. out[0..2]: mismatch counts, out[3..5]: cases. */
 int pbf_selftest_math(uint64_t div_samples, uint64_t seed, uint64_t out[6]);
+/* Self-test of the neighbour passes' workgroup renumbering (no reference
+ * counterpart): launches nblocks workgroups and writes, for workgroup b, the
+ * block of particles the passes of the given width (64, 128, 256, 512) give it
+ * (out[2b]) and the XCD that ran it (out[2b+1]); out holds 2*nblocks words. */
+int pbf_selftest_xcd_map(int32_t width, uint32_t nblocks, uint32_t *out);
 
 /* ---------------------------------------------------------------- SPH */
 /* SPH::SPH (src/SPH.h:40, src/SPH.cpp:24-151). numparticles must be a positive
@@ -140,6 +149,31 @@ int pbf_sph_dense_info(pbf_sph *s, int32_t *threshold, int32_t *count);
 int pbf_sph_positions_device(pbf_sph *s, void **dev, size_t *stride);
 /* SPH::GetVelocityBuffer (src/SPH.h:181-183). */
 int pbf_sph_velocities_device(pbf_sph *s, void **dev, size_t *stride);
+/* Render feed of the positions (the vertex data PointSprite::SetPositionBuffer,
+ * src/PointSprite.cpp:64-73, and Selection::GetParticle, src/Selection.cpp:
+ * 55-85, read from the position buffer): the first `components` (3 or 4)
+ * floats of every particle's position, in id order, to dst with dst_stride
+ * bytes between particles (12 for packed xyz, 16 for the float4 layout), e.g.
+ * into a GL buffer mapped through HIP-GL interop. host != 0: dst is host
+ * memory and the call synchronizes; otherwise it is enqueued on the stream. */
+int pbf_sph_export_positions(pbf_sph *s, void *dst, size_t dst_stride, int32_t components, int host);
+/* SPH::GetHighlightBuffer (src/SPH.h:205-207): one uint32 word per particle id
+ * (stride 4), cleared to zero at creation (src/SPH.cpp:106-109). The caller
+ * sets bit 0 to select a particle; every step keeps only bit 0 of each word
+ * (clearhighlight.glsl:5-8) and sets bit 1 on every neighbour candidate of a
+ * selected particle (highlight.glsl:16-30), as SPH::Run does at
+ * src/SPH.cpp:288-296; the renderer reads the words (PointSprite::
+ * SetHighlightBuffer, src/Simulation.cpp:79). */
+int pbf_sph_highlight_device(pbf_sph *s, void **dev, size_t *stride);
+/* The highlight words read into (get) or written from (set) uint32[n]; set with
+ * src == NULL clears them all, as Simulation::ResetParticleBuffer does with
+ * glClearBufferData (src/Simulation.cpp:270-272). host: synchronizes. */
+int pbf_sph_get_highlight(pbf_sph *s, uint32_t *dst, int host);
+int pbf_sph_set_highlight(pbf_sph *s, const uint32_t *src, int host);
+/* The selection toggle of Simulation::OnMouseDown (src/Simulation.cpp:163-193):
+ * a word above zero becomes 0, otherwise 1; *now (if not null) receives the
+ * new word (synchronizes the handle's stream). */
+int pbf_sph_toggle_highlight(pbf_sph *s, uint32_t id, uint32_t *now);
 /* SPH::OutputTiming (src/SPH.cpp:218-240): phase times of the last step
  * (synchronizes the handle's stream). */
 int pbf_sph_timing(pbf_sph *s, pbf_timing *out);

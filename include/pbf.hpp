@@ -4,7 +4,8 @@
 // setters and Run methods of the reference's classes (src/SPH.h:40-241,
 // src/RadixSort.h:41-57, src/NeighbourCellFinder.h:42-58). GL object names
 // become HIP device pointers: GetPositionBuffer returns the float4 positions
-// in id order, RadixSort::GetBuffer the float4 record buffer, GetResult the
+// in id order, GetHighlightBuffer the uint32 highlight words by id,
+// RadixSort::GetBuffer the float4 record buffer, GetResult the
 // packed 3 x ivec4 per particle neighbour table. glm::ivec3 becomes pbf::ivec3
 // (any type with x, y, z converts through its constructor).
 //
@@ -59,6 +60,25 @@ public:
     // Device pointer to float4 positions in id order, stride 16 bytes.
     void *GetPositionBuffer() const { return device_pointer(pbf_sph_positions_device); }
     void *GetVelocityBuffer() const { return device_pointer(pbf_sph_velocities_device); }
+    // Render feed: positions in id order, `components` (3 or 4) floats per
+    // particle, dst_stride bytes apart, into device memory on the handle's
+    // stream (host = false) or host memory (host = true).
+    void ExportPositions(void *dst, size_t dst_stride, int components = 3, bool host = false) {
+        detail::check(pbf_sph_export_positions(h_, dst, dst_stride, components, host ? 1 : 0),
+                      "SPH::ExportPositions");
+    }
+    // SPH::GetHighlightBuffer (src/SPH.h:205-207): uint32 words by particle id;
+    // bit 0 selects a particle, each step marks its neighbour candidates with bit 1.
+    void *GetHighlightBuffer() const { return device_pointer(pbf_sph_highlight_device); }
+    // Simulation::OnMouseDown's selection toggle (src/Simulation.cpp:163-193);
+    // returns the particle's new word.
+    uint32_t ToggleHighlight(uint32_t id) {
+        uint32_t now = 0;
+        detail::check(pbf_sph_toggle_highlight(h_, id, &now), "SPH::ToggleHighlight");
+        return now;
+    }
+    // Clears every highlight word (src/Simulation.cpp:270-272).
+    void ClearHighlight() { detail::check(pbf_sph_set_highlight(h_, nullptr, 1), "SPH::ClearHighlight"); }
 
     float GetRestDensity() const { return 1.0f / params_.one_over_rho_0; }
     void SetRestDensity(const float &rho) { set(params_.one_over_rho_0, 1.0f / rho); }

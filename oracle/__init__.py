@@ -52,6 +52,8 @@ def lib():
             "pbfo_lambda": [cp, vp, vp, vp], "pbfo_updatepos": [cp, vp, vp, vp, vp, C.c_int],
             "pbfo_update": [cp, vp, vp, vp], "pbfo_vorticity": [cp, vp, vp, vp, vp, C.c_int],
             "pbfo_step": [cp, vp, vp, vp, vp, C.c_int, C.POINTER(Stages)],
+            "pbfo_step_hl": [cp, vp, vp, vp, vp, C.c_int, C.POINTER(Stages), vp],
+            "pbfo_highlight": [cp, vp, vp, vp],
             "pbfo_step_f64": [cp, vp, vp, vp, vp], "pbfo_step_fixed_f64": [cp, vp, vp, vp, vp],
         }
         for name, args in sig.items():
@@ -174,8 +176,18 @@ def vorticity(cfg, rec, nbr, vel, vort, order=ORDER_JACOBI):
     return vel, vort
 
 
-def step(cfg, pos, vel, end=None, vort=None, order=ORDER_JACOBI, stages=False):
-    """One step; returns (pos, vel, end, vort[, stage dict])."""
+def highlight(cfg, sorted_rec, nbr, flags):
+    """The two highlight passes over the flag words (by id); returns the new flags."""
+    sorted_rec = f32(sorted_rec)
+    nbr = np.ascontiguousarray(nbr, np.int32)
+    out = np.array(flags, dtype=np.uint32, copy=True)
+    lib().pbfo_highlight(C.byref(cfg), _p(sorted_rec), _p(nbr), _p(out))
+    return out
+
+
+def step(cfg, pos, vel, end=None, vort=None, order=ORDER_JACOBI, stages=False, highlight=None):
+    """One step; returns (pos, vel, end, vort[, stage dict]). highlight: flag words
+    (uint32, by id) that the step's highlight passes update in place."""
     n = cfg.n
     pos = np.array(pos, np.float32, copy=True)
     vel = np.array(vel, np.float32, copy=True)
@@ -190,8 +202,10 @@ def step(cfg, pos, vel, end=None, vort=None, order=ORDER_JACOBI, stages=False):
               "lambda": np.zeros((it, n), np.float32), "solved": np.zeros((it, n, 4), np.float32)}
         st = Stages(_p(sd["predict"]), _p(sd["sorted"]), _p(sd["start"]), _p(sd["nbr"]), _p(sd["lambda"]),
                     _p(sd["solved"]))
-    lib().pbfo_step(C.byref(cfg), _p(pos), _p(vel), _p(end), _p(vort), int(order),
-                    C.byref(st) if st is not None else None)
+    if highlight is not None:
+        assert highlight.dtype == np.uint32 and highlight.flags["C_CONTIGUOUS"] and highlight.shape == (n,)
+    lib().pbfo_step_hl(C.byref(cfg), _p(pos), _p(vel), _p(end), _p(vort), int(order),
+                       C.byref(st) if st is not None else None, _p(highlight))
     if stages:
         return pos, vel, end, vort, sd
     return pos, vel, end, vort

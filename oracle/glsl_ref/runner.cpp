@@ -4,8 +4,8 @@
 // files (read as text at run time from <refdir>/shaders, never copied) under
 // Mesa llvmpipe through the DRI swrast loader, with no display, and replays
 // the dispatch and binding sequence of the reference's step:
-//   SPH::Run                     src/SPH.cpp:246-334 (minus the two UI highlight
-//                                passes at :288-296)
+//   SPH::Run                     src/SPH.cpp:246-334, including the two UI
+//                                highlight passes at :288-296
 //   RadixSort::Run / SortBits    src/RadixSort.cpp:124-133, 154-200
 //   NeighbourCellFinder::FindNeighbourCells  src/NeighbourCellFinder.cpp:113-148
 // It restates the shader-source rule of src/ShaderProgram.cpp:63-93 and the
@@ -14,7 +14,9 @@
 //
 // Usage: glsl_runner <job.bin> <outdir> [refdir]
 // Job layout (little endian): see struct JobHeader below, then pos[N*4] f32,
-// vel[N*4] f32, optional end table [gx*gy*gz] i32, optional vorticity [N] f32.
+// vel[N*4] f32, optional end table [gx*gy*gz] i32, optional vorticity [N] f32,
+// optional highlight flags [N] u32 (zeros otherwise, as glClearBufferData
+// leaves them at src/SPH.cpp:106-109).
 // Outputs raw arrays as <outdir>/s<step>_<stage>.bin plus final_*.bin and
 // timing.txt (wall clock with glFinish at the five phase boundaries of
 // SPH::OutputTiming, src/SPH.cpp:218-240).
@@ -37,7 +39,7 @@
 namespace {
 
 struct JobHeader {
-    char magic[8];  // "PBFJOB1"
+    char magic[8];  // "PBFJOB2"
     int32_t n, gx, gy, gz;
     float h;
     float params[8];  // one_over_rho_0, epsilon, gravity, timestep, k, scale, xsph c, vorticity eps
@@ -47,6 +49,7 @@ struct JobHeader {
     int32_t has_end_init, has_vort_init;
     int32_t dump_passes;         // also dump every radix pass (bit-exactness of the 2-bit passes)
     int32_t warmup;              // steps excluded from the timing summary
+    int32_t has_highlight_init;
 };
 
 #define GLFN(T, name) T name = nullptr;
@@ -258,7 +261,7 @@ int main(int argc, char **argv) {
     FILE *jf = fopen(argv[1], "rb");
     if (!jf) die("cannot open job");
     JobHeader job;
-    if (fread(&job, sizeof job, 1, jf) != 1 || strncmp(job.magic, "PBFJOB1", 8) != 0) die("bad job header");
+    if (fread(&job, sizeof job, 1, jf) != 1 || strncmp(job.magic, "PBFJOB2", 8) != 0) die("bad job header");
     const int N = job.n;
     const int gx = job.gx, gy = job.gy, gz = job.gz;
     const size_t C = (size_t)gx * gy * gz;
@@ -266,6 +269,7 @@ int main(int argc, char **argv) {
     std::vector<float> pos(4 * (size_t)N), vel(4 * (size_t)N);
     std::vector<int32_t> end_init;
     std::vector<float> vort_init;
+    std::vector<uint32_t> highlight_init((size_t)N, 0u);
     if (fread(pos.data(), 4, pos.size(), jf) != pos.size()) die("short pos");
     if (fread(vel.data(), 4, vel.size(), jf) != vel.size()) die("short vel");
     if (job.has_end_init) {
@@ -276,6 +280,7 @@ int main(int argc, char **argv) {
         vort_init.resize(N);
         if (fread(vort_init.data(), 4, N, jf) != (size_t)N) die("short vorticity");
     }
+    if (job.has_highlight_init && fread(highlight_init.data(), 4, N, jf) != (size_t)N) die("short highlight");
     fclose(jf);
 
     create_context();
@@ -304,6 +309,8 @@ int main(int argc, char **argv) {
     GLuint p_updpos = build_program({fe, "sph/updatepos.glsl"}, sph_hdr.str());
     GLuint p_vort = build_program({fe, "sph/vorticity.glsl"}, sph_hdr.str());
     GLuint p_update = build_program({fe, "sph/update.glsl"}, sph_hdr.str());
+    GLuint p_highlight = build_program({fe, "sph/highlight.glsl"}, sph_hdr.str());
+    GLuint p_clearhighlight = build_program({fe, "sph/clearhighlight.glsl"}, sph_hdr.str());
     GLuint p_counting = build_program({"radixsort/counting.glsl"}, sort_hdr.str());
     GLuint p_blockscan = build_program({"radixsort/blockscan.glsl"}, sort_hdr.str());
     GLuint p_globalsort = build_program({"radixsort/globalsort.glsl"}, sort_hdr.str());
@@ -385,9 +392,15 @@ int main(int argc, char **argv) {
     check_gl("ncf setup");
 
     // -------- SPH objects (src/SPH.cpp:95-150)
-    GLuint sb[5];
-    glGenBuffers(5, sb);
-    GLuint b_lambda = sb[0], b_vort = sb[1], b_pos = sb[2], b_vel = sb[3], b_params = sb[4];
+    GLuint sb[6];
+    glGenBuffers(6, sb);
+    GLuint b_lambda = sb[0], b_vort = sb[1], b_pos = sb[2], b_vel = sb[3], b_params = sb[4], b_highlight = sb[5];
+    GLuint t_highlight;
+    glGenTextures(1, &t_highlight);
+    glBindBuffer(GL_SHADER_STORAGE_BUFFER, b_highlight);
+    glBufferData(GL_SHADER_STORAGE_BUFFER, 4 * (size_t)N, highlight_init.data(), GL_DYNAMIC_COPY);
+    glBindTexture(GL_TEXTURE_BUFFER, t_highlight);
+    glTexBuffer(GL_TEXTURE_BUFFER, GL_R32UI, b_highlight);
     glBindBuffer(GL_SHADER_STORAGE_BUFFER, b_lambda);
     glBufferData(GL_SHADER_STORAGE_BUFFER, 4 * (size_t)N, nullptr, GL_DYNAMIC_COPY);
     glBindTexture(GL_TEXTURE_BUFFER, t_lambda);
@@ -534,6 +547,15 @@ int main(int argc, char **argv) {
         glActiveTexture(GL_TEXTURE3);
         glBindTexture(GL_TEXTURE_BUFFER, t_lambda);
         glActiveTexture(GL_TEXTURE0);
+        // particle highlighting (src/SPH.cpp:288-296)
+        glBindImageTexture(0, t_highlight, 0, GL_FALSE, 0, GL_READ_WRITE, GL_R32UI);
+        glUseProgram(p_clearhighlight);
+        glDispatchCompute(groups, 1, 1);
+        glMemoryBarrier(GL_SHADER_IMAGE_ACCESS_BARRIER_BIT);
+        glUseProgram(p_highlight);
+        glDispatchCompute(groups, 1, 1);
+        glMemoryBarrier(GL_SHADER_IMAGE_ACCESS_BARRIER_BIT);
+        dumpbuf("highlight", b_highlight, 4 * (size_t)N);
         glBindImageTexture(0, t_lambda, 0, GL_FALSE, 0, GL_WRITE_ONLY, GL_R32F);
         for (int it = 0; it < job.iterations; it++) {
             glUseProgram(p_lambda);
@@ -590,6 +612,7 @@ int main(int argc, char **argv) {
     write_file(outdir + "/final_vort.bin", read_buffer(b_vort, 4 * (size_t)N).data(), 4 * (size_t)N);
     write_file(outdir + "/final_end.bin", read_tex3d(t_end, C).data(), 4 * C);
     write_file(outdir + "/final_sorted.bin", read_buffer(rs_buffer, 16 * (size_t)N).data(), 16 * (size_t)N);
+    write_file(outdir + "/final_highlight.bin", read_buffer(b_highlight, 4 * (size_t)N).data(), 4 * (size_t)N);
     check_gl("final readback");
     fprintf(stderr, "glsl_runner: %d steps done\n", job.nsteps);
     return 0;

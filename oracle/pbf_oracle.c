@@ -263,8 +263,40 @@ void pbfo_vorticity(const pbfo_cfg *c, const float *rec4, const int32_t *nbr12, 
     vorticity_f32(c, (const rec_f32 *)rec4, nbr12, vel4, vort, order);
 }
 
+/* clearhighlight.glsl:5-8, then highlight.glsl:16-30 (dispatched by SPH::Run
+ * at src/SPH.cpp:288-296, after the neighbour cells and before the solver):
+ * every flag keeps only bit 0 (imageAtomicAnd by index); then each sorted
+ * particle gid whose id has bit 0 set ORs 2 into the flags of the ids of every
+ * candidate j != gid of its nine packed rows. There is no distance test, and
+ * an index past n reads the zero record, whose id is 0. The passes only ever
+ * set bit 1, which the test of bit 0 does not read, so the result does not
+ * depend on the order in which the invocations run. */
+void pbfo_highlight(const pbfo_cfg *c, const float *sorted4, const int32_t *nbr12, uint32_t *hl) {
+    const int32_t n = c->n;
+    const rec_f32 *r = (const rec_f32 *)sorted4;
+    for (int32_t i = 0; i < n; i++) hl[i] &= 1u;
+    for (int32_t g = 0; g < n; g++) {
+        if ((hl[r[g].id] & 1u) != 1u) continue;
+        for (int o = 0; o < 3; o++)
+            for (int k = 0; k < 3; k++) {
+                const int32_t d = nbr12[(size_t)g * 12 + o * 4 + k];
+                const int32_t e = d >> 24, s0 = d & 0xFFFFFF;
+                for (int32_t j = s0; j < s0 + e; j++) {
+                    if (j == g) continue;
+                    const int32_t id = (j >= 0 && j < n) ? r[j].id : 0;
+                    hl[id] |= 2u;
+                }
+            }
+    }
+}
+
 void pbfo_step(const pbfo_cfg *c, float *pos4, float *vel4, int32_t *end, float *vort, int order,
                pbfo_stages *st) {
+    pbfo_step_hl(c, pos4, vel4, end, vort, order, st, NULL);
+}
+
+void pbfo_step_hl(const pbfo_cfg *c, float *pos4, float *vel4, int32_t *end, float *vort, int order,
+                  pbfo_stages *st, uint32_t *highlight) {
     const int32_t n = c->n, C = cells_of(c);
     float *rec = (float *)malloc(sizeof(float) * 4 * (size_t)n);
     float *sorted = (float *)malloc(sizeof(float) * 4 * (size_t)n);
@@ -281,6 +313,7 @@ void pbfo_step(const pbfo_cfg *c, float *pos4, float *vel4, int32_t *end, float 
     if (st && st->start) memcpy(st->start, start, sizeof(int32_t) * (size_t)C);
     pbfo_neighbourcells(c, sorted, start, end, nbr);
     if (st && st->nbr12) memcpy(st->nbr12, nbr, sizeof(int32_t) * 12 * (size_t)n);
+    if (highlight) pbfo_highlight(c, sorted, nbr, highlight);
     for (int it = 0; it < c->iterations; it++) {
         pbfo_lambda(c, sorted, nbr, lambda);
         if (st && st->lambda) memcpy(st->lambda + (size_t)it * n, lambda, sizeof(float) * (size_t)n);

@@ -54,9 +54,17 @@ void pbfo_update(const pbfo_cfg *c, const float *rec4, float *pos4, float *vel4)
 void pbfo_vorticity(const pbfo_cfg *c, const float *rec4, const int32_t *nbr12, float *vel4, float *vort,
                     int order);
 
-/* One whole step (SPH::Run, src/SPH.cpp:246-334, without the UI highlight
- * passes). pos4/vel4 are in id order; end (grid cells) and vort (n) are the
- * state the reference keeps between steps. Optional stage outputs may be NULL. */
+/* The two highlight passes of SPH::Run (src/SPH.cpp:288-296): clear every flag
+ * but bit 0, then mark (|= 2) the candidates of each particle with bit 0 set.
+ * sorted4 and nbr12 are the step's sorted records and packed table; hl holds
+ * one flag word per particle id. */
+void pbfo_highlight(const pbfo_cfg *c, const float *sorted4, const int32_t *nbr12, uint32_t *hl);
+
+/* One whole step (SPH::Run, src/SPH.cpp:246-334). pos4/vel4 are in id order;
+ * end (grid cells) and vort (n) are the state the reference keeps between
+ * steps. Optional stage outputs may be NULL. pbfo_step_hl also runs the
+ * highlight passes on the flag words in highlight (n, by id) unless it is
+ * NULL; pbfo_step leaves them out. */
 typedef struct pbfo_stages {
     float *predict;    /* n*4 */
     float *sorted;     /* n*4 */
@@ -68,6 +76,8 @@ typedef struct pbfo_stages {
 
 void pbfo_step(const pbfo_cfg *c, float *pos4, float *vel4, int32_t *end, float *vort, int order,
                pbfo_stages *stages);
+void pbfo_step_hl(const pbfo_cfg *c, float *pos4, float *vel4, int32_t *end, float *vort, int order,
+                  pbfo_stages *stages, uint32_t *highlight);
 
 /* FP64 replay of one step, race-free order, same neighbour-table semantics. */
 void pbfo_step_f64(const pbfo_cfg *c, double *pos4, double *vel4, int32_t *end, double *vort);

@@ -61,6 +61,7 @@ SIGNATURES = {
     "pbf_wpoly6": ([C.c_float, C.c_float], C.c_float),
     "pbf_key_bits": ([i32, i32, i32], C.c_int),
     "pbf_selftest_math": ([C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64)], C.c_int),
+    "pbf_selftest_xcd_map": ([i32, u32, vp], C.c_int),
     "pbf_sph_create": ([C.POINTER(vp), u32, i32, i32, i32], C.c_int),
     "pbf_sph_destroy": ([vp], C.c_int),
     "pbf_sph_set_params": ([vp, pp], C.c_int),
@@ -73,6 +74,11 @@ SIGNATURES = {
     "pbf_sph_dense_info": ([vp, vp, vp], C.c_int),
     "pbf_sph_positions_device": ([vp, C.POINTER(vp), C.POINTER(C.c_size_t)], C.c_int),
     "pbf_sph_velocities_device": ([vp, C.POINTER(vp), C.POINTER(C.c_size_t)], C.c_int),
+    "pbf_sph_export_positions": ([vp, vp, C.c_size_t, i32, C.c_int], C.c_int),
+    "pbf_sph_highlight_device": ([vp, C.POINTER(vp), C.POINTER(C.c_size_t)], C.c_int),
+    "pbf_sph_get_highlight": ([vp, vp, C.c_int], C.c_int),
+    "pbf_sph_set_highlight": ([vp, vp, C.c_int], C.c_int),
+    "pbf_sph_toggle_highlight": ([vp, u32, C.POINTER(u32)], C.c_int),
     "pbf_sph_timing": ([vp, C.POINTER(PbfTiming)], C.c_int),
     "pbf_sph_set_kernel_timing": ([vp, C.c_int], C.c_int),
     "pbf_sph_kernel_times": ([vp, C.POINTER(PbfKernelTime)], C.c_int),

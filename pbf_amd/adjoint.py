@@ -37,6 +37,10 @@ class Rollout:
         check(self._lib.pbf_adjoint_create(C.byref(h), self.n, *self.grid, C.byref(params), self.steps,
                                            int(checkpoint_every), int(self.fp64)))
         self._h = h
+        # Counts the calls that replace what the reverse pass uses (the kept
+        # structure and checkpoints of a forward, the coefficients); a replay
+        # keeps them. A backward belongs to the forward of one generation only.
+        self.generation = 0
 
     @classmethod
     def for_scene(cls, scene, steps=None, checkpoint_every=8, fp64=False):
@@ -61,6 +65,7 @@ class Rollout:
 
     def set_coefficients(self, xsph_c, vorticity_epsilon):
         check(self._lib.pbf_adjoint_set_coefficients(self._h, float(xsph_c), float(vorticity_epsilon)))
+        self.generation += 1
 
     def set_stream(self, stream_handle):
         check(self._lib.pbf_adjoint_set_stream(self._h, stream_handle))
@@ -68,6 +73,7 @@ class Rollout:
     # host (numpy) interface
     def forward(self, pos4, vel4):
         pos4, vel4 = self._arr(pos4), self._arr(vel4)
+        self.generation += 1
         check(self._lib.pbf_adjoint_forward(self._h, pos4.ctypes.data, vel4.ctypes.data, 1))
         out = np.empty((self.n, 4), self.dtype)
         check(self._lib.pbf_adjoint_positions(self._h, out.ctypes.data, 1))
@@ -99,6 +105,7 @@ class Rollout:
     def forward_device(self, pos4, vel4):
         import torch
         torch.cuda.current_stream().synchronize()
+        self.generation += 1
         check(self._lib.pbf_adjoint_forward(self._h, pos4.data_ptr(), vel4.data_ptr(), 0))
         out = torch.empty_like(pos4)
         check(self._lib.pbf_adjoint_positions(self._h, out.data_ptr(), 0))
@@ -133,6 +140,7 @@ def _autograd_function():
             pos4[:, :3] = pos0.to(dt)
             out = rollout.forward_device(pos4.contiguous(), vel4.contiguous())
             ctx.rollout = rollout
+            ctx.generation = rollout.generation
             ctx.dtypes = (v0.dtype, xsph_c.dtype, vorticity_epsilon.dtype)
             ctx.devices = (xsph_c.device, vorticity_epsilon.device)
             return out[:, :3].to(v0.dtype)
@@ -140,6 +148,10 @@ def _autograd_function():
         @staticmethod
         def backward(ctx, grad_pos):
             r = ctx.rollout
+            if r.generation != ctx.generation:
+                raise RuntimeError("the Rollout handle ran another forward or coefficient change "
+                                   "after the forward this gradient belongs to; the reverse pass would use "
+                                   "that later rollout. Call backward first, or use one Rollout per rollout")
             dt = torch.float64 if r.fp64 else torch.float32
             g4 = torch.zeros((r.n, 4), dtype=dt, device=grad_pos.device)
             g4[:, :3] = grad_pos.to(dt)
@@ -156,7 +168,11 @@ _fn = None
 
 
 def rollout_positions(v0, xsph_c, vorticity_epsilon, rollout: Rollout, pos0):
-    """Differentiable final positions (n, 3) of ``rollout`` from pos0 (n, 3) and v0 (n, 3)."""
+    """Differentiable final positions (n, 3) of ``rollout`` from pos0 (n, 3) and v0 (n, 3).
+
+    The handle keeps one forward's structure and checkpoints: the gradient of a
+    result is available until the handle's next forward or coefficient change,
+    after which its backward raises."""
     global _fn
     if _fn is None:
         _fn = _autograd_function()

@@ -859,7 +859,12 @@ struct Adj : AdjBase {
         c.vorticity = p.vorticity;
     }
     void set_coefficients(double cc, double eps) override { set_consts(cc, eps); }
-    void set_stream(hipStream_t s) override { st = s ? s : own; }
+    // work queued on the old stream finishes first, so calls stay in order
+    void set_stream(hipStream_t s) override {
+        const hipStream_t next = s ? s : own;
+        if (next != st) (void)hipStreamSynchronize(st);
+        st = next;
+    }
     hipStream_t stream() const override { return st; }
 
     template <class U>

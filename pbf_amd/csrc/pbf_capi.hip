@@ -214,6 +214,7 @@ struct pbf_sph {
     float4 *vel_sorted = nullptr, *vxsph = nullptr, *omega = nullptr;
     float *lambda = nullptr, *vort = nullptr;
     uint32_t *keys = nullptr, *perm = nullptr;
+    uint32_t *highlight = nullptr;               // highlight words by id (SPH::GetHighlightBuffer)
     CellTables cells_t;
     RadixScratch rs;
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -257,6 +258,7 @@ struct pbf_sph {
         dfree(vort);
         dfree(keys);
         dfree(perm);
+        dfree(highlight);
         cells_t.release();
         radix_free(rs);
         for (auto &e : ev)
@@ -399,6 +401,9 @@ int pbf_sph_create(pbf_sph **out, uint32_t n, int32_t gx, int32_t gy, int32_t gz
     TRY(dalloc(&s->vort, (size_t)n + kPad));
     TRY(dalloc(&s->keys, n));
     TRY(dalloc(&s->perm, n));
+    TRY(dalloc(&s->highlight, n));
+    // cleared like the reference's glClearBufferData (src/SPH.cpp:106-109)
+    TRY(hipMemset(s->highlight, 0, (size_t)n * sizeof(uint32_t)));
     TRY(s->cells_t.alloc(n, s->cells));
     if (!radix_alloc(s->rs, (int32_t)n)) TRY(hipErrorOutOfMemory);
     for (auto &ev : s->ev) TRY(hipEventCreate(&ev));
@@ -428,8 +433,10 @@ int pbf_sph_set_params(pbf_sph *s, const pbf_params *p) {
     if (!s) return invalid("null SPH handle");
     int rc = check_params(p);
     if (rc) return rc;
-    if (s->exec) PBF_HIP(hipStreamSynchronize(s->stream));
-    s->drop_graph();  // the captured launches carry the old constants
+    // the captured launches carry the old constants; a replay may still run
+    // on the stream it was captured on
+    if (s->exec) PBF_HIP(hipStreamSynchronize(s->graph_stream));
+    s->drop_graph();
     s->params = *p;
     s->consts = make_consts(s->n, s->grid, s->params, s->key_bits);
     return PBF_OK;
@@ -478,7 +485,8 @@ int sph_enqueue(pbf_sph *s, hipStream_t st, bool events) {
     } while (0)
     PHASE(0);
     // predict positions (src/SPH.cpp:247-261) + cell keys
-    TIMED(KID_PREDICT, launch_predict(c, s->pos, s->vel, s->xs, s->keys, st));
+    // with the clear of the highlight words (clearhighlight.glsl, src/SPH.cpp:290-293)
+    TIMED(KID_PREDICT, launch_predict(c, s->pos, s->vel, s->xs, s->keys, st, s->highlight));
     PHASE(1);
     // sort (src/SPH.cpp:263-268)
     // the sort's last pass writes the sorted records (globalsort.glsl:64 as a gather)
@@ -488,7 +496,10 @@ int sph_enqueue(pbf_sph *s, hipStream_t st, bool events) {
     CellTables &t = s->cells_t;
     TIMED(KID_CLEAR_START, launch_begin_cells(t.start, t.cells, t.epoch, st, &s->dense));
     TIMED(KID_FINDCELLS, launch_findcells(c, s->rec0, t.start, t.end64, t.epoch, st));
-    TIMED(KID_NEIGHBOURCELLS, launch_neighbourcells(c, s->rec0, t.start, t.end64, t.nbr9, st, &s->dense));
+    // with the highlight pass (highlight.glsl, src/SPH.cpp:294-296), which reads
+    // the same sorted records and rows
+    TIMED(KID_NEIGHBOURCELLS,
+          launch_neighbourcells(c, s->rec0, t.start, t.end64, t.nbr9, st, &s->dense, s->highlight));
     PHASE(3);
     // solver iterations (src/SPH.cpp:303-311), race-free: x* -> a -> b -> a ...
     PairList pl;
@@ -606,6 +617,55 @@ int pbf_sph_velocities_device(pbf_sph *s, void **dev, size_t *stride) {
     return PBF_OK;
 }
 
+int pbf_sph_export_positions(pbf_sph *s, void *dst, size_t dst_stride, int32_t components, int host) {
+    if (!s || !dst) return invalid("null argument");
+    if (components != 3 && components != 4) return invalid("components must be 3 or 4");
+    const size_t width = (size_t)components * sizeof(float);
+    if (dst_stride < width) return invalid("destination stride below the exported width");
+    PBF_HIP(hipMemcpy2DAsync(dst, dst_stride, s->pos, sizeof(float4), width, s->n, from_dev(host), s->stream));
+    if (host) PBF_HIP(hipStreamSynchronize(s->stream));
+    return PBF_OK;
+}
+
+int pbf_sph_highlight_device(pbf_sph *s, void **dev, size_t *stride) {
+    if (!s || !dev) return invalid("null argument");
+    *dev = s->highlight;
+    if (stride) *stride = sizeof(uint32_t);
+    return PBF_OK;
+}
+
+int pbf_sph_get_highlight(pbf_sph *s, uint32_t *dst, int host) {
+    if (!s || !dst) return invalid("null argument");
+    PBF_HIP(hipMemcpyAsync(dst, s->highlight, (size_t)s->n * sizeof(uint32_t), from_dev(host), s->stream));
+    if (host) PBF_HIP(hipStreamSynchronize(s->stream));
+    return PBF_OK;
+}
+
+int pbf_sph_set_highlight(pbf_sph *s, const uint32_t *src, int host) {
+    if (!s) return invalid("null SPH handle");
+    const size_t bytes = (size_t)s->n * sizeof(uint32_t);
+    if (src) {
+        PBF_HIP(hipMemcpyAsync(s->highlight, src, bytes, to_dev(host), s->stream));
+        if (host) PBF_HIP(hipStreamSynchronize(s->stream));
+    } else {
+        PBF_HIP(hipMemsetAsync(s->highlight, 0, bytes, s->stream));
+    }
+    return PBF_OK;
+}
+
+int pbf_sph_toggle_highlight(pbf_sph *s, uint32_t id, uint32_t *now) {
+    if (!s) return invalid("null SPH handle");
+    if (id >= s->n) return invalid("particle id out of range");
+    uint32_t w = 0;
+    PBF_HIP(hipMemcpyAsync(&w, s->highlight + id, sizeof w, hipMemcpyDeviceToHost, s->stream));
+    PBF_HIP(hipStreamSynchronize(s->stream));
+    w = w > 0 ? 0u : 1u;  // src/Simulation.cpp:181-185
+    PBF_HIP(hipMemcpyAsync(s->highlight + id, &w, sizeof w, hipMemcpyHostToDevice, s->stream));
+    PBF_HIP(hipStreamSynchronize(s->stream));
+    if (now) *now = w;
+    return PBF_OK;
+}
+
 int pbf_sph_timing(pbf_sph *s, pbf_timing *out) {
     if (!s || !out) return invalid("null argument");
     std::memset(out, 0, sizeof *out);
@@ -642,9 +702,17 @@ int pbf_sph_kernel_times(pbf_sph *s, pbf_kernel_time out[PBF_NUM_KERNEL_IDS]) {
 
 const char *pbf_kernel_name(int id) { return id >= 0 && id < KID_COUNT ? kKernelNames[id] : nullptr; }
 
+// The work queued on the old stream (eager launches, or a graph replay on the
+// stream it was captured on) finishes before the switch, so steps keep their
+// order across streams; the graph, captured on the old stream, is dropped.
 int pbf_sph_set_stream(pbf_sph *s, void *stream) {
     if (!s) return invalid("null SPH handle");
-    s->stream = stream ? (hipStream_t)stream : s->own_stream;
+    const hipStream_t next = stream ? (hipStream_t)stream : s->own_stream;
+    if (next == s->stream) return PBF_OK;
+    PBF_HIP(hipStreamSynchronize(s->stream));
+    if (s->graph_stream && s->graph_stream != s->stream) PBF_HIP(hipStreamSynchronize(s->graph_stream));
+    s->drop_graph();
+    s->stream = next;
     return PBF_OK;
 }
 
@@ -780,7 +848,9 @@ int pbf_radixsort_permutation(pbf_radixsort *r, void **dev) {
 
 int pbf_radixsort_set_stream(pbf_radixsort *r, void *stream) {
     if (!r) return invalid("null RadixSort handle");
-    r->stream = stream ? (hipStream_t)stream : r->own_stream;
+    const hipStream_t next = stream ? (hipStream_t)stream : r->own_stream;
+    if (next != r->stream) PBF_HIP(hipStreamSynchronize(r->stream));  // runs stay in order
+    r->stream = next;
     return PBF_OK;
 }
 
@@ -878,7 +948,9 @@ int pbf_ncf_set_end_table(pbf_ncf *f, const int32_t *end, int host) {
 
 int pbf_ncf_set_stream(pbf_ncf *f, void *stream) {
     if (!f) return invalid("null NeighbourCellFinder handle");
-    f->stream = stream ? (hipStream_t)stream : f->own_stream;
+    const hipStream_t next = stream ? (hipStream_t)stream : f->own_stream;
+    if (next != f->stream) PBF_HIP(hipStreamSynchronize(f->stream));  // calls stay in order
+    f->stream = next;
     return PBF_OK;
 }
 

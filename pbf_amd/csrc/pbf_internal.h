@@ -57,9 +57,32 @@ struct DomMany {
     __device__ __forceinline__ Seg at(int32_t i) const { return segs[scene_of[i]]; }
 };
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
+// Renumbering them so that every XCD runs chunks of kXcdChunkParticles
+// consecutive sorted particles keeps a chunk's neighbour rows in one L2, while
+// consecutive chunks go to different XCDs, so a dense region of the sorted
+// array (whose particles have many more neighbours than the rest) is shared by
+// all XCDs instead of landing on one. Bijective on [0, gridDim.x): full groups
+// of 8 chunks are permuted, the remainder keeps its numbers.
+constexpr uint32_t kXcdChunkParticles = 2048;
+
+// b: the workgroup's number among nb (counted from a multiple of 8).
+template <int NB>
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+    constexpr uint32_t chunk = kXcdChunkParticles / NB;
+    static_assert(chunk >= 1 && chunk * NB == kXcdChunkParticles, "chunk of whole workgroups");
+    const uint32_t group = 8u * chunk;
+    if (b >= nb / group * group) return b;
+    const uint32_t x = b % 8u, k = b / 8u;
+    return (k / chunk) * group + x * chunk + (k % chunk);
+}
+
 // ------------------------------------------------------------ launchers
+// highlight (or null): the SPH handle's highlight words by id, of which the
+// predict pass keeps bit 0 (clearhighlight.glsl) and the neighbour-cell pass
+// then marks the candidates of the flagged particles (highlight.glsl).
 void launch_predict(const SolverConsts &c, const float4 *pos, const float4 *vel, float4 *rec, uint32_t *keys,
-                    hipStream_t st);
+                    hipStream_t st, uint32_t *highlight = nullptr);
 void launch_gather_records(int32_t n, const uint32_t *perm, const float4 *rec_in, float4 *rec_out,
                            hipStream_t st);
 // out[g] = src[id of rec[g]]
@@ -76,7 +99,7 @@ void launch_findcells(const SolverConsts &c, const float4 *sorted, int32_t *star
 // dl: also list the step's dense particles and count the dense levels.
 void launch_neighbourcells(const SolverConsts &c, const float4 *sorted, const int32_t *start,
                            const unsigned long long *end64, int32_t *nbr9, hipStream_t st,
-                           const DenseList *dl = nullptr);
+                           const DenseList *dl = nullptr, uint32_t *highlight = nullptr);
 // Per-particle list of the neighbours within h at one set of positions:
 // written by the lambda pass (vorticity pass A), replayed by the position
 // update (vorticity pass B). entries [cap][n] of 16-bit (row, offset) codes.

@@ -69,26 +69,6 @@ __device__ __forceinline__ uint32_t cell_hash(const Seg &s, float x, float y, fl
 
 __device__ __forceinline__ int32_t id_of(const float4 &r) { return __float_as_int(r.w); }
 
-// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
-// Renumbering them so that every XCD runs chunks of kXcdChunkParticles
-// consecutive sorted particles keeps a chunk's neighbour rows in one L2, while
-// consecutive chunks go to different XCDs, so a dense region of the sorted
-// array (whose particles have many more neighbours than the rest) is shared by
-// all XCDs instead of landing on one. Bijective on [0, gridDim.x): full groups
-// of 8 chunks are permuted, the remainder keeps its numbers.
-constexpr uint32_t kXcdChunkParticles = 2048;
-
-// b: the workgroup's number among nb (counted from a multiple of 8).
-template <int NB>
-__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
-    constexpr uint32_t chunk = kXcdChunkParticles / NB;
-    static_assert(chunk >= 1 && chunk * NB == kXcdChunkParticles, "chunk of whole workgroups");
-    const uint32_t group = 8u * chunk;
-    if (b >= nb / group * group) return b;
-    const uint32_t x = b % 8u, k = b / 8u;
-    return (k / chunk) * group + x * chunk + (k % chunk);
-}
-
 // ----------------------------------------------------------------- predict
 // predictpos.glsl:18-38 fused with the key of counting.glsl:70. Records carry
 // the scene-local id; keys put the scene number above the cell key, so one
@@ -121,11 +101,18 @@ __device__ __forceinline__ void predict_one(const SolverConsts &c, const Dom &do
     for (int32_t k_ = 0, i = (int32_t)(blockIdx.x * blockDim.x) * ppt + (int32_t)threadIdx.x; \
          k_ < ppt && i < c.n; k_++, i += (int32_t)blockDim.x)
 
+// hl (one scene, or null): the highlight words by id, of which the step keeps
+// only bit 0 (clearhighlight.glsl:5-8, imageAtomicAnd (highlight[gid], 1) over
+// every index; no other pass of the step runs in between, so a plain
+// read-modify-write of one's own word is the same).
 template <class Dom>
 __global__ __launch_bounds__(1024) void k_predict(SolverConsts c, Dom dom, int32_t ppt, const float4 *__restrict__ pos,
                                                   const float4 *__restrict__ vel, float4 *__restrict__ rec,
-                                                  uint32_t *__restrict__ keys) {
-    PBF_STREAM_FOR(i) predict_one(c, dom, i, pos, vel, rec, keys);
+                                                  uint32_t *__restrict__ keys, uint32_t *__restrict__ hl) {
+    PBF_STREAM_FOR(i) {
+        predict_one(c, dom, i, pos, vel, rec, keys);
+        if (hl) hl[i] = hl[i] & 1u;
+    }
 }
 
 template <class Dom>
@@ -240,10 +227,10 @@ __device__ __forceinline__ int32_t neighbourcells_one(const SolverConsts &c, con
                                                       const float4 *__restrict__ sorted,
                                                       const int32_t *__restrict__ start,
                                                       const unsigned long long *__restrict__ end64,
-                                                      int32_t *__restrict__ nbr9) {
+                                                      int32_t *__restrict__ nbr9, const float4 &p,
+                                                      int32_t words[9]) {
     int32_t cand = 0;
     const Seg sg = dom.at(i);
-    const float4 p = sorted[i];
     const int32_t px = (int32_t)p.x, py = (int32_t)p.y, pz = (int32_t)p.z;  // ivec3 (pos): truncation
     // The 27 cells' table words in two rounds of loads in flight together
     // (a load under a per-lane condition is waited for before the next):
@@ -279,9 +266,29 @@ __device__ __forceinline__ int32_t neighbourcells_one(const SolverConsts &c, con
         }
         const int32_t word = (int32_t)((uint32_t)cell + ((uint32_t)entries << 24));
         nbr9[(size_t)o * c.n + i] = word;
+        words[o] = word;
         cand += max(word >> 24, 0);
     }
     return cand;
+}
+
+// highlight.glsl:16-30 for sorted slot i with record p and row words w: when
+// the particle's word has bit 0 set, every candidate of its nine rows but its
+// own slot gets bit 1 (no distance test); an index past the scene's n reads
+// the zero record, whose id is 0. The pass only sets bit 1 and only tests bit
+// 0, so it needs no ordering against itself; the clear ran in the predict pass.
+__device__ __forceinline__ void highlight_one(const Seg &sg, int32_t i, const float4 &p, const int32_t w[9],
+                                              const float4 *__restrict__ sorted, uint32_t *__restrict__ hl) {
+    if ((hl[sg.off + id_of(p)] & 1u) != 1u) return;
+    const int32_t li = i - sg.off;
+    for (int o = 0; o < 9; o++) {
+        const int32_t s0 = w[o] & 0xFFFFFF, e = w[o] >> 24;
+        for (int32_t j = s0; j < s0 + e; j++) {
+            if (j == li) continue;
+            const int32_t jid = j < sg.n ? id_of(sorted[sg.off + j]) : 0;
+            atomicOr(&hl[sg.off + jid], 2u);
+        }
+    }
 }
 
 // Candidate levels of the dense-particle selection.
@@ -291,17 +298,22 @@ __constant__ int32_t kDenseLevelValue[kDenseLevels] = {24, 32, 48, 64, 96, 128, 
 // each dense level: per workgroup in LDS, then one atomic per level into one
 // of kDenseCopies words (few workgroups per word: atomics on one word
 // serialize).
+// hl (one scene, or null): also runs the highlight pass (highlight_one).
 template <class Dom>
 __global__ __launch_bounds__(1024) void k_neighbourcells(SolverConsts c, Dom dom, int32_t ppt,
                                                          const float4 *__restrict__ sorted,
                                                          const int32_t *__restrict__ start,
                                                          const unsigned long long *__restrict__ end64,
-                                                         int32_t *__restrict__ nbr9, DenseList dl) {
+                                                         int32_t *__restrict__ nbr9, DenseList dl,
+                                                         uint32_t *__restrict__ hl) {
     __shared__ uint32_t above[kDenseLevels];
     if (threadIdx.x < kDenseLevels) above[threadIdx.x] = 0u;
     __syncthreads();
     PBF_STREAM_FOR(i) {
-        const int32_t cand = neighbourcells_one(c, dom, i, sorted, start, end64, nbr9);
+        const float4 p = sorted[i];
+        int32_t w[9];
+        const int32_t cand = neighbourcells_one(c, dom, i, sorted, start, end64, nbr9, p, w);
+        if (hl) highlight_one(dom.at(i), i, p, w, sorted, hl);
         if (dl.cand) {
             dl.cand[i] = (uint16_t)cand;  // at most 9 * 127
             const int lane = (int)(threadIdx.x & 63u);
@@ -1422,8 +1434,8 @@ void with_nb(int nb, bool pad, F &&f) {
 
 template <class Dom>
 void predict_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, const float4 *pos, const float4 *vel,
-                  float4 *rec, uint32_t *keys, hipStream_t st) {
-    if (c.n > 0) k_predict<Dom><<<stream_blocks(c.n, ls), ls.sb, 0, st>>>(c, d, ls.ppt, pos, vel, rec, keys);
+                  float4 *rec, uint32_t *keys, hipStream_t st, uint32_t *hl = nullptr) {
+    if (c.n > 0) k_predict<Dom><<<stream_blocks(c.n, ls), ls.sb, 0, st>>>(c, d, ls.ppt, pos, vel, rec, keys, hl);
 }
 template <class Dom>
 void findcells_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, const float4 *sorted, int32_t *start,
@@ -1434,10 +1446,10 @@ void findcells_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, 
 template <class Dom>
 void neighbourcells_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, const float4 *sorted,
                          const int32_t *start, const unsigned long long *end64, int32_t *nbr9, hipStream_t st,
-                         const DenseList *dl = nullptr) {
+                         const DenseList *dl = nullptr, uint32_t *hl = nullptr) {
     if (c.n > 0)
         k_neighbourcells<Dom><<<stream_blocks(c.n, ls), ls.sb, 0, st>>>(c, d, ls.ppt, sorted, start, end64, nbr9,
-                                                                         dl ? *dl : DenseList());
+                                                                         dl ? *dl : DenseList(), hl);
     if (dl && c.n > 0) launch_dense_select(*dl, st);
 }
 
@@ -1538,8 +1550,8 @@ void vorticity_b_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls
 
 // one scene
 void launch_predict(const SolverConsts &c, const float4 *pos, const float4 *vel, float4 *rec, uint32_t *keys,
-                    hipStream_t st) {
-    predict_impl(c, one(c), kDefaultShape, pos, vel, rec, keys, st);
+                    hipStream_t st, uint32_t *highlight) {
+    predict_impl(c, one(c), kDefaultShape, pos, vel, rec, keys, st, highlight);
 }
 void launch_keys_of_records(const SolverConsts &c, const float4 *rec, uint32_t *keys, hipStream_t st) {
     if (c.n > 0) k_keys<DomOne><<<blocks_for(c.n), BLOCK, 0, st>>>(c, one(c), rec, keys);
@@ -1564,8 +1576,9 @@ void launch_findcells(const SolverConsts &c, const float4 *sorted, int32_t *star
     findcells_impl(c, one(c), kDefaultShape, sorted, start, end64, epoch, st);
 }
 void launch_neighbourcells(const SolverConsts &c, const float4 *sorted, const int32_t *start,
-                           const unsigned long long *end64, int32_t *nbr9, hipStream_t st, const DenseList *dl) {
-    neighbourcells_impl(c, one(c), kDefaultShape, sorted, start, end64, nbr9, st, dl);
+                           const unsigned long long *end64, int32_t *nbr9, hipStream_t st, const DenseList *dl,
+                           uint32_t *highlight) {
+    neighbourcells_impl(c, one(c), kDefaultShape, sorted, start, end64, nbr9, st, dl, highlight);
 }
 void launch_lambda(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, float *lambda, const PairList &pl,
                    hipStream_t st, bool padded, const DenseList *dl) {

@@ -103,8 +103,48 @@ __global__ __launch_bounds__(ST_BLOCK) void k_test_div(uint64_t count, uint64_t 
     }
 }
 
+// The neighbour passes' workgroup renumbering as the kernels compute it:
+// out[2 b] = xcd_block<NB>(b, nb), out[2 b + 1] = the XCD (XCC_ID hardware
+// register, read-only) that ran workgroup b.
+template <int NB>
+__global__ __launch_bounds__(64) void k_xcd_map(uint32_t nb, uint32_t *out) {
+    const uint32_t b = blockIdx.x;
+    if (threadIdx.x != 0 || b >= nb) return;
+    uint32_t xcc;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID, 0, 4)" : "=s"(xcc));
+    out[2 * (size_t)b] = xcd_block<NB>(b, nb);
+    out[2 * (size_t)b + 1] = xcc;
+}
+
 }  // namespace
 }  // namespace pbf
+
+extern "C" int pbf_selftest_xcd_map(int32_t width, uint32_t nblocks, uint32_t *out) {
+    using namespace pbf;
+    if (!out || nblocks == 0) {
+        set_error("null output or no workgroups");
+        return PBF_ERR_INVALID;
+    }
+    uint32_t *d = nullptr;
+    PBF_HIP(hipMalloc(&d, 2 * (size_t)nblocks * sizeof(uint32_t)));
+    hipError_t e = hipSuccess;
+    switch (width) {
+        case 64: hipLaunchKernelGGL(k_xcd_map<64>, dim3(nblocks), dim3(64), 0, nullptr, nblocks, d); break;
+        case 128: hipLaunchKernelGGL(k_xcd_map<128>, dim3(nblocks), dim3(64), 0, nullptr, nblocks, d); break;
+        case 256: hipLaunchKernelGGL(k_xcd_map<256>, dim3(nblocks), dim3(64), 0, nullptr, nblocks, d); break;
+        case 512: hipLaunchKernelGGL(k_xcd_map<512>, dim3(nblocks), dim3(64), 0, nullptr, nblocks, d); break;
+        default:
+            (void)hipFree(d);
+            set_error("width must be 64, 128, 256 or 512");
+            return PBF_ERR_INVALID;
+    }
+    e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpy(out, d, 2 * (size_t)nblocks * sizeof(uint32_t), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_fail(e, "pbf_selftest_xcd_map");
+    return PBF_OK;
+}
 
 extern "C" int pbf_selftest_math(uint64_t div_samples, uint64_t seed, uint64_t out[6]) {
     using namespace pbf;

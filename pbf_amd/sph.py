@@ -163,6 +163,41 @@ class SPH:
         check(self._lib.pbf_sph_velocities_device(self._h, C.byref(ptr), None))
         return ptr.value
 
+    def export_positions(self, components: int = 3, stride: int | None = None) -> np.ndarray:
+        """Render feed (pbf_sph_export_positions): positions in id order, `components` floats
+        per particle, `stride` bytes apart (default packed), as a host array."""
+        stride = 4 * components if stride is None else int(stride)
+        assert stride % 4 == 0
+        buf = np.zeros((self.numparticles, stride // 4), np.float32)
+        check(self._lib.pbf_sph_export_positions(self._h, buf.ctypes.data, stride, int(components), 1))
+        return buf
+
+    def GetHighlightBuffer(self) -> int:
+        """SPH::GetHighlightBuffer (src/SPH.h:205-207): uint32 words by particle id."""
+        ptr = C.c_void_p()
+        check(self._lib.pbf_sph_highlight_device(self._h, C.byref(ptr), None))
+        return ptr.value
+
+    def highlight(self) -> np.ndarray:
+        out = np.empty(self.numparticles, np.uint32)
+        check(self._lib.pbf_sph_get_highlight(self._h, out.ctypes.data, 1))
+        return out
+
+    def set_highlight(self, words=None):
+        """Writes the highlight words (uint32 by id); None clears them (src/Simulation.cpp:270-272)."""
+        if words is None:
+            check(self._lib.pbf_sph_set_highlight(self._h, None, 1))
+            return
+        words = np.ascontiguousarray(words, np.uint32)
+        assert words.shape == (self.numparticles,)
+        check(self._lib.pbf_sph_set_highlight(self._h, words.ctypes.data, 1))
+
+    def toggle_highlight(self, particle_id: int) -> int:
+        """Selection toggle of Simulation::OnMouseDown (src/Simulation.cpp:163-193); the new word."""
+        now = C.c_uint32()
+        check(self._lib.pbf_sph_toggle_highlight(self._h, int(particle_id), C.byref(now)))
+        return now.value
+
     def upload(self, pos4: np.ndarray, vel4: np.ndarray):
         """State upload in id order (Simulation::ResetParticleBuffer's copies)."""
         pos4 = np.ascontiguousarray(pos4, dtype=np.float32)

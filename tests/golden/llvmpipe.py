@@ -23,14 +23,15 @@ def available() -> bool:
 
 
 def write_job(path, scene, nsteps, dump_from=0, dump_to=0, pos=None, vel=None, end_init=None,
-              vort_init=None, dump_passes=False, warmup=0):
+              vort_init=None, dump_passes=False, warmup=0, highlight_init=None):
     p = scene.params
     gx, gy, gz = scene.grid
     pos = scene.pos if pos is None else pos
     vel = scene.vel if vel is None else vel
-    hdr = struct.pack("<8s4if8f10i", b"PBFJOB1\0", pos.shape[0], gx, gy, gz, p.h, *[float(x) for x in p.ubo()],
+    hdr = struct.pack("<8s4if8f11i", b"PBFJOB2\0", pos.shape[0], gx, gy, gz, p.h, *[float(x) for x in p.ubo()],
                       p.iterations, int(p.vorticity), int(p.extforce), nsteps, dump_from, dump_to,
-                      int(end_init is not None), int(vort_init is not None), int(dump_passes), warmup)
+                      int(end_init is not None), int(vort_init is not None), int(dump_passes), warmup,
+                      int(highlight_init is not None))
     with open(path, "wb") as f:
         f.write(hdr)
         f.write(np.ascontiguousarray(pos, dtype=np.float32).tobytes())
@@ -39,6 +40,8 @@ def write_job(path, scene, nsteps, dump_from=0, dump_to=0, pos=None, vel=None, e
             f.write(np.ascontiguousarray(end_init, dtype=np.int32).tobytes())
         if vort_init is not None:
             f.write(np.ascontiguousarray(vort_init, dtype=np.float32).tobytes())
+        if highlight_init is not None:
+            f.write(np.ascontiguousarray(highlight_init, dtype=np.uint32).tobytes())
 
 
 class Run:
@@ -61,6 +64,9 @@ class Run:
 
     def floats(self, name):
         return self._load(name, np.float32, (self.n,))
+
+    def words(self, name):
+        return self._load(name, np.uint32, (self.n,))
 
     def table(self, name):
         return self._load(name, np.int32, (self.cells,))

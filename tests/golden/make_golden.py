@@ -6,7 +6,7 @@ deterministic).
 Run in the development container only (it needs the reference checkout and
 the runner built by `make -C oracle ref`):
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [--only highlight]
 
 Each fixture holds, for a few steps, the state before the step (id-order
 pos/vel, the persistent end table and vorticity buffer) and every stage output
@@ -157,9 +157,55 @@ def make(name, sc, nsteps, dumped, final=True):
     print(f"{name}: {os.path.getsize(path) / 1024:.0f} KiB")
 
 
+def highlight_flags(n, seed):
+    """Initial highlight words: bit 0 (the host's selection, src/Simulation.cpp:
+    172-190) on a few particles, some with other bits that the clear pass must
+    drop, and stray high bits on particles without bit 0."""
+    f = np.zeros(n, np.uint32)
+    pick = (S.uniform(seed, 48) * n).astype(np.int64)
+    f[pick[:24]] = 1
+    f[pick[24:32]] = 3
+    f[pick[32:40]] = 0xFFFFFFF2
+    f[pick[40:44]] = 0xFFFFFFFF
+    f[pick[44:48]] = 2
+    return f
+
+
+def make_highlight():
+    """Highlight passes (src/SPH.cpp:288-296, clearhighlight.glsl, highlight.glsl):
+    the flag words after each step of a 3-step dam break, and after one step of
+    the key-collision scene (keys >= 2^24, particles outside the grid)."""
+    out = {}
+    for name, sc, nsteps in (("dam", S.dam_break((16, 16, 16), (64, 32, 64), steps=3, name="hl_dam"), 3),
+                             ("collide", collide_scene(), 1)):
+        work = tempfile.mkdtemp(prefix=f"golden_hl_{name}_")
+        hl0 = highlight_flags(sc.n, 41 if name == "dam" else 42)
+        run = llvmpipe.run(sc, nsteps, 0, nsteps, threads=1, workdir=work, highlight_init=hl0)
+        p = sc.params
+        out[f"{name}_grid"] = np.array(sc.grid, np.int32)
+        out[f"{name}_h"] = np.float32(p.h)
+        out[f"{name}_ubo"] = p.ubo()
+        out[f"{name}_iterations"] = np.int32(p.iterations)
+        out[f"{name}_vorticity"] = np.int32(p.vorticity)
+        out[f"{name}_pos0"] = sc.pos
+        out[f"{name}_vel0"] = sc.vel
+        out[f"{name}_hl0"] = hl0
+        out[f"{name}_hl"] = np.stack([run.words(f"s{k}_highlight") for k in range(nsteps)])
+        shutil.rmtree(work)
+    path = os.path.join(HERE, "highlight.npz")
+    np.savez_compressed(path, **out)
+    print(f"highlight: {os.path.getsize(path) / 1024:.0f} KiB")
+
+
 def main():
     if not llvmpipe.available():
         raise SystemExit("llvmpipe runner not built or reference shaders missing (make -C oracle ref)")
+    if "--only" in sys.argv:
+        only = sys.argv[sys.argv.index("--only") + 1]
+        if only != "highlight":
+            raise SystemExit("--only takes: highlight")
+        make_highlight()
+        return
     # Dam break, 4096 particles, steps 0-1 and 38-39 of a 40-step rollout.
     make("dam4k", S.dam_break((16, 16, 16), (64, 32, 64), steps=40, name="dam4k"), 40, [0, 1, 39])
     make("pairs", pairs_scene(), 1, [0], final=False)
@@ -169,6 +215,7 @@ def main():
     ext.pos[:, 2] += 8.0  # half of the block above GRID_SIZE.z / 2
     make("extforce", ext, 3, [0])
     make("reference_default", reference_default_scene(), 3, [2])
+    make_highlight()
 
 
 if __name__ == "__main__":

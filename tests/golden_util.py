@@ -53,3 +53,18 @@ def solved_records(g, s, it):
     r = sorted_records(g, s).copy()
     r[:, :3] = g[f"s{s}_solved"][it]
     return r
+
+
+def highlight_case(g, name):
+    """(scene params, grid, pos0, vel0, flags before the first step, flags after each step) of one
+    case of highlight.npz."""
+    u = g[f"{name}_ubo"]
+    p = S.config_params()
+    p.one_over_rho_0, p.epsilon, p.gravity, p.timestep = (float(x) for x in u[:4])
+    p.tensile_instability_k, p.tensile_instability_scale, p.xsph_viscosity_c, p.vorticity_epsilon = (
+        float(x) for x in u[4:])
+    p.iterations = int(g[f"{name}_iterations"])
+    p.vorticity = bool(g[f"{name}_vorticity"])
+    p.h = float(g[f"{name}_h"])
+    grid = tuple(int(x) for x in g[f"{name}_grid"])
+    return p, grid, g[f"{name}_pos0"], g[f"{name}_vel0"], g[f"{name}_hl0"], g[f"{name}_hl"]

@@ -189,3 +189,29 @@ def test_torch_autograd_function(gpu_lib):
     assert np.abs(v0.grad.cpu().numpy() - gv[:, :3]).max() <= 1e-4 * scale
     assert abs(c.grad.item() - gc) <= 1e-4 * abs(gc) and abs(e.grad.item() - ge) <= 1e-4 * max(abs(ge), 1e-20)
     r.close()
+
+
+def test_torch_backward_after_a_second_forward_raises(gpu_lib):
+    """A Rollout keeps one forward's structure and checkpoints: the gradient of a result whose
+    handle has since run another forward (or changed its coefficients) is refused, not silently
+    taken from the later rollout."""
+    import torch
+    sc = adjoint_scene()
+    r = A.Rollout.for_scene(sc, steps=2, checkpoint_every=1)
+    dev = torch.device("cuda:0")
+    pos0 = torch.tensor(sc.pos[:, :3], device=dev)
+    v0 = torch.tensor(sc.vel[:, :3], device=dev, requires_grad=True)
+    c = torch.tensor(sc.params.xsph_viscosity_c, dtype=torch.float64, requires_grad=True)
+    e = torch.tensor(sc.params.vorticity_epsilon, dtype=torch.float64, requires_grad=True)
+    x1 = A.rollout_positions(v0, c, e, r, pos0)
+    x2 = A.rollout_positions(v0 * 0.5, c, e, r, pos0)
+    with pytest.raises(RuntimeError, match="another forward"):
+        x1.sum().backward()
+    x2.sum().backward()  # the latest forward's gradient is still available
+    assert v0.grad is not None and torch.isfinite(v0.grad).all()
+    v0.grad = None
+    x3 = A.rollout_positions(v0, c, e, r, pos0)
+    r.set_coefficients(0.02, sc.params.vorticity_epsilon)
+    with pytest.raises(RuntimeError, match="another forward"):
+        x3.sum().backward()
+    r.close()

@@ -8,6 +8,7 @@ state where it found it and its choice changes no bit of the results."""
 import numpy as np
 import pytest
 
+import oracle as O
 from pbf_amd import batch as B
 from pbf_amd import scene as S
 from pbf_amd import sph as P
@@ -59,6 +60,43 @@ def test_batched_equals_plain_per_scene(gpu_lib):
         assert np.array_equal(ends[cell_off[k]:cell_off[k + 1]], we), f"scene {k} end table"
         moved = np.abs(wp[:, :3] - sc.pos[:, :3]).max()
         assert moved > 1e-3
+
+
+def test_full_256_scene_mix_against_oracle(gpu_lib):
+    """The whole 256-scene workload (4,277,760 particles, n from 2,048 to 65,536; composite keys of
+    8 scene bits above up to 18 cell-key bits) for 3 graph-replayed steps at the tuned launch shape:
+    every scene's positions, velocities and end table bit-exact against the oracle's race-free step
+    with that scene's gravity direction, walls and XSPH coefficient (the per-scene forms of
+    predictpos.glsl:32, updatepos.glsl:98-100 and vorticity.glsl:57; the step src/SPH.cpp:246-334)."""
+    scenes = S.batched_scenes()
+    steps = 3
+    total = sum(s.n for s in scenes)
+    say(f"[full mix] scenes={len(scenes)} particles={total} max n={max(s.n for s in scenes)} steps={steps}")
+    assert len(scenes) == 256 and total == 4277760
+    assert len({s.params.gravity_dir for s in scenes}) == 256
+    assert len({s.params.xsph_viscosity_c for s in scenes}) > 200
+    b = B.Batch(scenes)
+    b.set_tuning(256, 128, 3)
+    b.set_graph(True)
+    b.upload()
+    b.step(steps)
+    pos, vel = b.download()
+    ends = b.end_tables()
+    b.close()
+    cell_off = np.concatenate([[0], np.cumsum([int(np.prod(s.grid)) for s in scenes])])
+    moved = 0
+    for k, sc in enumerate(scenes):
+        cfg = O.make_cfg(sc.n, sc.grid, sc.params)
+        p, v, e, vo = sc.pos, sc.vel, None, None
+        for _ in range(steps):
+            p, v, e, vo = O.step(cfg, p, v, e, vo, O.ORDER_JACOBI)
+        o = slice(b.offsets[k], b.offsets[k + 1])
+        assert same(pos[o], p), f"scene {k} (n={sc.n}) positions"
+        assert same(vel[o], v), f"scene {k} (n={sc.n}) velocities"
+        assert np.array_equal(ends[cell_off[k]:cell_off[k + 1]], e), f"scene {k} end table"
+        moved += int(np.abs(p[:, :3] - sc.pos[:, :3]).max() > 1e-3)
+    say(f"[full mix] all {len(scenes)} scenes bit-exact; {moved} moved by more than 1e-3 h")
+    assert moved == len(scenes)
 
 
 def test_graph_replays_equal_eager_steps(gpu_lib):

@@ -65,3 +65,32 @@ def test_graph_replays_equal_eager_steps(gpu_lib):
     assert same(pe, pg) and same(ve, vg)
     eager.close()
     graph.close()
+
+
+@pytest.mark.parametrize("graph_on", [False, True])
+def test_stream_switch_keeps_step_order(gpu_lib, graph_on):
+    """Steps queued on one stream, then more on another (torch's streams, then the handle's own
+    again), equal a rollout on one stream bit for bit: set_stream waits for the old stream's work
+    (and a graph replay captured on it) before the new stream's steps start."""
+    import torch
+    sc = S.cfg_dam_32k(seed=6)
+    say(f"[stream switch graph={graph_on}] n={sc.n}")
+    ref, sw = handle(sc), handle(sc)
+    ref.set_graph(graph_on)
+    sw.set_graph(graph_on)
+    ref.step(9)
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    sw.step(3)                         # own stream, not waited for
+    sw.set_stream(s1.cuda_stream)
+    sw.step(3)
+    sw.set_stream(s2.cuda_stream)
+    sw.step(2)
+    sw.set_stream(None)                # back to the handle's own stream
+    sw.step(1)
+    assert sw.stream() != s2.cuda_stream
+    pr, vr = ref.download()
+    ps, vs = sw.download()
+    assert same(pr, ps) and same(vr, vs)
+    assert np.array_equal(ref.end_table(), sw.end_table())
+    ref.close()
+    sw.close()

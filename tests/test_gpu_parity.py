@@ -194,10 +194,10 @@ def test_error_paths(gpu_lib):
         P.SPH(512, (0, 64, 64))
 
 
-def test_violent_rollout_exercises_list_fallback(gpu_lib):
-    """Large random velocities move particles by more than the superset list's
-    skin within a step, so the displacement guard sends them to the full
-    candidate scan; the rollout must still equal the oracle bit for bit."""
+def test_violent_rollout(gpu_lib):
+    """Large random velocities (30 h/s per axis) scatter a dam-break block within a few steps:
+    cells empty and refill, rows change length every step and particles leave through the walls'
+    clamp; the rollout must still equal the oracle bit for bit."""
     sc = S.dam_break((16, 16, 16), (64, 32, 64), steps=12, name="violent")
     n = sc.n
     sc.vel = np.zeros_like(sc.pos)
@@ -207,3 +207,112 @@ def test_violent_rollout_exercises_list_fallback(gpu_lib):
     wpos, wvel, _ = oracle_rollout(sc, 12)
     assert_bitexact(pos, wpos, "violent positions")
     assert_bitexact(vel, wvel, "violent velocities")
+
+
+@pytest.mark.parametrize("index", [0, 97, 200])
+def test_rotated_gravity_one_cell_walls_rollout(gpu_lib, index):
+    """The plain handle on scenes of the batched workload: gravity turned off the y axis, walls one
+    cell in from the x and z faces (so the wall clamp and the neighbour rows reach the grid's
+    edge cells), a per-scene XSPH coefficient; 12 steps, each bit-exact against the oracle."""
+    sc = S.batched_scenes()[index]
+    steps = 12
+    say(f"[rotated gravity] scene {index} n={sc.n} grid={sc.grid} g={sc.params.gravity_dir} "
+        f"c={sc.params.xsph_viscosity_c:.4f}")
+    assert sc.params.gravity_dir[0] != 0.0 and sc.params.wall_lo == (1.0, 0.0, 1.0)
+    sph = P.SPH(sc.n, sc.grid)
+    sph.set_params(P.scene_params(sc))
+    sph.upload(sc.pos, sc.vel)
+    cfg = O.make_cfg(sc.n, sc.grid, sc.params)
+    pos, vel, end, vort = sc.pos, sc.vel, None, None
+    for k in range(steps):
+        sph.Run()
+        pos, vel, end, vort = O.step(cfg, pos, vel, end, vort, O.ORDER_JACOBI)
+        gp, gv = sph.download()
+        assert_bitexact(gp, pos, f"scene {index} step {k} positions")
+        assert_bitexact(gv, vel, f"scene {index} step {k} velocities")
+    assert_bitexact(sph.end_table(), end, f"scene {index} end table")
+    # the walls were reached: some particle sits on a one-cell wall
+    gp, _ = sph.download()
+    assert ((gp[:, 0] == 1.0) | (gp[:, 2] == 1.0) | (gp[:, 0] == sc.grid[0] - 1.0) |
+            (gp[:, 2] == sc.grid[2] - 1.0)).any()
+    sph.close()
+
+
+def test_position_export_render_feed(gpu_lib):
+    """SPH::GetPositionBuffer / GetVelocityBuffer (src/SPH.h:49-51, 181-183) read through the
+    device pointer: float4 in id order, stride 16 bytes, w = 0, equal to DownloadState and to the
+    oracle's id-order state; the render feed (pbf_sph_export_positions) writes the same xyz packed at
+    12 bytes and as float4 at 16 bytes and at a padded 32-byte stride."""
+    sc = S.dam_break((16, 16, 16), (64, 32, 64), steps=3, name="export")
+    sph = P.SPH(sc.n, sc.grid)
+    sph.set_params(P.scene_params(sc))
+    sph.upload(sc.pos, sc.vel)
+    cfg = O.cfg_for(sc)
+    pos, vel, end, vort = sc.pos, sc.vel, None, None
+    for _ in range(3):
+        sph.Run()
+        pos, vel, end, vort = O.step(cfg, pos, vel, end, vort, O.ORDER_JACOBI)
+    import ctypes as C
+    for fn, want in ((sph._lib.pbf_sph_positions_device, pos), (sph._lib.pbf_sph_velocities_device, vel)):
+        ptr, stride = C.c_void_p(), C.c_size_t()
+        assert fn(sph._h, C.byref(ptr), C.byref(stride)) == 0 and stride.value == 16
+        raw = U.copy_from_device(ptr.value, (sc.n, 4), np.float32, sph.stream())
+        assert_bitexact(raw, want, "device buffer against the oracle")
+        assert (raw[:, 3] == 0.0).all()
+    assert sph.GetPositionBuffer() == C.c_void_p(sph.GetPositionBuffer()).value
+    dpos, dvel = sph.download()
+    assert_bitexact(dpos, pos, "DownloadState positions")
+    assert_bitexact(dvel, vel, "DownloadState velocities")
+    assert_bitexact(sph.export_positions(3), np.ascontiguousarray(pos[:, :3]), "packed xyz feed")
+    assert_bitexact(sph.export_positions(4), pos, "float4 feed")
+    wide = sph.export_positions(3, 32)
+    assert_bitexact(np.ascontiguousarray(wide[:, :3]), np.ascontiguousarray(pos[:, :3]), "32-byte stride feed")
+    assert (wide[:, 3:] == 0.0).all()
+    with pytest.raises(Exception):
+        sph.export_positions(3, 8)
+    sph.close()
+
+
+def test_pair_list_overflow_on_the_one_thread_path(gpu_lib):
+    """Two compact clumps of 176 particles, each inside a ball of radius 0.45h around a cell corner
+    (22 per cell, 44 per row, so no row count wraps), beside a 48x32x48 lattice whose 216-candidate
+    interior puts the step's dense threshold at 256 candidates. A clump particle then has 176
+    candidates (one-thread path) of which 175 lie within h: more than the 128 entries its pair list
+    holds, so the position update rescans its candidates. Bit-exact against the oracle."""
+    bg = S.lattice_block((48, 32, 48), 0.5, (16.25, 0.25, 16.25), 0.01, 91)[:73728 - 352]
+    u = S.uniform(92, 2 * 176 * 3).reshape(2 * 176, 3)
+    d = S.normal(93, 2 * 176 * 3).reshape(2 * 176, 3)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = 0.45 * np.cbrt(u[:, 0:1])
+    centre = np.repeat(np.array([[70.0, 10.0, 70.0], [70.0, 10.0, 60.0]]), 176, axis=0)
+    clump = np.zeros((352, 4), np.float32)
+    clump[:, :3] = (centre + r * d).astype(np.float32)
+    pos = np.concatenate([bg, clump])
+    sc = S.Scene("clumps", (96, 48, 96), S.config_params(1.0, 0.008, 4), pos, np.zeros_like(pos), 3)
+    assert sc.n % 512 == 0
+    say(f"[list overflow] n={sc.n}")
+    cfg = O.cfg_for(sc)
+    # the first step's structure: clump particles have <= 256 candidates, > 128 within h
+    srt, _ = O.sort(cfg, O.predict(cfg, sc.pos, sc.vel))
+    start, end = O.findcells(cfg, srt, np.zeros(96 * 48 * 96, np.int32))
+    nbr = O.neighbourcells(cfg, srt, start, end)[:, :, :3].reshape(sc.n, 9)
+    cand = np.maximum(nbr >> 24, 0).sum(1)
+    assert (nbr >> 24).max() < 128, "no wrapped row counts"
+    ids = srt[:, 3].view(np.int32)
+    cl = np.nonzero(ids >= 73728 - 352)[0]
+    inr = np.array([(np.linalg.norm(srt[cl, :3] - srt[g, :3], axis=1) <= 1.0).sum() - 1 for g in cl])
+    assert (cand[cl] <= 256).all() and (inr > 128).sum() >= 300
+    sph = P.SPH(sc.n, sc.grid)
+    sph.set_params(P.scene_params(sc))
+    sph.upload(sc.pos, sc.vel)
+    pos_o, vel_o, end_o, vort_o = sc.pos, sc.vel, None, None
+    for k in range(3):
+        sph.Run()
+        th, count = sph.dense_info()
+        if k == 0:
+            assert th == 256, f"dense threshold {th}"
+        pos_o, vel_o, end_o, vort_o = O.step(cfg, pos_o, vel_o, end_o, vort_o, O.ORDER_JACOBI)
+        gp, gv = sph.download()
+        assert_bitexact(gp, pos_o, f"clumps step {k} positions")
+        assert_bitexact(gv, vel_o, f"clumps step {k} velocities")
+    sph.close()

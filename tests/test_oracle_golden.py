@@ -87,3 +87,42 @@ def test_collide_fixture_has_rounded_keys():
     exact = ix[:, 0] + ix[:, 1] * gx * gz + ix[:, 2] * gx
     assert (exact >= 1 << 24).sum() > 1000
     assert (k.astype(np.int64) != exact).sum() > 100
+
+
+@pytest.mark.parametrize("case", ["dam", "collide"])
+def test_highlight_reference_order_bitexact(case):
+    """The highlight passes (clearhighlight.glsl, highlight.glsl; src/SPH.cpp:288-296), run inside
+    whole steps in the reference's order, reproduce the flag words llvmpipe wrote after each step."""
+    g = G.load("highlight")
+    p, grid, pos, vel, hl0, want = G.highlight_case(g, case)
+    cfg = O.make_cfg(pos.shape[0], grid, p)
+    end = np.zeros(int(np.prod(grid)), np.int32)
+    vort = np.zeros(pos.shape[0], np.float32)
+    hl = hl0.copy()
+    for s in range(want.shape[0]):
+        pos, vel, end, vort = O.step(cfg, pos, vel, end, vort, O.ORDER_REFERENCE, highlight=hl)
+        assert_bitexact(hl, want[s], f"highlight {case} step {s}")
+    # the fixture marks neighbours and keeps only bit 0 of the host's words
+    assert (want[0] & 2).sum() > 0 and (want[0] & ~np.uint32(3)).sum() == 0
+
+
+def test_highlight_closed_form():
+    """Flags after the passes = (bit 0 of the old word) | 2 for every id that is a candidate (not
+    the particle's own slot) in a row of a flagged particle, counted by brute force over the rows."""
+    g = G.load("highlight")
+    p, grid, pos, vel, hl0, _ = G.highlight_case(g, "collide")
+    cfg = O.make_cfg(pos.shape[0], grid, p)
+    n = pos.shape[0]
+    srt, _ = O.sort(cfg, O.predict(cfg, pos, vel))
+    start, end = O.findcells(cfg, srt, np.zeros(int(np.prod(grid)), np.int32))
+    nbr = O.neighbourcells(cfg, srt, start, end)
+    ids = srt[:, 3].view(np.int32)
+    want = hl0 & np.uint32(1)
+    words = nbr[:, :, :3].reshape(n, 9)
+    for gsl in np.nonzero((hl0[ids] & 1) == 1)[0]:
+        for d in words[gsl]:
+            e, s0 = int(d) >> 24, int(d) & 0xFFFFFF
+            for j in range(s0, s0 + e):
+                if j != gsl:
+                    want[ids[j] if j < n else 0] |= 2
+    assert_bitexact(O.highlight(cfg, srt, nbr, hl0), want, "highlight closed form")
