@@ -134,6 +134,7 @@ SIGNATURES = {
     "pbf_stage_updatepos": ([u32, grid3, pp, vp, vp, vp, vp, vp], C.c_int),
     "pbf_stage_update": ([u32, pp, vp, vp, vp, vp], C.c_int),
     "pbf_stage_vorticity": ([u32, grid3, pp, vp, vp, vp, vp, vp], C.c_int),
+    "pbf_stage_vorticity_noalloc": ([u32, grid3, pp, vp, vp, vp, vp, vp], C.c_int),
 }
 
 # Entry points added by later units are bound when present (see OPTIONAL).

@@ -3,6 +3,7 @@
 // orchestration only; the kernels live in pbf_kernels.hip and pbf_sort.hip.
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -983,6 +984,21 @@ static int nbr_scratch(uint32_t n, const void *nbr12, int32_t **nbr9, hipStream_
     return PBF_OK;
 }
 
+// The two vorticity passes of the stage on caller memory, with the [9][n]
+// table in nbr9 (already converted) and 3 n float4 of scratch in tmp.
+static void launch_vorticity_stage(uint32_t n, const int32_t grid[3], const pbf_params &p, const void *rec4,
+                                   void *vel4, void *vort_out, const int32_t *nbr9, float4 *tmp, hipStream_t st) {
+    const SolverConsts c = make_consts(n, grid, p, key_bits_of(grid));
+    const float4 *rec = (const float4 *)rec4;
+    float4 *vel = (float4 *)vel4;
+    float4 *vs = tmp, *vx = tmp + n, *om = tmp + 2 * (size_t)n;
+    // sorted copy of the velocities, vs[g] = vel[id(rec[g])] (SPH::Run keeps
+    // one from the update pass)
+    launch_gather_by_id((int32_t)n, rec, vel, vs, st);
+    launch_vorticity_a(c, rec, nbr9, vs, vel, vx, om, (float *)vort_out, PairList(), st);
+    launch_vorticity_b(c, rec, nbr9, PairList(), vx, om, (const float *)vort_out, vel, st);
+}
+
 static int release_scratch(void *p, hipStream_t st) {
     hipError_t e = hipStreamSynchronize(st);
     (void)hipFree(p);
@@ -1042,19 +1058,73 @@ int pbf_stage_vorticity(uint32_t n, const int32_t grid[3], const pbf_params *p, 
         (void)hipFree(nbr9);
         return hip_fail(e, "pbf_stage_vorticity");
     }
-    const SolverConsts c = make_consts(n, grid, *p, key_bits_of(grid));
-    const float4 *rec = (const float4 *)rec4;
-    float4 *vel = (float4 *)vel4;
-    float4 *vs = tmp, *vx = tmp + n, *om = tmp + 2 * (size_t)n;
-    // sorted copy of the velocities, vs[g] = vel[id(rec[g])] (SPH::Run keeps
-    // one from the update pass)
-    launch_gather_by_id((int32_t)n, rec, vel, vs, st);
-    launch_vorticity_a(c, rec, nbr9, vs, vel, vx, om, (float *)vort_out, PairList(), st);
-    launch_vorticity_b(c, rec, nbr9, PairList(), vx, om, (const float *)vort_out, vel, st);
+    launch_vorticity_stage(n, grid, *p, rec4, vel4, vort_out, nbr9, tmp, st);
     e = hipStreamSynchronize(st);
     (void)hipFree(nbr9);
     (void)hipFree(tmp);
     if (e != hipSuccess) return hip_fail(e, "pbf_stage_vorticity");
+    PBF_HIP(hipGetLastError());
+    return PBF_OK;
+}
+
+
+// ------------------------------------------------------ vorticity, no allocation
+// Scratch of pbf_stage_vorticity_noalloc: the [9][n] neighbour table (36 B per
+// particle, then aligned) followed by the sorted, XSPH and omega velocities
+// (48 B per particle), for at most PBF_NOALLOC_MAX_N particles. Each GPU the
+// library runs on holds one copy until the process ends. In host code the
+// name `scratch` is a host-side stand-in, not a device address; the address
+// on a device comes from hipGetSymbolAddress, looked up once per device with
+// that device current, and only that address is passed to kernels.
+}  // extern "C"
+
+static constexpr size_t kNoAllocTableBytes = (size_t)36 * PBF_NOALLOC_MAX_N;
+static constexpr size_t kNoAllocBytes = kNoAllocTableBytes + (size_t)48 * PBF_NOALLOC_MAX_N;
+static_assert(kNoAllocTableBytes % 256 == 0, "the velocity arrays start 256-byte aligned");
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_dev[kMaxScratchDevices];  // device address of `scratch`, per device
+
+// The current device's address of the scratch array (looked up on its first use).
+int scratch_of_current_device(void **out) {
+    int dev = 0;
+    PBF_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxScratchDevices) return invalid("device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "pbf_stage_vorticity_noalloc: hipGetSymbolAddress(scratch)");
+        if (!p) return invalid("pbf_stage_vorticity_noalloc: no device address for the scratch array");
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return PBF_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int pbf_stage_vorticity_noalloc(uint32_t n, const int32_t grid[3], const pbf_params *p, const void *rec4,
+                                const void *nbr12, void *vel4, void *vort_out, void *stream) {
+    int rc = stage_check(n, grid, p);
+    if (rc) return rc;
+    if (n > PBF_NOALLOC_MAX_N)
+        return invalid("pbf_stage_vorticity_noalloc: n = " + std::to_string(n) + " exceeds PBF_NOALLOC_MAX_N = " +
+                       std::to_string(PBF_NOALLOC_MAX_N) + " (the fixed scratch array); use pbf_stage_vorticity");
+    if (!rec4 || !nbr12 || !vel4 || !vort_out) return invalid("null pointer");
+    void *base = nullptr;
+    if ((rc = scratch_of_current_device(&base))) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    int32_t *nbr9 = (int32_t *)base;
+    float4 *tmp = (float4 *)((unsigned char *)base + kNoAllocTableBytes);
+    // every part of the scratch is written before it is read: nothing has
+    // to start at zero, whatever an earlier call left there
+    launch_nbr12_to_9((int32_t)n, (const int32_t *)nbr12, nbr9, st);
+    launch_vorticity_stage(n, grid, *p, rec4, vel4, vort_out, nbr9, tmp, st);
     PBF_HIP(hipGetLastError());
     return PBF_OK;
 }

@@ -390,3 +390,10 @@ def stage_update(n, p: PbfParams, rec_dev, pos_dev, vel_dev, stream=None):
 def stage_vorticity(n, grid, p: PbfParams, rec_dev, nbr12_dev, vel_dev, vort_dev, stream=None):
     check(_lib.load().pbf_stage_vorticity(n, _grid(grid), C.byref(p), rec_dev, nbr12_dev, vel_dev, vort_dev,
                                           stream))
+
+
+def stage_vorticity_noalloc(n, grid, p: PbfParams, rec_dev, nbr12_dev, vel_dev, vort_dev, stream=None):
+    """stage_vorticity with the library's fixed scratch array instead of device
+    allocations (n at most PBF_NOALLOC_MAX_N); asynchronous on `stream`."""
+    check(_lib.load().pbf_stage_vorticity_noalloc(n, _grid(grid), C.byref(p), rec_dev, nbr12_dev, vel_dev,
+                                                  vort_dev, stream))

@@ -58,3 +58,14 @@ def test_cpp_classes_compile_and_link(tmp_path):
         build_cpp_demo()
     res = subprocess.run([CPP_DEMO], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     assert res.returncode == 2 and "usage" in res.stdout
+
+
+def test_noalloc_stage_rejects_sizes_beyond_its_scratch_without_a_gpu():
+    """The size check of pbf_stage_vorticity_noalloc comes before any HIP call."""
+    lib = _lib.load()
+    p = _lib.PbfParams()
+    lib.pbf_params_default(C.byref(p), 128, 64, 128)
+    g3 = (C.c_int32 * 3)(128, 64, 128)
+    dummy = C.c_void_p(256)  # never dereferenced
+    assert lib.pbf_stage_vorticity_noalloc(131072 + 256, g3, C.byref(p), dummy, dummy, dummy, dummy, None) == -1
+    assert b"PBF_NOALLOC_MAX_N" in lib.pbf_last_error()
