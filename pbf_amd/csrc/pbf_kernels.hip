@@ -43,13 +43,19 @@ __device__ __forceinline__ float wpoly6(const SolverConsts &c, float sq, float r
 // quotients correctly rounded (pbf_math.h). With h = 1 the denominator l is at
 // most 1 and the numerators at most 14.33 (1 + 2^-22) in magnitude (each
 // component of r within l up to rounding), inside the fast upper bounds.
+// A component whose numerator is zero (the two positions share that
+// coordinate) may come out +0 where the reference has -0: every use of a
+// gradient component, alone or in a product, is added to a sum that starts
+// at +0 and is never -0 (x + y is -0 in round-to-nearest only when both are
+// -0), and adding a zero of either sign to such a sum leaves it unchanged, so
+// no result depends on that sign.
 template <bool H1>
 __device__ __forceinline__ void grad_spiky(const SolverConsts &c, float dx, float dy, float dz, float l, float &gx,
                                            float &gy, float &gz) {
     const float t = c.h - l;
     const float s = (SPIKY_GRAD * t) * t;
     const float den = H1 ? l : (((((l * c.h) * c.h) * c.h) * c.h) * c.h) * c.h;
-    div3_rn<H1>(s * dx, s * dy, s * dz, den, gx, gy, gz);
+    div3_rn<H1, false>(s * dx, s * dy, s * dz, den, gx, gy, gz);
 }
 
 __device__ __forceinline__ float clamp_glsl(float x, float lo, float hi) {
@@ -897,7 +903,7 @@ struct LambdaPass {
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const float sq = sq_len(dx, dy, dz);
         if (sq > c.h2cut) return false;
-        const float r = sqrt_rn(sq);
+        const float r = sqrt_rn_finite(sq);
         if (r > c.h) return false;
         P[0] = wpoly6<H1>(c, sq, r);
         if (r == 0.0f) return true;
@@ -951,7 +957,7 @@ struct UpdatePosPass {
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const float sq = sq_len(dx, dy, dz);
         if (!listed && sq > c.h2cut) return false;
-        const float r = sqrt_rn(sq);
+        const float r = sqrt_rn_finite(sq);
         if ((!listed && r > c.h) || r == 0.0f) return false;
         float sc = c.scale * wpoly6<H1>(c, sq, r);
         sc *= sc;
@@ -1000,7 +1006,7 @@ struct VorticityAPass {
         const float px = pi.x - pj.x, py = pi.y - pj.y, pz = pi.z - pj.z;
         const float sq = sq_len(px, py, pz);
         if (sq > c.h2cut) return false;
-        const float r = sqrt_rn(sq);
+        const float r = sqrt_rn_finite(sq);
         if (r > c.h) return false;
         const float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
         const float w = wpoly6<H1>(c, sq, r);
@@ -1043,7 +1049,7 @@ struct VorticityBPass {
         const float px = pi.x - pj.x, py = pi.y - pj.y, pz = pi.z - pj.z;
         const float sq = sq_len(px, py, pz);
         if (!listed && sq > c.h2cut) return false;
-        const float r = sqrt_rn(sq);
+        const float r = sqrt_rn_finite(sq);
         if ((!listed && r > c.h) || r == 0.0f) return false;
         float gx, gy, gz;
         grad_spiky<H1>(c, px, py, pz, r, gx, gy, gz);
@@ -1108,7 +1114,7 @@ __global__ __launch_bounds__(NB) void k_lambda(SolverConsts c, Dom dom, const fl
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const float sq = sq_len(dx, dy, dz);
         if (sq > c.h2cut) return;
-        const float r = sqrt_rn(sq);
+        const float r = sqrt_rn_finite(sq);
         if (r > c.h) return;
         out.add(i, c.n, code);
         rho += wpoly6<H1>(c, sq, r);
@@ -1166,7 +1172,7 @@ __global__ __launch_bounds__(NB) void k_updatepos(SolverConsts c, Dom dom, const
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const float sq = sq_len(dx, dy, dz);
         if (!listed && sq > c.h2cut) return;
-        const float r = sqrt_rn(sq);
+        const float r = sqrt_rn_finite(sq);
         if ((!listed && r > c.h) || r == 0.0f) return;
         float sc = c.scale * wpoly6<H1>(c, sq, r);
         sc *= sc;
@@ -1238,7 +1244,7 @@ __global__ __launch_bounds__(NB) void k_vorticity_a(SolverConsts c, Dom dom, con
         const float px = pi.x - pj.x, py = pi.y - pj.y, pz = pi.z - pj.z;
         const float sq = sq_len(px, py, pz);
         if (sq > c.h2cut) return;
-        const float r = sqrt_rn(sq);
+        const float r = sqrt_rn_finite(sq);
         if (r > c.h) return;
         out.add(i, c.n, code);
         const float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
@@ -1297,7 +1303,7 @@ __global__ __launch_bounds__(NB) void k_vorticity_b(SolverConsts c, Dom dom, con
         const float px = pi.x - pj.x, py = pi.y - pj.y, pz = pi.z - pj.z;
         const float sq = sq_len(px, py, pz);
         if (!listed && sq > c.h2cut) return;
-        const float r = sqrt_rn(sq);
+        const float r = sqrt_rn_finite(sq);
         if ((!listed && r > c.h) || r == 0.0f) return;
         float gx, gy, gz;
         grad_spiky<H1>(c, px, py, pz, r, gx, gy, gz);

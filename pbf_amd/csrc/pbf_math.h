@@ -61,6 +61,18 @@ __device__ __forceinline__ float sqrt_rn(float x) {
     return r;
 }
 
+// sqrt_rn for callers that know x is not +inf (a squared distance already
+// compared with a finite bound): only the lower end of the fast range is
+// tested; NaN fails that test and takes the general sequence.
+__device__ __forceinline__ float sqrt_rn_finite(float x) {
+    float r = sqrt_rn_fast(x);
+    if (lanes(x >= kSqrtLo) != active_lanes()) {
+        asm volatile("" ::: "memory");
+        if (!(x >= kSqrtLo)) r = sqrtf(x);
+    }
+    return r;
+}
+
 // RN(1 / b) for |b| in [2^-60, 2^60): the hardware reciprocal refined by one
 // Newton step in FMA form. The self-test checks it against the IEEE division
 // for every float of that range.
@@ -74,12 +86,15 @@ __device__ __forceinline__ float rcp_rn_fast(float b) {
 // a/b, the remainder r = a - b q is exact (one FMA) and RN(q + r y) = RN(a/b)
 // when nothing underflows (Markstein 1990; Muller et al., Handbook of
 // Floating-Point Arithmetic, Thm. 4.10). Valid for |a| in [2^-96, 2^60] and b
-// in [2^-60, 2^30]; zero a returns a (the signed zero a / b gives, b > 0).
+// in [2^-60, 2^30]. A zero a gives +0 from the two FMAs whatever its sign;
+// SIGNED_ZERO returns a instead (the signed zero a / b gives, b > 0).
+template <bool SIGNED_ZERO = true>
 __device__ __forceinline__ float div_markstein(float a, float b, float y) {
     const float q = a * y;
     const float r = fmaf(-q, b, a);
     const float q1 = fmaf(r, y, q);
-    return a == 0.0f ? a : q1;
+    if constexpr (SIGNED_ZERO) return a == 0.0f ? a : q1;
+    return q1;
 }
 
 // True when a is zero or |a| lies in the fast numerator range.
@@ -97,13 +112,15 @@ __device__ __forceinline__ bool fast_numerator_lo(float a) {
 // RN(a_k / b), k = 0..2, for b > 0: the fast path for every lane, replaced by
 // the IEEE division in the lanes whose operands leave the fast ranges.
 // BOUNDED: the caller guarantees b <= kDenHi and |a_k| <= kNumHi, so only the
-// lower bounds are tested.
-template <bool BOUNDED = false>
+// lower bounds are tested. SIGNED_ZERO false: a zero quotient may come out as
+// +0 where a / b is -0 (every other quotient is a / b bit for bit), for
+// callers whose results do not depend on the sign of a zero quotient.
+template <bool BOUNDED = false, bool SIGNED_ZERO = true>
 __device__ __forceinline__ void div3_rn(float a0, float a1, float a2, float b, float &q0, float &q1, float &q2) {
     const float y = rcp_rn_fast(b);
-    q0 = div_markstein(a0, b, y);
-    q1 = div_markstein(a1, b, y);
-    q2 = div_markstein(a2, b, y);
+    q0 = div_markstein<SIGNED_ZERO>(a0, b, y);
+    q1 = div_markstein<SIGNED_ZERO>(a1, b, y);
+    q2 = div_markstein<SIGNED_ZERO>(a2, b, y);
     const bool fast = BOUNDED ? (b >= kDenLo) & fast_numerator_lo(a0) & fast_numerator_lo(a1) & fast_numerator_lo(a2)
                               : (b >= kDenLo) & (b <= kDenHi) & fast_numerator(a0) & fast_numerator(a1) &
                                     fast_numerator(a2);

@@ -4,7 +4,8 @@
 //   square root every float x in [2^-96, 2^8)                      (exhaustive)
 //   division    pseudo-random a, b of the fast ranges, including the
 //               solver's operands a = s dx, b = l in (0, 1]         (sampled;
-//               the bounded form on the classes inside its bounds)
+//               the bounded form on the classes inside its bounds, and its
+//               form without the sign of a zero quotient up to that sign)
 #include "pbf_internal.h"
 #include "pbf_math.h"
 
@@ -97,6 +98,11 @@ __global__ __launch_bounds__(ST_BLOCK) void k_test_div(uint64_t count, uint64_t 
             if ((k & 3) != 1) {  // classes within b <= 2^30, |a| <= 2^60: the bounded form too
                 div3_rn<true>(a, a, a, b, q0, q1, q2);
                 mis |= __float_as_uint(q0) != __float_as_uint(a / b);
+                // and the form that may drop the sign of a zero quotient (the
+                // solver's gradient): equal bits, or two zeros
+                div3_rn<true, false>(a, a, a, b, q0, q1, q2);
+                const float ref = a / b;
+                mis |= (__float_as_uint(q0) != __float_as_uint(ref)) & !((q0 == 0.0f) & (ref == 0.0f));
             }
         }
         count_mismatch(mis, bad);
