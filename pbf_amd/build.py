@@ -14,6 +14,8 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libpbf_amd.so")
 CPP_DEMO_SRC = os.path.join(ROOT, "tests", "cpp", "sph_cpp.cpp")
 CPP_DEMO = os.path.join(ROOT, "tests", "cpp", "sph_cpp")
+CLI_SRC = os.path.join(HERE, "cli", "pbf_run.cpp")
+CLI = os.path.join(HERE, "pbf_run")
 SOURCES = ["pbf_kernels.hip", "pbf_sort.hip", "pbf_selftest.hip", "pbf_batched.hip", "pbf_adjoint.hip",
            "pbf_capi.hip"]
 
@@ -57,6 +59,7 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if os.path.exists(CPP_DEMO_SRC):
         build_cpp_demo()
+    build_cli()
     return LIB
 
 
@@ -69,6 +72,18 @@ def build_cpp_demo() -> str:
         raise RuntimeError(f"g++ failed ({res.returncode}):\n{res.stdout}")
     os.replace(CPP_DEMO + ".tmp", CPP_DEMO)
     return CPP_DEMO
+
+
+def build_cli() -> str:
+    """g++ build of pbf_run (scene files through include/pbf.hpp), linked to the
+    in-tree library by rpath."""
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-ffp-contract=off", "-I", os.path.join(ROOT, "include"),
+           "-o", CLI + ".tmp", CLI_SRC, "-L", HERE, "-l:libpbf_amd.so", "-Wl,-rpath,$ORIGIN"]
+    res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"g++ failed ({res.returncode}):\n{res.stdout}")
+    os.replace(CLI + ".tmp", CLI)
+    return CLI
 
 
 if __name__ == "__main__":
