@@ -333,6 +333,20 @@ int pbf_adjoint_backward(pbf_adjoint *a, const void *grad_pos4, int host, void *
 /* steps steps from (pos4, vel4) with the structure kept by the last forward
  * (the map whose derivative backward computes); final positions to pos_out4. */
 int pbf_adjoint_replay(pbf_adjoint *a, const void *pos4, const void *vel4, int host, void *pos_out4);
+/* pbf_adjoint_replay with every clamp to the walls (updatepos.glsl:98-100)
+ * taking the decision the last forward took at that step, iteration, particle
+ * and axis (inside, lower wall, upper wall): the smooth piece of the map that
+ * backward differentiates. clamp_flips: how many of those decisions the clamp
+ * would have taken otherwise on this replay's path (synchronizes). */
+int pbf_adjoint_replay_frozen(pbf_adjoint *a, const void *pos4, const void *vel4, int host, void *pos_out4,
+                              uint64_t *clamp_flips);
+/* FP64 handles: the directional derivative of pbf_adjoint_replay's final
+ * positions along (dvel4, dxsph_c, dvorticity_epsilon), by forward-mode
+ * differentiation (dual numbers) of the same forward kernels, into dpos_out4
+ * (and the positions into pos_out4 unless null): a second derivative of the
+ * map, independent of backward's hand-written reverse kernels. */
+int pbf_adjoint_jvp(pbf_adjoint *a, const void *pos4, const void *vel4, const void *dvel4, double dxsph_c,
+                    double dvorticity_epsilon, int host, void *pos_out4, void *dpos_out4);
 int pbf_adjoint_set_stream(pbf_adjoint *a, void *stream);
 void *pbf_adjoint_get_stream(const pbf_adjoint *a);
 

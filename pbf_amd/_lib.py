@@ -127,6 +127,8 @@ SIGNATURES = {
     "pbf_adjoint_velocities": ([vp, vp, C.c_int], C.c_int),
     "pbf_adjoint_backward": ([vp, vp, C.c_int, vp, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
     "pbf_adjoint_replay": ([vp, vp, vp, C.c_int, vp], C.c_int),
+    "pbf_adjoint_replay_frozen": ([vp, vp, vp, C.c_int, vp, C.POINTER(C.c_uint64)], C.c_int),
+    "pbf_adjoint_jvp": ([vp, vp, vp, vp, C.c_double, C.c_double, C.c_int, vp, vp], C.c_int),
     "pbf_adjoint_set_stream": ([vp, vp], C.c_int),
     "pbf_adjoint_get_stream": ([vp], vp),
     "pbf_stage_predict": ([u32, grid3, pp, vp, vp, vp, vp], C.c_int),

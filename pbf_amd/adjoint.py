@@ -97,6 +97,27 @@ class Rollout:
         check(self._lib.pbf_adjoint_replay(self._h, pos4.ctypes.data, vel4.ctypes.data, 1, out.ctypes.data))
         return out
 
+    def replay_frozen(self, pos4, vel4):
+        """replay with the last forward's clamp decisions imposed (the smooth
+        piece that backward differentiates); returns (positions, number of
+        clamp decisions this replay's path would have taken otherwise)."""
+        pos4, vel4 = self._arr(pos4), self._arr(vel4)
+        out = np.empty((self.n, 4), self.dtype)
+        flips = C.c_uint64()
+        check(self._lib.pbf_adjoint_replay_frozen(self._h, pos4.ctypes.data, vel4.ctypes.data, 1, out.ctypes.data,
+                                                  C.byref(flips)))
+        return out, int(flips.value)
+
+    def jvp(self, pos4, vel4, dvel4, dc=0.0, de=0.0):
+        """FP64 handles: (positions, d positions) of replay along (dvel4, dc, de) by
+        forward-mode dual numbers."""
+        pos4, vel4, dvel4 = self._arr(pos4), self._arr(vel4), self._arr(dvel4)
+        out = np.empty((self.n, 4), self.dtype)
+        dout = np.empty((self.n, 4), self.dtype)
+        check(self._lib.pbf_adjoint_jvp(self._h, pos4.ctypes.data, vel4.ctypes.data, dvel4.ctypes.data, float(dc),
+                                        float(de), 1, out.ctypes.data, dout.ctypes.data))
+        return out, dout
+
     # device (torch) interface: tensors on the GPU. The handle works on its own
     # stream, so torch's stream is drained before and the handle's after.
     def _sync(self):

@@ -24,6 +24,7 @@
 // each segment's steps from its checkpoint into a tape, then runs the segment
 // in reverse.
 #include <cstring>
+#include <type_traits>
 #include <new>
 #include <vector>
 
@@ -35,9 +36,60 @@ namespace {
 constexpr int AB = 256;  // workgroup size of the adjoint kernels
 
 template <class T>
-struct alignas(sizeof(T) * 4) T4 {
+struct alignas(sizeof(T) * 4 > 32 ? 32 : sizeof(T) * 4) T4 {
     T x, y, z, w;
 };
+
+// Forward-mode dual number in double: a value and its derivative along one
+// direction of the inputs. The forward kernels run on it unchanged, which
+// gives directional derivatives of the FP64 map without any finite step
+// (pbf_adjoint_jvp): an AD path independent of the hand-written reverse
+// kernels. Comparisons and the clamp act on the value, so the derivative is
+// that of the same smooth piece the reverse pass differentiates.
+struct Dual {
+    double v, d;
+    __host__ __device__ Dual() : v(0.0), d(0.0) {}
+    __host__ __device__ Dual(double x) : v(x), d(0.0) {}
+    __host__ __device__ Dual(int x) : v((double)x), d(0.0) {}
+    __host__ __device__ Dual(double x, double dx) : v(x), d(dx) {}
+    __host__ __device__ explicit operator float() const { return (float)v; }
+    __device__ Dual &operator+=(const Dual &b) {
+        v += b.v;
+        d += b.d;
+        return *this;
+    }
+    __device__ Dual &operator-=(const Dual &b) {
+        v -= b.v;
+        d -= b.d;
+        return *this;
+    }
+    __device__ Dual &operator*=(const Dual &b) {
+        d = d * b.v + v * b.d;
+        v *= b.v;
+        return *this;
+    }
+};
+__device__ __forceinline__ Dual operator+(Dual a, const Dual &b) { return a += b; }
+__device__ __forceinline__ Dual operator-(Dual a, const Dual &b) { return a -= b; }
+__device__ __forceinline__ Dual operator-(const Dual &a) { return Dual(-a.v, -a.d); }
+__device__ __forceinline__ Dual operator*(Dual a, const Dual &b) { return a *= b; }
+__device__ __forceinline__ Dual operator/(const Dual &a, const Dual &b) {
+    const double q = a.v / b.v;
+    return Dual(q, (a.d - q * b.d) / b.v);
+}
+__device__ __forceinline__ bool operator<(const Dual &a, const Dual &b) { return a.v < b.v; }
+__device__ __forceinline__ bool operator>(const Dual &a, const Dual &b) { return a.v > b.v; }
+__device__ __forceinline__ bool operator<=(const Dual &a, const Dual &b) { return a.v <= b.v; }
+__device__ __forceinline__ bool operator>=(const Dual &a, const Dual &b) { return a.v >= b.v; }
+__device__ __forceinline__ bool operator==(const Dual &a, const Dual &b) { return a.v == b.v; }
+__device__ __forceinline__ bool operator!=(const Dual &a, const Dual &b) { return a.v != b.v; }
+// (the overloads of the element types stay visible next to the dual one)
+__device__ __forceinline__ float sqrt(float x) { return ::sqrtf(x); }
+__device__ __forceinline__ double sqrt(double x) { return ::sqrt(x); }
+__device__ __forceinline__ Dual sqrt(const Dual &a) {
+    const double s = ::sqrt(a.v);
+    return Dual(s, s > 0.0 ? a.d / (2.0 * s) : 0.0);
+}
 
 template <class T>
 __device__ __forceinline__ T sqlen(T x, T y, T z) {
@@ -68,6 +120,10 @@ template <>
 __device__ __forceinline__ float spikyc<float>() { return -3.0f * 4.774648292756860f; }
 template <>
 __device__ __forceinline__ double spikyc<double>() { return -3.0 * 4.774648292756860; }
+template <>
+__device__ __forceinline__ Dual poly6c<Dual>() { return Dual(1.56668147106); }
+template <>
+__device__ __forceinline__ Dual spikyc<Dual>() { return Dual(-3.0 * 4.774648292756860); }
 
 // W_poly6 of sq = r^2 (r <= h).
 template <class T>
@@ -264,11 +320,18 @@ __global__ __launch_bounds__(AB) void ka_lambda(AConsts<T> c, const T4<T> *__res
     }
 }
 
-// updatepos.glsl:42-105, race-free; mask bit a set where axis a was not clamped.
+// updatepos.glsl:42-105, race-free. Clamp state per particle (mask): bit a
+// set where axis a was not clamped, bit 3 + a where it was clamped to the
+// upper wall. frozen (a forward's recorded states, or null): every axis takes
+// the recorded decision instead of its own (the map whose derivative the
+// reverse pass computes), and flips counts the decisions the clamp would have
+// made otherwise.
 template <class T>
 __global__ __launch_bounds__(AB) void ka_updatepos(AConsts<T> c, const T4<T> *__restrict__ p,
                                                    const T *__restrict__ lam, const int32_t *__restrict__ nbr9,
-                                                   T4<T> *__restrict__ out, uint8_t *__restrict__ mask) {
+                                                   T4<T> *__restrict__ out, uint8_t *__restrict__ mask,
+                                                   const uint8_t *__restrict__ frozen,
+                                                   unsigned long long *__restrict__ flips) {
     const int32_t i = blockIdx.x * AB + threadIdx.x;
     if (i >= c.n) return;
     const T4<T> pi = p[i];
@@ -297,6 +360,19 @@ __global__ __launch_bounds__(AB) void ka_updatepos(AConsts<T> c, const T4<T> *__
     for (int a = 0; a < 3; a++) {
         o[a] = q[a] < c.lo[a] ? c.lo[a] : (q[a] > c.hi[a] ? c.hi[a] : q[a]);
         if (q[a] >= c.lo[a] && q[a] <= c.hi[a]) m |= (uint8_t)(1u << a);
+        else if (q[a] > c.hi[a]) m |= (uint8_t)(8u << a);
+    }
+    if (frozen) {
+        const uint8_t f = frozen[i];
+        unsigned k = 0;  // axes whose decision (inside, lower wall, upper wall) differs
+        for (int a = 0; a < 3; a++) {
+            const unsigned df = ((f >> a) & 1u) ? 0u : (((f >> (3 + a)) & 1u) ? 2u : 1u);
+            const unsigned dm = ((m >> a) & 1u) ? 0u : (((m >> (3 + a)) & 1u) ? 2u : 1u);
+            o[a] = df == 0u ? q[a] : (df == 2u ? c.hi[a] : c.lo[a]);
+            k += df != dm;
+        }
+        if (k) atomicAdd(flips, (unsigned long long)k);
+        m = f;
     }
     out[i] = T4<T>{o[0], o[1], o[2], pi.w};
     if (mask) mask[i] = m;
@@ -662,6 +738,24 @@ __global__ __launch_bounds__(AB) void kr_predict(AConsts<T> c, const uint32_t *_
     vbar_old[id] = T4<T>{c.dt * b.x, c.dt * b.y, c.dt * b.z, 0};
 }
 
+// State to dual numbers (derivative from dx, or zero) and back.
+__global__ __launch_bounds__(AB) void ka_to_dual(int32_t n, const T4<double> *__restrict__ x,
+                                                 const T4<double> *__restrict__ dx, T4<Dual> *__restrict__ out) {
+    const int32_t i = blockIdx.x * AB + threadIdx.x;
+    if (i >= n) return;
+    const T4<double> a = x[i];
+    const T4<double> b = dx ? dx[i] : T4<double>{0.0, 0.0, 0.0, 0.0};
+    out[i] = T4<Dual>{Dual(a.x, b.x), Dual(a.y, b.y), Dual(a.z, b.z), Dual(a.w, 0.0)};
+}
+__global__ __launch_bounds__(AB) void ka_from_dual(int32_t n, const T4<Dual> *__restrict__ in,
+                                                   T4<double> *__restrict__ val, T4<double> *__restrict__ der) {
+    const int32_t i = blockIdx.x * AB + threadIdx.x;
+    if (i >= n) return;
+    const T4<Dual> a = in[i];
+    val[i] = T4<double>{a.x.v, a.y.v, a.z.v, 0.0};
+    der[i] = T4<double>{a.x.d, a.y.d, a.z.d, 0.0};
+}
+
 template <class T>
 __global__ __launch_bounds__(AB) void ka_zero(T *__restrict__ p, size_t count) {
     for (size_t k = (size_t)blockIdx.x * AB + threadIdx.x; k < count; k += (size_t)gridDim.x * AB) p[k] = (T)0;
@@ -680,7 +774,9 @@ struct AdjBase {
     virtual int positions(void *pos4, int host) = 0;
     virtual int velocities(void *vel4, int host) = 0;
     virtual int backward(const void *gpos4, int host, void *gvel4, double *gc, double *geps) = 0;
-    virtual int replay(const void *pos4, const void *vel4, int host, void *pos_out4) = 0;
+    virtual int replay(const void *pos4, const void *vel4, int host, void *pos_out4, uint64_t *clamp_flips) = 0;
+    virtual int jvp(const void *pos4, const void *vel4, const void *dvel4, double dc, double de, int host,
+                    void *pos_out4, void *dpos_out4) = 0;
     virtual void set_coefficients(double c, double eps) = 0;
     virtual void set_stream(hipStream_t st) = 0;
     virtual hipStream_t stream() const = 0;
@@ -724,6 +820,8 @@ struct Adj : AdjBase {
     // structure of every step (kept on request, for replays)
     uint32_t *all_perm = nullptr;
     int32_t *all_nbr = nullptr;
+    uint8_t *all_clamp = nullptr;            // clamp states of every step and iteration of the forward
+    unsigned long long *flips = nullptr;     // clamp decisions a frozen replay overrode
     bool have_all = false;
     // tape of one segment
     std::vector<StepTape<T>> tape;
@@ -735,6 +833,10 @@ struct Adj : AdjBase {
     T *ob = nullptr, *lb = nullptr;
     double *acc = nullptr;
     bool forwarded = false;
+    // dual-number state of pbf_adjoint_jvp (FP64 handles, allocated on first use)
+    T4<Dual> *jx = nullptr, *jv = nullptr, *jxs = nullptr, *jpa = nullptr, *jpb = nullptr, *jvs = nullptr,
+             *jvx = nullptr, *jom = nullptr;
+    Dual *jlam = nullptr, *jo = nullptr;
 
     ~Adj() override { release(); }
 
@@ -756,6 +858,8 @@ struct Adj : AdjBase {
         for (auto &p : ck_epoch) dfree(p);
         dfree(all_perm);
         dfree(all_nbr);
+        dfree(all_clamp);
+        dfree(flips);
         for (auto &t : tape) {
             dfree(t.p);
             dfree(t.lam);
@@ -782,6 +886,16 @@ struct Adj : AdjBase {
         dfree(ob);
         dfree(lb);
         dfree(acc);
+        dfree(jx);
+        dfree(jv);
+        dfree(jxs);
+        dfree(jpa);
+        dfree(jpb);
+        dfree(jvs);
+        dfree(jvx);
+        dfree(jom);
+        dfree(jlam);
+        dfree(jo);
         if (own) (void)hipStreamDestroy(own);
         own = nullptr;
     }
@@ -815,6 +929,8 @@ struct Adj : AdjBase {
                 return e;
         }
         if ((e = dalloc(&all_perm, (size_t)steps * n)) || (e = dalloc(&all_nbr, (size_t)steps * 9 * n))) return e;
+        if ((e = dalloc(&all_clamp, (size_t)steps * (iters > 0 ? iters : 1) * n)) || (e = dalloc(&flips, 1)))
+            return e;
         tape.assign(every, StepTape<T>());
         for (auto &t : tape) {
             if ((e = dalloc(&t.p, (size_t)(iters + 1) * n)) || (e = dalloc(&t.lam, (size_t)iters * n)) ||
@@ -885,7 +1001,10 @@ struct Adj : AdjBase {
 
     // One step on (x, v) in place. With a tape, everything the reverse pass
     // reads is kept; perm/nbr9 given: fixed structure, else computed into them.
-    void step(StepTape<T> *tp, uint32_t *perm, int32_t *nbr9, bool compute_structure) {
+    // clamp: the step's clamp states (iters x n) to record, or with frozen,
+    // the recorded ones to impose (counting overridden decisions in flips).
+    void step(StepTape<T> *tp, uint32_t *perm, int32_t *nbr9, bool compute_structure, uint8_t *clamp = nullptr,
+              bool frozen = false) {
         const unsigned nb = ablocks(n);
         ka_predict<T><<<nb, AB, 0, st>>>(c, x, v, xs, xs_f);
         if (compute_structure) structure(perm, nbr9);
@@ -897,7 +1016,10 @@ struct Adj : AdjBase {
             T *lam = tp ? tp->lam + (size_t)it * n : lam_tmp;
             ka_lambda<T><<<nb, AB, 0, st>>>(c, cur, nbr9, lam, tp ? tp->sden + (size_t)it * n : nullptr,
                                             tp ? tp->G + (size_t)it * n : nullptr);
-            ka_updatepos<T><<<nb, AB, 0, st>>>(c, cur, lam, nbr9, nxt, tp ? tp->mask + (size_t)it * n : nullptr);
+            uint8_t *cl = clamp ? clamp + (size_t)it * n : nullptr;
+            ka_updatepos<T><<<nb, AB, 0, st>>>(c, cur, lam, nbr9, nxt,
+                                               tp ? tp->mask + (size_t)it * n : (frozen ? nullptr : cl),
+                                               frozen ? cl : nullptr, flips);
             cur = nxt;
         }
         T4<T> *vs = tp ? tp->vs : vsb;
@@ -944,7 +1066,8 @@ struct Adj : AdjBase {
         // states and tapes of a segment from its checkpoint with it
         for (int32_t s = 0; s < steps; s++) {
             if (s % every == 0) PBF_HIP(save_ck(s / every));
-            step(nullptr, all_perm + (size_t)s * n, all_nbr + (size_t)s * 9 * n, true);
+            step(nullptr, all_perm + (size_t)s * n, all_nbr + (size_t)s * 9 * n, true,
+                 all_clamp + (size_t)s * iters * n);
         }
         PBF_HIP(hipMemcpyAsync(xf, x, n * sizeof(T4<T>), hipMemcpyDeviceToDevice, st));
         PBF_HIP(hipMemcpyAsync(vf, v, n * sizeof(T4<T>), hipMemcpyDeviceToDevice, st));
@@ -1031,15 +1154,98 @@ struct Adj : AdjBase {
         return PBF_OK;
     }
 
-    int replay(const void *pos4, const void *vel4, int host, void *pos_out4) override {
+    int replay(const void *pos4, const void *vel4, int host, void *pos_out4, uint64_t *clamp_flips) override {
         if (!have_all) return invalid("pbf_adjoint_replay: no forward rollout");
         PBF_HIP(upload(x, v, pos4, vel4, host));
-        for (int32_t s = 0; s < steps; s++) step(nullptr, all_perm + (size_t)s * n, all_nbr + (size_t)s * 9 * n, false);
+        const bool frozen = clamp_flips != nullptr;
+        if (frozen) zero(flips, 1);
+        for (int32_t s = 0; s < steps; s++)
+            step(nullptr, all_perm + (size_t)s * n, all_nbr + (size_t)s * 9 * n, false,
+                 frozen ? all_clamp + (size_t)s * iters * n : nullptr, frozen);
         PBF_HIP(hipGetLastError());
         PBF_HIP(hipMemcpyAsync(pos_out4, x, n * sizeof(T4<T>), host ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice,
                                st));
+        if (frozen) {
+            unsigned long long h = 0;
+            PBF_HIP(hipMemcpyAsync(&h, flips, sizeof h, hipMemcpyDeviceToHost, st));
+            PBF_HIP(hipStreamSynchronize(st));
+            *clamp_flips = h;
+        }
         if (host) PBF_HIP(hipStreamSynchronize(st));
         return PBF_OK;
+    }
+
+    // Directional derivative of the replay map (the structure of the last
+    // forward, the reverse pass's clamp and eta = 0 conventions) along
+    // (dvel4, dc, de), by the forward kernels on dual numbers.
+    int jvp(const void *pos4, const void *vel4, const void *dvel4, double dc, double de, int host, void *pos_out4,
+            void *dpos_out4) override {
+        if constexpr (!std::is_same<T, double>::value) {
+            return invalid("pbf_adjoint_jvp: FP64 handles only");
+        } else {
+            if (!have_all) return invalid("pbf_adjoint_jvp: no forward rollout");
+            hipError_t e = hipSuccess;
+            if (!jx && ((e = dalloc(&jx, n)) || (e = dalloc(&jv, n)) || (e = dalloc(&jxs, n)) ||
+                        (e = dalloc(&jpa, n)) || (e = dalloc(&jpb, n)) || (e = dalloc(&jvs, n)) ||
+                        (e = dalloc(&jvx, n)) || (e = dalloc(&jom, n)) || (e = dalloc(&jlam, n)) ||
+                        (e = dalloc(&jo, n))))
+                return hip_fail(e, "pbf_adjoint_jvp");
+            const hipMemcpyKind kd = host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+            PBF_HIP(hipMemcpyAsync(x, pos4, n * sizeof(T4<T>), kd, st));
+            PBF_HIP(hipMemcpyAsync(v, vel4, n * sizeof(T4<T>), kd, st));
+            PBF_HIP(hipMemcpyAsync(xb, dvel4, n * sizeof(T4<T>), kd, st));
+            const unsigned nb = ablocks(n);
+            ka_to_dual<<<nb, AB, 0, st>>>((int32_t)n, x, nullptr, jx);
+            ka_to_dual<<<nb, AB, 0, st>>>((int32_t)n, v, xb, jv);
+            AConsts<Dual> d;
+            d.n = c.n;
+            d.gx = c.gx;
+            d.gy = c.gy;
+            d.gz = c.gz;
+            d.h = c.h;
+            d.hh = c.hh;
+            d.h6 = c.h6;
+            d.h9 = c.h9;
+            d.inv_rho0 = c.inv_rho0;
+            d.eps = c.eps;
+            d.gravity = c.gravity;
+            d.dt = c.dt;
+            d.k = c.k;
+            d.scale = c.scale;
+            d.xsph_c = Dual(c.xsph_c, dc);
+            d.vort_eps = Dual(c.vort_eps, de);
+            for (int a = 0; a < 3; a++) {
+                d.lo[a] = c.lo[a];
+                d.hi[a] = c.hi[a];
+                d.gdir[a] = c.gdir[a];
+            }
+            d.extforce = c.extforce;
+            d.vorticity = c.vorticity;
+            for (int32_t s = 0; s < steps; s++) {
+                const uint32_t *perm = all_perm + (size_t)s * n;
+                const int32_t *nbr9 = all_nbr + (size_t)s * 9 * n;
+                ka_predict<Dual><<<nb, AB, 0, st>>>(d, jx, jv, jxs, xs_f);
+                ka_gather<Dual><<<nb, AB, 0, st>>>((int32_t)n, perm, jxs, jpa);
+                T4<Dual> *cur = jpa, *nxt = jpb;
+                for (int32_t it = 0; it < iters; it++) {
+                    ka_lambda<Dual><<<nb, AB, 0, st>>>(d, cur, nbr9, jlam, nullptr, nullptr);
+                    ka_updatepos<Dual><<<nb, AB, 0, st>>>(d, cur, jlam, nbr9, nxt, nullptr, nullptr, nullptr);
+                    std::swap(cur, nxt);
+                }
+                ka_update<Dual><<<nb, AB, 0, st>>>(d, perm, cur, jx, jv, jvs);
+                if (d.vorticity) {
+                    ka_vort_a<Dual><<<nb, AB, 0, st>>>(d, cur, nbr9, jvs, jv, jvx, jom, jo);
+                    ka_vort_b<Dual><<<nb, AB, 0, st>>>(d, perm, cur, nbr9, jvx, jom, jo, jv, nullptr);
+                }
+            }
+            ka_from_dual<<<nb, AB, 0, st>>>((int32_t)n, jx, x, xb);
+            PBF_HIP(hipGetLastError());
+            const hipMemcpyKind ko = host ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+            if (pos_out4) PBF_HIP(hipMemcpyAsync(pos_out4, x, n * sizeof(T4<T>), ko, st));
+            PBF_HIP(hipMemcpyAsync(dpos_out4, xb, n * sizeof(T4<T>), ko, st));
+            if (host) PBF_HIP(hipStreamSynchronize(st));
+            return PBF_OK;
+        }
     }
 };
 
@@ -1128,7 +1334,19 @@ int pbf_adjoint_backward(pbf_adjoint *a, const void *grad_pos4, int host, void *
 
 int pbf_adjoint_replay(pbf_adjoint *a, const void *pos4, const void *vel4, int host, void *pos_out4) {
     if (!a || !pos4 || !vel4 || !pos_out4) return invalid("null argument");
-    return a->impl->replay(pos4, vel4, host, pos_out4);
+    return a->impl->replay(pos4, vel4, host, pos_out4, nullptr);
+}
+
+int pbf_adjoint_jvp(pbf_adjoint *a, const void *pos4, const void *vel4, const void *dvel4, double dxsph_c,
+                    double dvorticity_epsilon, int host, void *pos_out4, void *dpos_out4) {
+    if (!a || !pos4 || !vel4 || !dvel4 || !dpos_out4) return invalid("null argument");
+    return a->impl->jvp(pos4, vel4, dvel4, dxsph_c, dvorticity_epsilon, host, pos_out4, dpos_out4);
+}
+
+int pbf_adjoint_replay_frozen(pbf_adjoint *a, const void *pos4, const void *vel4, int host, void *pos_out4,
+                              uint64_t *clamp_flips) {
+    if (!a || !pos4 || !vel4 || !pos_out4 || !clamp_flips) return invalid("null argument");
+    return a->impl->replay(pos4, vel4, host, pos_out4, clamp_flips);
 }
 
 int pbf_adjoint_set_stream(pbf_adjoint *a, void *stream) {

@@ -124,17 +124,59 @@ def test_adjoint_matches_finite_differences_short(gpu_lib):
     assert abs(ge - fe) <= 1e-5 * abs(fe)
 
 
+def test_adjoint_matches_forward_mode_32_steps(gpu_lib):
+    """The workload (32 steps x 3 iterations, checkpoints every 8 steps):
+    the hand-written reverse pass against forward-mode differentiation of the
+    same FP64 map (pbf_adjoint_jvp: the forward kernels on dual numbers, no
+    finite step), on 64 random velocity components and both coefficients.
+    The two share only the forward formulas, which the FP64-replay test pins
+    to the oracle; they agree to rounding (about 1e-13 relative)."""
+    sc = adjoint_scene()
+    n = sc.n
+    r = A.Rollout.for_scene(sc, steps=32, checkpoint_every=8, fp64=True)
+    t = r.forward(sc.pos, adjoint_scene(seed=40).vel)
+    x = r.forward(sc.pos, sc.vel)
+    g = mse_grad(x, t)
+    gv, gc, ge = r.backward(g)
+    v64 = sc.vel.astype(np.float64)
+    xr = r.replay(sc.pos, v64)
+    xf, flips = r.replay_frozen(sc.pos, v64)
+    assert flips == 0 and np.array_equal(xf, xr), "a replay of the forward's own inputs takes its clamp decisions"
+    comps = np.random.default_rng(0).choice(n * 3, 64, replace=False)
+    jv = np.empty(64)
+    for k, cidx in enumerate(comps):
+        i, a = divmod(int(cidx), 3)
+        dv = np.zeros_like(v64)
+        dv[i, a] = 1.0
+        xj, dx = r.jvp(sc.pos, v64, dv)
+        if k == 0:
+            assert np.array_equal(xj, xr), "dual values differ from the FP64 replay"
+        jv[k] = float(np.sum(g[:, :3] * dx[:, :3]))
+    zero = np.zeros_like(v64)
+    jc = float(np.sum(g[:, :3] * r.jvp(sc.pos, v64, zero, 1.0, 0.0)[1][:, :3]))
+    je = float(np.sum(g[:, :3] * r.jvp(sc.pos, v64, zero, 0.0, 1.0)[1][:, :3]))
+    r.close()
+    ad = np.array([gv[int(c) // 3, int(c) % 3] for c in comps])
+    err = np.abs(ad - jv) / np.abs(jv)
+    say(f"[adjoint vs forward mode, 32 steps] components max rel {err.max():.2e} median {np.median(err):.2e}; "
+        f"c {gc:.12e} vs {jc:.12e}; eps {ge:.12e} vs {je:.12e}")
+    assert err.max() <= 1e-8
+    assert abs(gc - jc) <= 1e-8 * abs(jc)
+    assert abs(ge - je) <= 1e-8 * abs(je)
+
+
 def test_adjoint_matches_finite_differences_32_steps(gpu_lib):
-    """The workload: 32 steps x 3 iterations, checkpoints every 8 steps, 64
-    random velocity components, central differences with step 1e-4. Over 32
-    steps the central differences themselves move by 1e-5 to 1e-3 between steps
-    1e-4 and 1e-5 (the map amplifies perturbations), so agreement is asserted
-    for the median and for most components. The loss is far more nonlinear in
-    the two coefficients: their central differences are meaningless at 1e-4
-    (c: 3.2e-3 against an adjoint of 329) and settle only below a step of 1e-8,
-    where they still scatter by about 15 % from step to step (c: 377 at 1e-8,
-    360 at 1e-9, 332 at 1e-10); the adjoint is asserted within that scatter.
-    The coefficient derivatives are pinned to 1e-5 by the 4-step check."""
+    """Finite differences of the 32-step loss (central, step 1e-4, the loss
+    difference summed term by term) against the adjoint, as a check on the map
+    itself. Over 32 steps the map is smooth piecewise but its derivative varies
+    on small scales: central differences move by 1e-5 to 1e-3 between steps of
+    1e-4 and 1e-5, and get worse, not better, at smaller steps, also when every
+    wall-clamp decision is frozen to the forward's (pbf_adjoint_replay_frozen),
+    so the roughness comes from the dynamics, not from clamp events. In the two
+    coefficients central differences only settle below steps of 1e-8 (c: 377 at
+    1e-8, 360 at 1e-9, 332 at 1e-10 against 329). The exact derivative check at
+    this horizon is the forward-mode test above; here the median and most
+    components are asserted at 1e-4, and every component at the 4-step horizon."""
     sc = adjoint_scene()
     ad, fd, (gc, fc), (ge, fe) = fd_check(sc, steps=32, every=8, ncomp=64, delta=1e-4, delta_coef=1e-9)
     floor = 1e-3 * np.abs(fd).max()
