@@ -833,6 +833,8 @@ struct Adj : AdjBase {
     T *ob = nullptr, *lb = nullptr;
     double *acc = nullptr;
     bool forwarded = false;
+    // pair lists of the plain step's kernels (FP32 handles)
+    uint16_t *list_entries = nullptr, *list_count = nullptr;
     // dual-number state of pbf_adjoint_jvp (FP64 handles, allocated on first use)
     T4<Dual> *jx = nullptr, *jv = nullptr, *jxs = nullptr, *jpa = nullptr, *jpb = nullptr, *jvs = nullptr,
              *jvx = nullptr, *jom = nullptr;
@@ -886,6 +888,8 @@ struct Adj : AdjBase {
         dfree(ob);
         dfree(lb);
         dfree(acc);
+        dfree(list_entries);
+        dfree(list_count);
         dfree(jx);
         dfree(jv);
         dfree(jxs);
@@ -944,6 +948,9 @@ struct Adj : AdjBase {
             (e = dalloc(&pb_a, n)) || (e = dalloc(&pb_b, n)) || (e = dalloc(&vxb, n)) || (e = dalloc(&omb, n)) ||
             (e = dalloc(&vsb, n)) || (e = dalloc(&ob, n)) || (e = dalloc(&lb, n)) || (e = dalloc(&acc, 2)))
             return e;
+        if (std::is_same<T, float>::value &&
+            ((e = dalloc(&list_entries, (size_t)kListRows * n)) || (e = dalloc(&list_count, n))))
+            return e;
         return hipSuccess;
     }
 
@@ -973,6 +980,8 @@ struct Adj : AdjBase {
         }
         c.extforce = p.extforce;
         c.vorticity = p.vorticity;
+        fc.xsph_c = (float)xsph;  // the plain kernels' constants (FP32 forward)
+        fc.vort_eps = (float)veps;
     }
     void set_coefficients(double cc, double eps) override { set_consts(cc, eps); }
     // work queued on the old stream finishes first, so calls stay in order
@@ -1005,6 +1014,9 @@ struct Adj : AdjBase {
     // the recorded ones to impose (counting overridden decisions in flips).
     void step(StepTape<T> *tp, uint32_t *perm, int32_t *nbr9, bool compute_structure, uint8_t *clamp = nullptr,
               bool frozen = false) {
+        if constexpr (std::is_same<T, float>::value) {
+            if (!frozen) return step_plain(tp, perm, nbr9, compute_structure, clamp);
+        }
         const unsigned nb = ablocks(n);
         ka_predict<T><<<nb, AB, 0, st>>>(c, x, v, xs, xs_f);
         if (compute_structure) structure(perm, nbr9);
@@ -1031,6 +1043,54 @@ struct Adj : AdjBase {
             // a zero record's velocity is that of id 0 (imageLoad (velocitytexture, 0))
             ka_vort_a<T><<<nb, AB, 0, st>>>(c, cur, nbr9, vs, v, vx, om, o);
             ka_vort_b<T><<<nb, AB, 0, st>>>(c, perm, cur, nbr9, vx, om, o, v, tp ? tp->eta : nullptr);
+        }
+    }
+
+    // The FP32 step on the plain step's kernels (pbf_kernels.hip: candidate
+    // scans with pair lists, the closing update in the last position update),
+    // whose results equal the forward kernels' bit for bit; the tape values
+    // come out of them as extra outputs.
+    void step_plain(StepTape<T> *tp, uint32_t *perm, int32_t *nbr9, bool compute_structure, uint8_t *clamp) {
+        const unsigned nb = ablocks(n);
+        ka_predict<T><<<nb, AB, 0, st>>>(c, x, v, xs, xs_f);
+        if (compute_structure) structure(perm, nbr9);
+        float4 *p0 = reinterpret_cast<float4 *>(tp ? tp->p : pb_a);
+        launch_gather_records((int32_t)n, perm, xs_f, p0, st);
+        PairList pl;
+        pl.entries = list_entries;
+        pl.count = list_count;
+        pl.cap = kListCap;
+        float4 *vs = reinterpret_cast<float4 *>(tp ? tp->vs : vsb);
+        StepUpdate up;
+        up.pos = reinterpret_cast<float4 *>(x);
+        up.vel = reinterpret_cast<float4 *>(v);
+        up.vel_sorted = vs;
+        float4 *cur = p0;
+        for (int32_t it = 0; it < iters; it++) {
+            float4 *nxt = reinterpret_cast<float4 *>(tp ? tp->p + (size_t)(it + 1) * n
+                                                        : (cur == reinterpret_cast<float4 *>(pb_a) ? pb_b : pb_a));
+            float *lam = reinterpret_cast<float *>(tp ? tp->lam + (size_t)it * n : lam_tmp);
+            SolverTape tl, tu;
+            if (tp) {
+                tl.sden = reinterpret_cast<float *>(tp->sden + (size_t)it * n);
+                tl.G = reinterpret_cast<float4 *>(tp->G + (size_t)it * n);
+                tu.clamp = tp->mask + (size_t)it * n;
+            } else if (clamp) {
+                tu.clamp = clamp + (size_t)it * n;
+            }
+            launch_lambda(fc, cur, nbr9, lam, pl, st, false, nullptr, &tl);
+            launch_updatepos(fc, cur, lam, nbr9, pl, nxt, st, false, nullptr, it + 1 == iters ? &up : nullptr, &tu);
+            cur = nxt;
+        }
+        if (iters == 0) launch_update(fc, cur, up.pos, up.vel, vs, st);
+        if (c.vorticity) {
+            float4 *vx = reinterpret_cast<float4 *>(tp ? tp->vx : vxb);
+            float4 *om = reinterpret_cast<float4 *>(tp ? tp->om : omb);
+            float *o = reinterpret_cast<float *>(tp ? tp->o : ob);
+            SolverTape te;
+            if (tp) te.eta = reinterpret_cast<float4 *>(tp->eta);
+            launch_vorticity_a(fc, cur, nbr9, vs, up.vel, vx, om, o, pl, st, false);
+            launch_vorticity_b(fc, cur, nbr9, pl, vx, om, o, up.vel, st, false, nullptr, &te);
         }
     }
 

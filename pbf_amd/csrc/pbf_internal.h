@@ -146,8 +146,17 @@ void launch_dense_select(const DenseList &dl, hipStream_t st);
 // dl: the step's dense list, whose particles the launch's first workgroups
 // handle, kDenseGroup lanes per particle. Null: every particle takes the
 // one-thread-per-particle path.
+// Values of a step that the adjoint's reverse pass reads, written as extra
+// outputs of the solver passes when given (all null in the plain step).
+struct SolverTape {
+    float *sden = nullptr;     // lambda: sum |g_j|^2 + |sum g_j|^2 + eps, sorted slot
+    float4 *G = nullptr;       // lambda: sum g_j (g_j = gradW / rho0)
+    uint8_t *clamp = nullptr;  // position update: bit a set where axis a was not clamped, bit 3 + a at the upper wall
+    float4 *eta = nullptr;     // vorticity B: eta before normalisation
+};
 void launch_lambda(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, float *lambda, const PairList &pl,
-                   hipStream_t st, bool padded = false, const DenseList *dl = nullptr);
+                   hipStream_t st, bool padded = false, const DenseList *dl = nullptr,
+                   const SolverTape *tape = nullptr);
 // The step's closing update (update.glsl:16-28: velocities from the
 // positions before and after the step, both stored by particle id; vel_sorted,
 // if set, also keeps the velocities in sorted order). Given to the last
@@ -160,7 +169,8 @@ struct StepUpdate {
 };
 void launch_updatepos(const SolverConsts &c, const float4 *rec_in, const float *lambda, const int32_t *nbr9,
                       const PairList &pl, float4 *rec_out, hipStream_t st, bool padded = false,
-                      const DenseList *dl = nullptr, const StepUpdate *up = nullptr);
+                      const DenseList *dl = nullptr, const StepUpdate *up = nullptr,
+                      const SolverTape *tape = nullptr);
 void launch_update(const SolverConsts &c, const float4 *rec, float4 *pos, float4 *vel, float4 *vel_sorted,
                    hipStream_t st);
 void launch_vorticity_a(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, const float4 *vel_sorted,
@@ -168,7 +178,7 @@ void launch_vorticity_a(const SolverConsts &c, const float4 *rec, const int32_t 
                         hipStream_t st, bool padded = false, const DenseList *dl = nullptr);
 void launch_vorticity_b(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, const PairList &pl,
                         const float4 *vxsph, const float4 *omega, const float *vort, float4 *vel_id, hipStream_t st,
-                        bool padded = false, const DenseList *dl = nullptr);
+                        bool padded = false, const DenseList *dl = nullptr, const SolverTape *tape = nullptr);
 // Launch shape of the batched form: workgroup size of the neighbour passes
 // (128, 256, 512), workgroup size (64 .. 1024, a power of two) and particles
 // per thread (1 .. 4) of the streaming passes. The one-scene form uses the

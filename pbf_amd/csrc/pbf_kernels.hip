@@ -63,6 +63,12 @@ __device__ __forceinline__ float clamp_glsl(float x, float lo, float hi) {
     return m > hi ? hi : m;
 }
 
+// Clamp state of one axis for the adjoint's tape: 1 inside the walls, 8 at
+// the upper wall, 0 at the lower one (shifted by the axis).
+__device__ __forceinline__ uint32_t clamp_state(float x, float lo, float hi, int axis) {
+    return ((x >= lo) & (x <= hi) ? 1u : (x > hi ? 8u : 0u)) << axis;
+}
+
 // GetHash, counting.glsl:53-57: FP32 dot of the clamped integer cell.
 __device__ __forceinline__ uint32_t cell_hash(const Seg &s, float x, float y, float z) {
     int32_t ix = (int32_t)clamp_glsl(x, 0.0f, (float)s.gx);
@@ -896,6 +902,7 @@ struct LambdaPass {
     const SolverConsts &c;
     const float4 *__restrict__ rec;
     float *__restrict__ lambda;
+    SolverTape tp;
     float4 pi;
     __device__ __forceinline__ void begin(int32_t i, const Seg &) { pi = rec[i]; }
     __device__ __forceinline__ int extra(const Seg &, int32_t) const { return 0; }
@@ -922,6 +929,10 @@ struct LambdaPass {
         const float sk = a[1] + sq_len(a[2], a[3], a[4]);
         const float ci = a[0] * c.inv_rho0 - 1.0f;
         lambda[i] = -ci / (sk + c.eps);
+        if (tp.sden) {
+            tp.sden[i] = sk + c.eps;
+            tp.G[i] = make_float4(a[2], a[3], a[4], 0.0f);
+        }
     }
 };
 
@@ -946,6 +957,7 @@ struct UpdatePosPass {
     const float *__restrict__ lambda;
     float4 *__restrict__ rec_out;
     StepUpdate up;  // the step's closing update (last iteration), or none
+    SolverTape tp;
     float4 pi;
     float li;
     __device__ __forceinline__ void begin(int32_t i, const Seg &) {
@@ -973,6 +985,9 @@ struct UpdatePosPass {
     }
     __device__ __forceinline__ void finish(int32_t i, const Seg &sg, const float *a) const {
         float x = pi.x + c.inv_rho0 * a[0], y = pi.y + c.inv_rho0 * a[1], z = pi.z + c.inv_rho0 * a[2];
+        if (tp.clamp)
+            tp.clamp[i] = (uint8_t)(clamp_state(x, sg.lo[0], sg.hi[0], 0) | clamp_state(y, sg.lo[1], sg.hi[1], 1) |
+                                    clamp_state(z, sg.lo[2], sg.hi[2], 2));
         x = clamp_glsl(x, sg.lo[0], sg.hi[0]);
         y = clamp_glsl(y, sg.lo[1], sg.hi[1]);
         z = clamp_glsl(z, sg.lo[2], sg.hi[2]);
@@ -1042,6 +1057,7 @@ struct VorticityBPass {
     const float4 *__restrict__ omega;
     const float *__restrict__ vort;
     float4 *__restrict__ vel_id;
+    SolverTape tp;
     float4 pi;
     __device__ __forceinline__ void begin(int32_t i, const Seg &) { pi = rec[i]; }
     __device__ __forceinline__ float extra(const Seg &sg, int32_t j) const { return val_at<PAD>(vort, sg, j); }
@@ -1060,6 +1076,7 @@ struct VorticityBPass {
     }
     __device__ __forceinline__ void finish(int32_t i, const Seg &sg, const float *a) const {
         float nx = a[0], ny = a[1], nz = a[2];
+        if (tp.eta) tp.eta[i] = make_float4(nx, ny, nz, 0.0f);
         const float l = sqrtf(sq_len(nx, ny, nz));
         if (l > 0.0f) {
             nx = nx / l;
@@ -1088,14 +1105,14 @@ __global__ __launch_bounds__(NB) void k_lambda(SolverConsts c, Dom dom, const fl
                                                      const int32_t *__restrict__ nbr9, float *__restrict__ lambda,
                                                      uint16_t *__restrict__ list_entries,
                                                      uint16_t *__restrict__ list_count, int32_t list_cap,
-                                                     DenseList dl, uint32_t dense_blocks) {
+                                                     DenseList dl, uint32_t dense_blocks, SolverTape tp) {
     __shared__ union {
         NbrLds<NB> light;
         DenseLds<5, 0, NB / 64> dense;
     } u;
     {
         if (blockIdx.x < dense_blocks) {
-            LambdaPass<H1, PAD> ps{c, rec, lambda, {}};
+            LambdaPass<H1, PAD> ps{c, rec, lambda, tp, {}};
             dense_pass<LambdaPass<H1, PAD>, PAD, NB / 64>(c, dom, rec, nbr9, dl, list_entries, list_count, list_cap, ps,
                                                  u.dense, blockIdx.x, dense_blocks);
             return;
@@ -1133,6 +1150,10 @@ __global__ __launch_bounds__(NB) void k_lambda(SolverConsts c, Dom dom, const fl
     sk += sq_len(gix, giy, giz);
     const float ci = rho * c.inv_rho0 - 1.0f;
     lambda[i] = -ci / (sk + c.eps);
+    if (tp.sden) {
+        tp.sden[i] = sk + c.eps;
+        tp.G[i] = make_float4(gix, giy, giz, 0.0f);
+    }
 }
 
 // updatepos.glsl:42-105, race-free: neighbours read from rec_in, result to
@@ -1144,14 +1165,14 @@ __global__ __launch_bounds__(NB) void k_updatepos(SolverConsts c, Dom dom, const
                                                         const uint16_t *__restrict__ list_entries,
                                                         const uint16_t *__restrict__ list_count,
                                                         float4 *__restrict__ rec_out, DenseList dl,
-                                                        uint32_t dense_blocks, StepUpdate up) {
+                                                        uint32_t dense_blocks, StepUpdate up, SolverTape tp) {
     __shared__ union {
         NbrLds<NB, kReplayLcap> light;
         DenseLds<3, 0, NB / 64> dense;
     } u;
     {
         if (blockIdx.x < dense_blocks) {
-            UpdatePosPass<H1, PAD> ps{c, rec_in, lambda, rec_out, up, {}, 0.0f};
+            UpdatePosPass<H1, PAD> ps{c, rec_in, lambda, rec_out, up, tp, {}, 0.0f};
             dense_pass<UpdatePosPass<H1, PAD>, PAD, NB / 64>(c, dom, rec_in, nbr9, dl, const_cast<uint16_t *>(list_entries),
                                                     const_cast<uint16_t *>(list_count), 0, ps, u.dense, blockIdx.x,
                                                     dense_blocks);
@@ -1186,6 +1207,9 @@ __global__ __launch_bounds__(NB) void k_updatepos(SolverConsts c, Dom dom, const
         dpz += f * gz;
     });
     float x = pi.x + c.inv_rho0 * dpx, y = pi.y + c.inv_rho0 * dpy, z = pi.z + c.inv_rho0 * dpz;
+    if (tp.clamp)
+        tp.clamp[i] = (uint8_t)(clamp_state(x, sg.lo[0], sg.hi[0], 0) | clamp_state(y, sg.lo[1], sg.hi[1], 1) |
+                                clamp_state(z, sg.lo[2], sg.hi[2], 2));
     x = clamp_glsl(x, sg.lo[0], sg.hi[0]);
     y = clamp_glsl(y, sg.lo[1], sg.hi[1]);
     z = clamp_glsl(z, sg.lo[2], sg.hi[2]);
@@ -1276,14 +1300,14 @@ __global__ __launch_bounds__(NB) void k_vorticity_b(SolverConsts c, Dom dom, con
                                                           const float4 *__restrict__ omega,
                                                           const float *__restrict__ vort,
                                                           float4 *__restrict__ vel_id, DenseList dl,
-                                                          uint32_t dense_blocks) {
+                                                          uint32_t dense_blocks, SolverTape tp) {
     __shared__ union {
         NbrLds<NB, kReplayLcap> light;
         DenseLds<3, 0, NB / 64> dense;
     } u;
     {
         if (blockIdx.x < dense_blocks) {
-            VorticityBPass<H1, PAD> ps{c, rec, vxsph, omega, vort, vel_id, {}};
+            VorticityBPass<H1, PAD> ps{c, rec, vxsph, omega, vort, vel_id, tp, {}};
             dense_pass<VorticityBPass<H1, PAD>, PAD, NB / 64>(c, dom, rec, nbr9, dl, const_cast<uint16_t *>(list_entries),
                                                      const_cast<uint16_t *>(list_count), 0, ps, u.dense, blockIdx.x,
                                                      dense_blocks);
@@ -1311,6 +1335,7 @@ __global__ __launch_bounds__(NB) void k_vorticity_b(SolverConsts c, Dom dom, con
         ny += om * gy;
         nz += om * gz;
     });
+    if (tp.eta) tp.eta[i] = make_float4(nx, ny, nz, 0.0f);
     const float l = sqrtf(sq_len(nx, ny, nz));
     if (l > 0.0f) {
         nx = nx / l;
@@ -1478,9 +1503,10 @@ void update_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, con
 template <class Dom>
 void lambda_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, bool pad, const float4 *rec,
                  const int32_t *nbr9, float *lambda, const PairList &pl, hipStream_t st,
-                 const DenseList *dl = nullptr) {
+                 const DenseList *dl = nullptr, const SolverTape *tape = nullptr) {
     if (c.n <= 0) return;
     const DenseList dlv = dl ? *dl : DenseList();
+    const SolverTape tp = tape ? *tape : SolverTape();
     with_nb<Dom>(ls.nb, pad, [&](auto nbc, auto padc) {
         constexpr int NB = decltype(nbc)::value;
         constexpr bool PAD = decltype(padc)::value;
@@ -1488,19 +1514,21 @@ void lambda_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, boo
         const unsigned grid = nb_blocks(c.n, NB) + db;
         if (c.h_is_one)
             k_lambda<true, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec, nbr9, lambda, pl.entries, pl.count, pl.cap,
-                                                              dlv, db);
+                                                              dlv, db, tp);
         else
             k_lambda<false, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec, nbr9, lambda, pl.entries, pl.count, pl.cap,
-                                                               dlv, db);
+                                                               dlv, db, tp);
     });
 }
 template <class Dom>
 void updatepos_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, bool pad, const float4 *rec_in,
                     const float *lambda, const int32_t *nbr9, const PairList &pl, float4 *rec_out, hipStream_t st,
-                    const DenseList *dl = nullptr, const StepUpdate *up = nullptr) {
+                    const DenseList *dl = nullptr, const StepUpdate *up = nullptr,
+                    const SolverTape *tape = nullptr) {
     if (c.n <= 0) return;
     const DenseList dlv = dl ? *dl : DenseList();
     const StepUpdate upv = up ? *up : StepUpdate();
+    const SolverTape tp = tape ? *tape : SolverTape();
     with_nb<Dom, true>(ls.nb, pad, [&](auto nbc, auto padc) {
         constexpr int NB = decltype(nbc)::value;
         constexpr bool PAD = decltype(padc)::value;
@@ -1508,10 +1536,10 @@ void updatepos_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, 
         const unsigned grid = nb_blocks(c.n, NB) + db;
         if (c.h_is_one)
             k_updatepos<true, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec_in, lambda, nbr9, pl.entries, pl.count,
-                                                                 rec_out, dlv, db, upv);
+                                                                 rec_out, dlv, db, upv, tp);
         else
             k_updatepos<false, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec_in, lambda, nbr9, pl.entries, pl.count,
-                                                                  rec_out, dlv, db, upv);
+                                                                  rec_out, dlv, db, upv, tp);
     });
 }
 template <class Dom>
@@ -1536,9 +1564,11 @@ void vorticity_a_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls
 template <class Dom>
 void vorticity_b_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, bool pad, const float4 *rec,
                       const int32_t *nbr9, const PairList &pl, const float4 *vxsph, const float4 *omega,
-                      const float *vort, float4 *vel_id, hipStream_t st, const DenseList *dl = nullptr) {
+                      const float *vort, float4 *vel_id, hipStream_t st, const DenseList *dl = nullptr,
+                      const SolverTape *tape = nullptr) {
     if (c.n <= 0) return;
     const DenseList dlv = dl ? *dl : DenseList();
+    const SolverTape tp = tape ? *tape : SolverTape();
     with_nb<Dom, true>(ls.nb, pad, [&](auto nbc, auto padc) {
         constexpr int NB = decltype(nbc)::value;
         constexpr bool PAD = decltype(padc)::value;
@@ -1546,10 +1576,10 @@ void vorticity_b_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls
         const unsigned grid = nb_blocks(c.n, NB) + db;
         if (c.h_is_one)
             k_vorticity_b<true, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec, nbr9, pl.entries, pl.count, vxsph, omega,
-                                                                   vort, vel_id, dlv, db);
+                                                                   vort, vel_id, dlv, db, tp);
         else
             k_vorticity_b<false, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec, nbr9, pl.entries, pl.count, vxsph,
-                                                                    omega, vort, vel_id, dlv, db);
+                                                                    omega, vort, vel_id, dlv, db, tp);
     });
 }
 }  // namespace
@@ -1587,13 +1617,13 @@ void launch_neighbourcells(const SolverConsts &c, const float4 *sorted, const in
     neighbourcells_impl(c, one(c), kDefaultShape, sorted, start, end64, nbr9, st, dl, highlight);
 }
 void launch_lambda(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, float *lambda, const PairList &pl,
-                   hipStream_t st, bool padded, const DenseList *dl) {
-    lambda_impl(c, one(c), kDefaultShape, padded, rec, nbr9, lambda, pl, st, dl);
+                   hipStream_t st, bool padded, const DenseList *dl, const SolverTape *tape) {
+    lambda_impl(c, one(c), kDefaultShape, padded, rec, nbr9, lambda, pl, st, dl, tape);
 }
 void launch_updatepos(const SolverConsts &c, const float4 *rec_in, const float *lambda, const int32_t *nbr9,
                       const PairList &pl, float4 *rec_out, hipStream_t st, bool padded, const DenseList *dl,
-                      const StepUpdate *up) {
-    updatepos_impl(c, one(c), kDefaultShape, padded, rec_in, lambda, nbr9, pl, rec_out, st, dl, up);
+                      const StepUpdate *up, const SolverTape *tape) {
+    updatepos_impl(c, one(c), kDefaultShape, padded, rec_in, lambda, nbr9, pl, rec_out, st, dl, up, tape);
 }
 void launch_update(const SolverConsts &c, const float4 *rec, float4 *pos, float4 *vel, float4 *vel_sorted,
                    hipStream_t st) {
@@ -1606,8 +1636,8 @@ void launch_vorticity_a(const SolverConsts &c, const float4 *rec, const int32_t 
 }
 void launch_vorticity_b(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, const PairList &pl,
                         const float4 *vxsph, const float4 *omega, const float *vort, float4 *vel_id, hipStream_t st,
-                        bool padded, const DenseList *dl) {
-    vorticity_b_impl(c, one(c), kDefaultShape, padded, rec, nbr9, pl, vxsph, omega, vort, vel_id, st, dl);
+                        bool padded, const DenseList *dl, const SolverTape *tape) {
+    vorticity_b_impl(c, one(c), kDefaultShape, padded, rec, nbr9, pl, vxsph, omega, vort, vel_id, st, dl, tape);
 }
 
 // many scenes
