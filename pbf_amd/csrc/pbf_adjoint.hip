@@ -238,6 +238,26 @@ __device__ __forceinline__ void row_candidates(const int32_t *__restrict__ nbr9,
     }
 }
 
+// The pair terms of particle i handled by lane plane k (0..8) of a reverse
+// kernel: with a recorded pair list (the neighbours within h of the forward,
+// in list order), entries k, k + 9, ... of it; without one (none recorded, or
+// the particle had more than the list holds), the candidates of row k.
+template <class T, class F>
+__device__ __forceinline__ void plane_pairs(const uint16_t *__restrict__ ents, const uint16_t *__restrict__ cnts,
+                                            const int32_t *__restrict__ nbr9, int32_t n, int32_t i, int32_t k,
+                                            const T4<T> *__restrict__ p, F &&body) {
+    const uint32_t cnt = cnts ? cnts[i] : kListNone;
+    if (cnt == kListNone) {
+        row_candidates(nbr9, n, i, k, p, body);
+        return;
+    }
+    for (uint32_t e = (uint32_t)k; e < cnt; e += 9) {
+        const uint32_t code = ents[(size_t)e * n + i];
+        const int32_t j = (nbr9[(size_t)(code >> 8) * n + i] & 0xFFFFFF) + (int32_t)(code & 255u);
+        body(j, load_nb(p, j, n));
+    }
+}
+
 __device__ __forceinline__ void block_add2(double a, double b, double *acc) {
     __shared__ double s[2][AB];
     s[0][threadIdx.x] = a;
@@ -468,6 +488,7 @@ __global__ __launch_bounds__(AB) void ka_vort_b(AConsts<T> c, const uint32_t *__
 template <class T>
 __global__ __launch_bounds__(AB) void kr_vort_b(AConsts<T> c, const uint32_t *__restrict__ perm,
                                                 const T4<T> *__restrict__ p, const int32_t *__restrict__ nbr9,
+                                                const uint16_t *__restrict__ ents, const uint16_t *__restrict__ cnts,
                                                 const T4<T> *__restrict__ om, const T *__restrict__ o,
                                                 const T4<T> *__restrict__ eta, const T4<T> *__restrict__ vbar,
                                                 T4<T> *__restrict__ vxbar, T4<T> *__restrict__ ombar,
@@ -504,7 +525,7 @@ __global__ __launch_bounds__(AB) void kr_vort_b(AConsts<T> c, const uint32_t *__
             const T nn = Nx * Nbx + Ny * Nby + Nz * Nbz;
             const T ebx = (Nbx - Nx * nn) / l, eby = (Nby - Ny * nn) / l, ebz = (Nbz - Nz * nn) / l;
             T pbx = 0, pby = 0, pbz = 0;
-            row_candidates(nbr9, c.n, i, row, p, [&](int32_t j, const T4<T> &pj) {
+            plane_pairs(ents, cnts, nbr9, c.n, i, row, p, [&](int32_t j, const T4<T> &pj) {
                 const T ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
                 const T sq = sqlen(ex, ey, ez);
                 const T r = sqrt(sq);
@@ -547,7 +568,8 @@ __global__ __launch_bounds__(AB) void kr_omega_norm(int32_t n, const T4<T> *__re
 // scattered), and d/d xsph_c.
 template <class T>
 __global__ __launch_bounds__(AB) void kr_vort_a(AConsts<T> c, const T4<T> *__restrict__ p,
-                                                const int32_t *__restrict__ nbr9, const T4<T> *__restrict__ vs,
+                                                const int32_t *__restrict__ nbr9, const uint16_t *__restrict__ ents,
+                                                const uint16_t *__restrict__ cnts, const T4<T> *__restrict__ vs,
                                                 const T4<T> *__restrict__ vxbar, const T4<T> *__restrict__ ombar,
                                                 T4<T> *__restrict__ vsbar, T4<T> *__restrict__ pbar,
                                                 double *__restrict__ acc) {
@@ -559,7 +581,7 @@ __global__ __launch_bounds__(AB) void kr_vort_a(AConsts<T> c, const T4<T> *__res
         T sx = 0, sy = 0, sz = 0;
         T vbx = row == 0 ? a.x : (T)0, vby = row == 0 ? a.y : (T)0, vbz = row == 0 ? a.z : (T)0;  // vx = v + ...
         T pbx = 0, pby = 0, pbz = 0;
-        row_candidates(nbr9, c.n, i, row, p, [&](int32_t j, const T4<T> &pj) {
+        plane_pairs(ents, cnts, nbr9, c.n, i, row, p, [&](int32_t j, const T4<T> &pj) {
             const T ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
             const T sq = sqlen(ex, ey, ez);
             const T r = sqrt(sq);
@@ -631,6 +653,8 @@ __global__ __launch_bounds__(AB) void kr_update(AConsts<T> c, const uint32_t *__
 template <class T>
 __global__ __launch_bounds__(AB) void kr_updatepos(AConsts<T> c, const T4<T> *__restrict__ p,
                                                    const T *__restrict__ lam, const int32_t *__restrict__ nbr9,
+                                                   const uint16_t *__restrict__ ents,
+                                                   const uint16_t *__restrict__ cnts,
                                                    const uint8_t *__restrict__ mask, const T4<T> *__restrict__ pbn,
                                                    T4<T> *__restrict__ pbp, T *__restrict__ lambar) {
     const int64_t tid = (int64_t)blockIdx.x * AB + threadIdx.x;
@@ -644,7 +668,7 @@ __global__ __launch_bounds__(AB) void kr_updatepos(AConsts<T> c, const T4<T> *__
     const T li = lam[i];
     T lb = 0;
     const T k4 = (T)-4 * c.k * c.scale * c.scale * c.scale * c.scale;
-    row_candidates(nbr9, c.n, i, row, p, [&](int32_t j, const T4<T> &pj) {
+    plane_pairs(ents, cnts, nbr9, c.n, i, row, p, [&](int32_t j, const T4<T> &pj) {
         const T ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
         const T sq = sqlen(ex, ey, ez);
         const T r = sqrt(sq);
@@ -679,7 +703,8 @@ __global__ __launch_bounds__(AB) void kr_updatepos(AConsts<T> c, const T4<T> *__
 // lambda reversed: from lambar to pbar (scattered).
 template <class T>
 __global__ __launch_bounds__(AB) void kr_lambda(AConsts<T> c, const T4<T> *__restrict__ p,
-                                                const int32_t *__restrict__ nbr9, const T *__restrict__ lam,
+                                                const int32_t *__restrict__ nbr9, const uint16_t *__restrict__ ents,
+                                                const uint16_t *__restrict__ cnts, const T *__restrict__ lam,
                                                 const T *__restrict__ sden, const T4<T> *__restrict__ G,
                                                 const T *__restrict__ lambar, T4<T> *__restrict__ pbar) {
     const int64_t tid = (int64_t)blockIdx.x * AB + threadIdx.x;
@@ -694,7 +719,7 @@ __global__ __launch_bounds__(AB) void kr_lambda(AConsts<T> c, const T4<T> *__res
     const T sb = -lam[i] * lb / S;
     const T4<T> Gi = G[i];
     T pbx = 0, pby = 0, pbz = 0;
-    row_candidates(nbr9, c.n, i, row, p, [&](int32_t j, const T4<T> &pj) {
+    plane_pairs(ents, cnts, nbr9, c.n, i, row, p, [&](int32_t j, const T4<T> &pj) {
         const T ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
         const T sq = sqlen(ex, ey, ez);
         const T r = sqrt(sq);
@@ -792,6 +817,10 @@ struct StepTape {
     uint8_t *mask = nullptr; // I x n
     T4<T> *vs = nullptr, *vx = nullptr, *om = nullptr, *eta = nullptr;
     T *o = nullptr;
+    // pair lists of the FP32 step ((I + 1) lists: the iterations', then vorticity's)
+    uint16_t *ents = nullptr, *cnts = nullptr;
+    const uint16_t *list_entries(int32_t k, uint32_t n) const { return ents ? ents + (size_t)k * kListRows * n : nullptr; }
+    const uint16_t *list_counts(int32_t k, uint32_t n) const { return cnts ? cnts + (size_t)k * n : nullptr; }
 };
 
 template <class T>
@@ -873,6 +902,8 @@ struct Adj : AdjBase {
             dfree(t.om);
             dfree(t.o);
             dfree(t.eta);
+            dfree(t.ents);
+            dfree(t.cnts);
         }
         dfree(xs);
         dfree(lam_tmp);
@@ -941,6 +972,10 @@ struct Adj : AdjBase {
                 (e = dalloc(&t.sden, (size_t)iters * n)) || (e = dalloc(&t.G, (size_t)iters * n)) ||
                 (e = dalloc(&t.mask, (size_t)iters * n)) || (e = dalloc(&t.vs, n)) || (e = dalloc(&t.vx, n)) ||
                 (e = dalloc(&t.om, n)) || (e = dalloc(&t.o, n)) || (e = dalloc(&t.eta, n)))
+                return e;
+            if (std::is_same<T, float>::value &&
+                ((e = dalloc(&t.ents, (size_t)(iters + 1) * kListRows * n)) ||
+                 (e = dalloc(&t.cnts, (size_t)(iters + 1) * n))))
                 return e;
         }
         if ((e = dalloc(&xs, n)) || (e = dalloc(&lam_tmp, n))) return e;
@@ -1078,6 +1113,10 @@ struct Adj : AdjBase {
             } else if (clamp) {
                 tu.clamp = clamp + (size_t)it * n;
             }
+            if (tp) {  // the reverse pass reads each iteration's list
+                pl.entries = tp->ents + (size_t)it * kListRows * n;
+                pl.count = tp->cnts + (size_t)it * n;
+            }
             launch_lambda(fc, cur, nbr9, lam, pl, st, false, nullptr, &tl);
             launch_updatepos(fc, cur, lam, nbr9, pl, nxt, st, false, nullptr, it + 1 == iters ? &up : nullptr, &tu);
             cur = nxt;
@@ -1088,7 +1127,11 @@ struct Adj : AdjBase {
             float4 *om = reinterpret_cast<float4 *>(tp ? tp->om : omb);
             float *o = reinterpret_cast<float *>(tp ? tp->o : ob);
             SolverTape te;
-            if (tp) te.eta = reinterpret_cast<float4 *>(tp->eta);
+            if (tp) {
+                te.eta = reinterpret_cast<float4 *>(tp->eta);
+                pl.entries = tp->ents + (size_t)iters * kListRows * n;
+                pl.count = tp->cnts + (size_t)iters * n;
+            }
             launch_vorticity_a(fc, cur, nbr9, vs, up.vel, vx, om, o, pl, st, false);
             launch_vorticity_b(fc, cur, nbr9, pl, vx, om, o, up.vel, st, false, nullptr, &te);
         }
@@ -1160,9 +1203,11 @@ struct Adj : AdjBase {
         if (c.vorticity) {
             zero(ob, n);
             zero(reinterpret_cast<T *>(vsb), 4 * (size_t)n);
-            kr_vort_b<T><<<nr, AB, 0, st>>>(c, t.perm, pI, t.nbr9, t.om, t.o, t.eta, vb, vxb, omb, ob, pb_a, acc);
+            kr_vort_b<T><<<nr, AB, 0, st>>>(c, t.perm, pI, t.nbr9, t.list_entries(iters, n), t.list_counts(iters, n),
+                                            t.om, t.o, t.eta, vb, vxb, omb, ob, pb_a, acc);
             kr_omega_norm<T><<<nb, AB, 0, st>>>((int32_t)n, t.om, t.o, ob, omb);
-            kr_vort_a<T><<<nr, AB, 0, st>>>(c, pI, t.nbr9, t.vs, vxb, omb, vsb, pb_a, acc);
+            kr_vort_a<T><<<nr, AB, 0, st>>>(c, pI, t.nbr9, t.list_entries(iters, n), t.list_counts(iters, n), t.vs,
+                                            vxb, omb, vsb, pb_a, acc);
             vsbar = vsb;
         }
         kr_update<T><<<nb, AB, 0, st>>>(c, t.perm, xb, vb, vsbar, pb_a, xb2);
@@ -1172,9 +1217,10 @@ struct Adj : AdjBase {
             const T *lam = t.lam + (size_t)it * n;
             zero(reinterpret_cast<T *>(prev), 4 * (size_t)n);
             zero(lb, n);
-            kr_updatepos<T><<<nr, AB, 0, st>>>(c, p, lam, t.nbr9, t.mask + (size_t)it * n, next, prev, lb);
-            kr_lambda<T><<<nr, AB, 0, st>>>(c, p, t.nbr9, lam, t.sden + (size_t)it * n, t.G + (size_t)it * n, lb,
-                                            prev);
+            kr_updatepos<T><<<nr, AB, 0, st>>>(c, p, lam, t.nbr9, t.list_entries(it, n), t.list_counts(it, n),
+                                               t.mask + (size_t)it * n, next, prev, lb);
+            kr_lambda<T><<<nr, AB, 0, st>>>(c, p, t.nbr9, t.list_entries(it, n), t.list_counts(it, n), lam,
+                                            t.sden + (size_t)it * n, t.G + (size_t)it * n, lb, prev);
             std::swap(next, prev);
         }
         kr_predict<T><<<nb, AB, 0, st>>>(c, t.perm, next, xb2, vb2);

@@ -110,6 +110,8 @@ constexpr int32_t kListCap = 128;
 // Rows allocated for the lists: a replay requests the rows of its next batch
 // (up to 16 past the list's end) before it knows which of them are entries.
 constexpr int32_t kListRows = kListCap + 16;
+// List count of a particle with more than kListCap neighbours within h (no list).
+constexpr uint16_t kListNone = 0xFFFF;
 
 // Zero entries kept after the n particles of the one-scene solver's sorted
 // records, lambda and |omega| arrays: a neighbour row starts below n and holds

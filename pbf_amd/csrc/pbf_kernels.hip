@@ -592,7 +592,7 @@ __device__ __forceinline__ void scan_neighbours(const SolverConsts &c, const Seg
 // position update (and vorticity pass B), which reads the same positions,
 // replays it instead of scanning. count LIST_NONE: more than cap neighbours,
 // the replaying pass scans.
-constexpr uint16_t LIST_NONE = 0xFFFF;
+constexpr uint16_t LIST_NONE = kListNone;
 
 // The next entry's address is kept and advanced by n per entry (an add, not
 // a 64-bit multiply per entry).
