@@ -856,6 +856,7 @@ struct Adj : AdjBase {
     std::vector<StepTape<T>> tape;
     T4<T> *xs = nullptr;
     T *lam_tmp = nullptr;
+    uint2 *qpos = nullptr;  // quantized positions of the FP32 step's current records
     // reverse buffers
     T4<T> *xb = nullptr, *vb = nullptr, *xb2 = nullptr, *vb2 = nullptr, *pb_a = nullptr, *pb_b = nullptr;
     T4<T> *vxb = nullptr, *omb = nullptr, *vsb = nullptr;
@@ -907,6 +908,7 @@ struct Adj : AdjBase {
         }
         dfree(xs);
         dfree(lam_tmp);
+        dfree(qpos);
         dfree(xb);
         dfree(vb);
         dfree(xb2);
@@ -978,7 +980,7 @@ struct Adj : AdjBase {
                  (e = dalloc(&t.cnts, (size_t)(iters + 1) * n))))
                 return e;
         }
-        if ((e = dalloc(&xs, n)) || (e = dalloc(&lam_tmp, n))) return e;
+        if ((e = dalloc(&xs, n)) || (e = dalloc(&lam_tmp, n)) || (e = dalloc(&qpos, n))) return e;
         if ((e = dalloc(&xb, n)) || (e = dalloc(&vb, n)) || (e = dalloc(&xb2, n)) || (e = dalloc(&vb2, n)) ||
             (e = dalloc(&pb_a, n)) || (e = dalloc(&pb_b, n)) || (e = dalloc(&vxb, n)) || (e = dalloc(&omb, n)) ||
             (e = dalloc(&vsb, n)) || (e = dalloc(&ob, n)) || (e = dalloc(&lb, n)) || (e = dalloc(&acc, 2)))
@@ -1091,6 +1093,7 @@ struct Adj : AdjBase {
         if (compute_structure) structure(perm, nbr9);
         float4 *p0 = reinterpret_cast<float4 *>(tp ? tp->p : pb_a);
         launch_gather_records((int32_t)n, perm, xs_f, p0, st);
+        launch_quantize(fc, p0, qpos, (int32_t)n, st);
         PairList pl;
         pl.entries = list_entries;
         pl.count = list_count;
@@ -1117,8 +1120,9 @@ struct Adj : AdjBase {
                 pl.entries = tp->ents + (size_t)it * kListRows * n;
                 pl.count = tp->cnts + (size_t)it * n;
             }
-            launch_lambda(fc, cur, nbr9, lam, pl, st, false, nullptr, &tl);
-            launch_updatepos(fc, cur, lam, nbr9, pl, nxt, st, false, nullptr, it + 1 == iters ? &up : nullptr, &tu);
+            launch_lambda(fc, cur, qpos, nbr9, lam, pl, st, false, nullptr, &tl);
+            launch_updatepos(fc, cur, lam, nbr9, pl, nxt, qpos, st, false, nullptr, it + 1 == iters ? &up : nullptr,
+                             &tu);
             cur = nxt;
         }
         if (iters == 0) launch_update(fc, cur, up.pos, up.vel, vs, st);
@@ -1132,7 +1136,7 @@ struct Adj : AdjBase {
                 pl.entries = tp->ents + (size_t)iters * kListRows * n;
                 pl.count = tp->cnts + (size_t)iters * n;
             }
-            launch_vorticity_a(fc, cur, nbr9, vs, up.vel, vx, om, o, pl, st, false);
+            launch_vorticity_a(fc, cur, qpos, nbr9, vs, up.vel, vx, om, o, pl, st, false);
             launch_vorticity_b(fc, cur, nbr9, pl, vx, om, o, up.vel, st, false, nullptr, &te);
         }
     }

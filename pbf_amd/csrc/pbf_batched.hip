@@ -37,6 +37,7 @@ struct pbf_batch {
     Seg *segs = nullptr;
     uint16_t *scene_of = nullptr;
     float4 *pos = nullptr, *vel = nullptr, *xs = nullptr, *rec0 = nullptr, *rec_a = nullptr, *rec_b = nullptr;
+    uint2 *qpos = nullptr;  // quantized positions of the current sorted records
     float4 *vel_sorted = nullptr, *vxsph = nullptr, *omega = nullptr;
     float *lambda = nullptr, *vort = nullptr;
     uint32_t *keys = nullptr, *perm = nullptr;
@@ -68,6 +69,7 @@ struct pbf_batch {
         dfree(rec0);
         dfree(rec_a);
         dfree(rec_b);
+        dfree(qpos);
         dfree(vel_sorted);
         dfree(vxsph);
         dfree(omega);
@@ -94,7 +96,8 @@ struct pbf_batch {
         const DomMany d = dom();
         const int32_t nn = (int32_t)n;
         launch_predict(c, d, shape, pos, vel, xs, keys, st);
-        radix_sort_perm(rs, keys, nn, key_bits, nullptr, st, xs, rec0);  // last pass writes the records
+        // the last pass writes the records and their quantized positions
+        radix_sort_perm(rs, keys, nn, key_bits, nullptr, st, xs, rec0, qpos, c.qscale);
         launch_begin_cells(start, cells, epoch, st, &dense);
         launch_findcells(c, d, shape, rec0, start, end64, epoch, st);
         launch_neighbourcells(c, d, shape, rec0, start, end64, nbr9, st, &dense);
@@ -110,14 +113,14 @@ struct pbf_batch {
         const float4 *cur = rec0;
         for (uint32_t it = 0; it < params.iterations; it++) {
             float4 *dst = (it & 1) ? rec_b : rec_a;
-            launch_lambda(c, d, shape, cur, nbr9, lambda, pl, st, &dense);
-            launch_updatepos(c, d, shape, cur, lambda, nbr9, pl, dst, st, &dense,
+            launch_lambda(c, d, shape, cur, qpos, nbr9, lambda, pl, st, &dense);
+            launch_updatepos(c, d, shape, cur, lambda, nbr9, pl, dst, qpos, st, &dense,
                              it + 1 == params.iterations ? &up : nullptr);
             cur = dst;
         }
         if (params.iterations == 0) launch_update(c, d, shape, cur, pos, vel, up.vel_sorted, st);
         if (params.vorticity) {
-            launch_vorticity_a(c, d, shape, cur, nbr9, vel_sorted, vel, vxsph, omega, vort, pl, st, &dense);
+            launch_vorticity_a(c, d, shape, cur, qpos, nbr9, vel_sorted, vel, vxsph, omega, vort, pl, st, &dense);
             launch_vorticity_b(c, d, shape, cur, nbr9, pl, vxsph, omega, vort, vel, st, &dense);
         }
     }
@@ -223,6 +226,7 @@ int pbf_batch_create(pbf_batch **out, uint32_t nscenes, const pbf_scene_desc *sc
     TRY(dalloc(&b->rec0, n));
     TRY(dalloc(&b->rec_a, n));
     TRY(dalloc(&b->rec_b, n));
+    TRY(dalloc(&b->qpos, n));
     TRY(dalloc(&b->vel_sorted, n));
     TRY(dalloc(&b->vxsph, n));
     TRY(dalloc(&b->omega, n));

@@ -74,6 +74,13 @@ SolverConsts make_consts(uint32_t n, const int32_t grid[3], const pbf_params &p,
     }
     c.extforce = p.extforce;
     c.key_mask = key_bits >= 32 ? 0xFFFFFFFFu : ((1u << key_bits) - 1u);
+    // qscale = 2^(12 - e) for h = m 2^e, m in [0.5, 1): h qscale in [2048, 4096)
+    int e = 0;
+    (void)std::frexp((double)h, &e);
+    const double qs = std::ldexp(1.0, 12 - e);
+    c.qscale = (float)qs;
+    const double qb = (double)h * qs * (1.0 + 2e-5) + 4.0;
+    c.qcut = (int32_t)std::ceil(qb * qb);
     return c;
 }
 
@@ -209,6 +216,7 @@ struct pbf_sph {
     float4 *xs = nullptr;                        // predicted records, id order
     float4 *rec0 = nullptr;                      // sorted records at x*
     float4 *rec_a = nullptr, *rec_b = nullptr;   // sorted records of the solver iterations (Jacobi)
+    uint2 *qpos = nullptr;                       // quantized positions of the current sorted records
     float4 *rec = nullptr;                       // sorted records after the last iteration (borrowed)
     uint16_t *list_entries = nullptr, *list_count = nullptr;  // neighbours within h, lambda -> update
     DenseList dense;                             // the step's particles with the most candidates
@@ -246,6 +254,7 @@ struct pbf_sph {
         dfree(rec0);
         dfree(rec_a);
         dfree(rec_b);
+        dfree(qpos);
         rec = nullptr;
         dfree(list_entries);
         dfree(list_count);
@@ -388,6 +397,7 @@ int pbf_sph_create(pbf_sph **out, uint32_t n, int32_t gx, int32_t gy, int32_t gz
     TRY(dalloc(&s->rec0, (size_t)n + kPad));
     TRY(dalloc(&s->rec_a, (size_t)n + kPad));
     TRY(dalloc(&s->rec_b, (size_t)n + kPad));
+    TRY(dalloc(&s->qpos, (size_t)n + kPad));
     s->rec = s->rec0;
     TRY(dalloc(&s->list_entries, (size_t)kListRows * n));
     TRY(dalloc(&s->list_count, n));
@@ -414,6 +424,7 @@ int pbf_sph_create(pbf_sph **out, uint32_t n, int32_t gx, int32_t gy, int32_t gz
     TRY(hipMemset(s->rec0, 0, ((size_t)n + kPad) * sizeof(float4)));
     TRY(hipMemset(s->rec_a, 0, ((size_t)n + kPad) * sizeof(float4)));
     TRY(hipMemset(s->rec_b, 0, ((size_t)n + kPad) * sizeof(float4)));
+    TRY(hipMemset(s->qpos, 0, ((size_t)n + kPad) * sizeof(uint2)));
     TRY(hipMemset(s->lambda, 0, ((size_t)n + kPad) * sizeof(float)));
     TRY(hipMemset(s->vort, 0, ((size_t)n + kPad) * sizeof(float)));
     TRY(hipDeviceSynchronize());
@@ -490,8 +501,9 @@ int sph_enqueue(pbf_sph *s, hipStream_t st, bool events) {
     TIMED(KID_PREDICT, launch_predict(c, s->pos, s->vel, s->xs, s->keys, st, s->highlight));
     PHASE(1);
     // sort (src/SPH.cpp:263-268)
-    // the sort's last pass writes the sorted records (globalsort.glsl:64 as a gather)
-    TIMED(KID_SORT, radix_sort_perm(s->rs, s->keys, n, s->key_bits, nullptr, st, s->xs, s->rec0));
+    // the sort's last pass writes the sorted records (globalsort.glsl:64 as a
+    // gather) and their quantized positions
+    TIMED(KID_SORT, radix_sort_perm(s->rs, s->keys, n, s->key_bits, nullptr, st, s->xs, s->rec0, s->qpos, c.qscale));
     PHASE(2);
     // neighbour cells (src/SPH.cpp:270-275; src/NeighbourCellFinder.cpp:113-148)
     CellTables &t = s->cells_t;
@@ -517,8 +529,8 @@ int sph_enqueue(pbf_sph *s, hipStream_t st, bool events) {
     const uint32_t iters = s->params.iterations;
     for (uint32_t it = 0; it < iters; it++) {
         float4 *dst = (it & 1) ? s->rec_b : s->rec_a;
-        TIMED(KID_LAMBDA, launch_lambda(c, cur, t.nbr9, s->lambda, pl, st, true, &s->dense));
-        TIMED(KID_UPDATEPOS, launch_updatepos(c, cur, s->lambda, t.nbr9, pl, dst, st, true, &s->dense,
+        TIMED(KID_LAMBDA, launch_lambda(c, cur, s->qpos, t.nbr9, s->lambda, pl, st, true, &s->dense));
+        TIMED(KID_UPDATEPOS, launch_updatepos(c, cur, s->lambda, t.nbr9, pl, dst, s->qpos, st, true, &s->dense,
                                               it + 1 == iters ? &up : nullptr));
         cur = dst;
     }
@@ -528,8 +540,8 @@ int sph_enqueue(pbf_sph *s, hipStream_t st, bool events) {
     if (iters == 0) TIMED(KID_UPDATE, launch_update(c, s->rec, s->pos, s->vel, up.vel_sorted, st));
     if (s->params.vorticity) {
         TIMED(KID_VORTICITY_A,
-              launch_vorticity_a(c, s->rec, t.nbr9, s->vel_sorted, s->vel, s->vxsph, s->omega, s->vort, pl, st, true,
-                                 &s->dense));
+              launch_vorticity_a(c, s->rec, s->qpos, t.nbr9, s->vel_sorted, s->vel, s->vxsph, s->omega, s->vort, pl,
+                                 st, true, &s->dense));
         TIMED(KID_VORTICITY_B,
               launch_vorticity_b(c, s->rec, t.nbr9, pl, s->vxsph, s->omega, s->vort, s->vel, st, true, &s->dense));
     }
@@ -985,19 +997,24 @@ static int nbr_scratch(uint32_t n, const void *nbr12, int32_t **nbr9, hipStream_
 }
 
 // The two vorticity passes of the stage on caller memory, with the [9][n]
-// table in nbr9 (already converted) and 3 n float4 of scratch in tmp.
+// table in nbr9 (already converted) and kVortStageBytes per particle of
+// scratch in tmp: three float4 arrays, then the quantized positions.
 static void launch_vorticity_stage(uint32_t n, const int32_t grid[3], const pbf_params &p, const void *rec4,
                                    void *vel4, void *vort_out, const int32_t *nbr9, float4 *tmp, hipStream_t st) {
     const SolverConsts c = make_consts(n, grid, p, key_bits_of(grid));
     const float4 *rec = (const float4 *)rec4;
     float4 *vel = (float4 *)vel4;
     float4 *vs = tmp, *vx = tmp + n, *om = tmp + 2 * (size_t)n;
+    uint2 *q = reinterpret_cast<uint2 *>(tmp + 3 * (size_t)n);
     // sorted copy of the velocities, vs[g] = vel[id(rec[g])] (SPH::Run keeps
     // one from the update pass)
     launch_gather_by_id((int32_t)n, rec, vel, vs, st);
-    launch_vorticity_a(c, rec, nbr9, vs, vel, vx, om, (float *)vort_out, PairList(), st);
+    launch_quantize(c, rec, q, (int32_t)n, st);
+    launch_vorticity_a(c, rec, q, nbr9, vs, vel, vx, om, (float *)vort_out, PairList(), st);
     launch_vorticity_b(c, rec, nbr9, PairList(), vx, om, (const float *)vort_out, vel, st);
 }
+
+constexpr size_t kVortStageBytes = 3 * sizeof(float4) + sizeof(uint2);
 
 static int release_scratch(void *p, hipStream_t st) {
     hipError_t e = hipStreamSynchronize(st);
@@ -1016,8 +1033,17 @@ int pbf_stage_lambda(uint32_t n, const int32_t grid[3], const pbf_params *p, con
     int32_t *nbr9 = nullptr;
     if ((rc = nbr_scratch(n, nbr12, &nbr9, st))) return rc;
     const SolverConsts c = make_consts(n, grid, *p, key_bits_of(grid));
-    launch_lambda(c, (const float4 *)rec4, nbr9, (float *)lambda_out, PairList(), st);
-    return release_scratch(nbr9, st);
+    uint2 *q = nullptr;  // quantized positions of the caller's records
+    const hipError_t e = dalloc(&q, n);
+    if (e != hipSuccess) {
+        (void)hipFree(nbr9);
+        return hip_fail(e, "pbf_stage_lambda");
+    }
+    launch_quantize(c, (const float4 *)rec4, q, (int32_t)n, st);
+    launch_lambda(c, (const float4 *)rec4, q, nbr9, (float *)lambda_out, PairList(), st);
+    rc = release_scratch(nbr9, st);
+    (void)hipFree(q);
+    return rc;
 }
 
 int pbf_stage_updatepos(uint32_t n, const int32_t grid[3], const pbf_params *p, const void *rec4_in,
@@ -1030,7 +1056,8 @@ int pbf_stage_updatepos(uint32_t n, const int32_t grid[3], const pbf_params *p, 
     int32_t *nbr9 = nullptr;
     if ((rc = nbr_scratch(n, nbr12, &nbr9, st))) return rc;
     const SolverConsts c = make_consts(n, grid, *p, key_bits_of(grid));
-    launch_updatepos(c, (const float4 *)rec4_in, (const float *)lambda, nbr9, PairList(), (float4 *)rec4_out, st);
+    launch_updatepos(c, (const float4 *)rec4_in, (const float *)lambda, nbr9, PairList(), (float4 *)rec4_out,
+                     nullptr, st);
     return release_scratch(nbr9, st);
 }
 
@@ -1051,9 +1078,9 @@ int pbf_stage_vorticity(uint32_t n, const int32_t grid[3], const pbf_params *p, 
     if (!rec4 || !nbr12 || !vel4 || !vort_out) return invalid("null pointer");
     hipStream_t st = (hipStream_t)stream;
     int32_t *nbr9 = nullptr;
-    float4 *tmp = nullptr;  // sorted velocities, XSPH velocities, omega
+    float4 *tmp = nullptr;  // sorted velocities, XSPH velocities, omega, quantized positions
     if ((rc = nbr_scratch(n, nbr12, &nbr9, st))) return rc;
-    hipError_t e = dalloc(&tmp, (size_t)3 * n);
+    hipError_t e = dalloc(&tmp, ((size_t)kVortStageBytes * n + sizeof(float4) - 1) / sizeof(float4));
     if (e != hipSuccess) {
         (void)hipFree(nbr9);
         return hip_fail(e, "pbf_stage_vorticity");
@@ -1071,7 +1098,8 @@ int pbf_stage_vorticity(uint32_t n, const int32_t grid[3], const pbf_params *p, 
 // ------------------------------------------------------ vorticity, no allocation
 // Scratch of pbf_stage_vorticity_noalloc: the [9][n] neighbour table (36 B per
 // particle, then aligned) followed by the sorted, XSPH and omega velocities
-// (48 B per particle), for at most PBF_NOALLOC_MAX_N particles. Each GPU the
+// and the quantized positions (56 B per particle), for at most
+// PBF_NOALLOC_MAX_N particles. Each GPU the
 // library runs on holds one copy until the process ends. In host code the
 // name `scratch` is a host-side stand-in, not a device address; the address
 // on a device comes from hipGetSymbolAddress, looked up once per device with
@@ -1079,7 +1107,7 @@ int pbf_stage_vorticity(uint32_t n, const int32_t grid[3], const pbf_params *p, 
 }  // extern "C"
 
 static constexpr size_t kNoAllocTableBytes = (size_t)36 * PBF_NOALLOC_MAX_N;
-static constexpr size_t kNoAllocBytes = kNoAllocTableBytes + (size_t)48 * PBF_NOALLOC_MAX_N;
+static constexpr size_t kNoAllocBytes = kNoAllocTableBytes + kVortStageBytes * PBF_NOALLOC_MAX_N;
 static_assert(kNoAllocTableBytes % 256 == 0, "the velocity arrays start 256-byte aligned");
 static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 

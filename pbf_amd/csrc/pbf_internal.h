@@ -25,6 +25,12 @@ struct SolverConsts {
     float lo[3], hi[3], gdir[3];
     int32_t extforce;
     uint32_t key_mask;
+    // Quantized positions of the candidate filter: a coordinate in units of
+    // 1/qscale (a power of two with h qscale in [2048, 4096)), truncated, low
+    // 16 bits; a pair with fl(sqrt(sq)) <= h has a quantized squared
+    // distance of at most qcut.
+    float qscale;
+    int32_t qcut;
 };
 
 SolverConsts make_consts(uint32_t n, const int32_t grid[3], const pbf_params &p, int key_bits);
@@ -75,6 +81,19 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
     if (b >= nb / group * group) return b;
     const uint32_t x = b % 8u, k = b / 8u;
     return (k / chunk) * group + x * chunk + (k % chunk);
+}
+
+// Quantized position of a record for the candidate filter: each coordinate
+// times qscale, clamped into the int32 range (NaN to 0), truncated, low 16
+// bits; x and y in the first word, z in the low half of the second. The scale
+// is a power of two, so the product is exact and each quantized coordinate is
+// within one unit of the scaled one; clamping only brings values closer.
+__device__ __forceinline__ uint32_t quantize1(float x, float qscale) {
+    const float v = fminf(fmaxf(x * qscale, -2147483648.0f), 2147483520.0f);
+    return (uint32_t)(int32_t)v & 0xFFFFu;
+}
+__device__ __forceinline__ uint2 quantize_pos(const float4 &r, float qscale) {
+    return make_uint2(quantize1(r.x, qscale) | (quantize1(r.y, qscale) << 16), quantize1(r.z, qscale));
 }
 
 // ------------------------------------------------------------ launchers
@@ -156,8 +175,11 @@ struct SolverTape {
     uint8_t *clamp = nullptr;  // position update: bit a set where axis a was not clamped, bit 3 + a at the upper wall
     float4 *eta = nullptr;     // vorticity B: eta before normalisation
 };
-void launch_lambda(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, float *lambda, const PairList &pl,
-                   hipStream_t st, bool padded = false, const DenseList *dl = nullptr,
+// q: the quantized positions of rec, slot for slot (launch_quantize, or the
+// sort's last pass and the position update's q_out), read by the candidate
+// scans of lambda and vorticity A; padded like rec.
+void launch_lambda(const SolverConsts &c, const float4 *rec, const uint2 *q, const int32_t *nbr9, float *lambda,
+                   const PairList &pl, hipStream_t st, bool padded = false, const DenseList *dl = nullptr,
                    const SolverTape *tape = nullptr);
 // The step's closing update (update.glsl:16-28: velocities from the
 // positions before and after the step, both stored by particle id; vel_sorted,
@@ -169,15 +191,18 @@ struct StepUpdate {
     float4 *vel = nullptr;
     float4 *vel_sorted = nullptr;
 };
+// q_out: the quantized positions of rec_out, written beside it (null: none).
 void launch_updatepos(const SolverConsts &c, const float4 *rec_in, const float *lambda, const int32_t *nbr9,
-                      const PairList &pl, float4 *rec_out, hipStream_t st, bool padded = false,
+                      const PairList &pl, float4 *rec_out, uint2 *q_out, hipStream_t st, bool padded = false,
                       const DenseList *dl = nullptr, const StepUpdate *up = nullptr,
                       const SolverTape *tape = nullptr);
 void launch_update(const SolverConsts &c, const float4 *rec, float4 *pos, float4 *vel, float4 *vel_sorted,
                    hipStream_t st);
-void launch_vorticity_a(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, const float4 *vel_sorted,
-                        const float4 *vel_id, float4 *vxsph, float4 *omega, float *vort, const PairList &pl,
-                        hipStream_t st, bool padded = false, const DenseList *dl = nullptr);
+void launch_vorticity_a(const SolverConsts &c, const float4 *rec, const uint2 *q, const int32_t *nbr9,
+                        const float4 *vel_sorted, const float4 *vel_id, float4 *vxsph, float4 *omega, float *vort,
+                        const PairList &pl, hipStream_t st, bool padded = false, const DenseList *dl = nullptr);
+// q[i] = the quantized position of rec[i], i < count.
+void launch_quantize(const SolverConsts &c, const float4 *rec, uint2 *q, int32_t count, hipStream_t st);
 void launch_vorticity_b(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, const PairList &pl,
                         const float4 *vxsph, const float4 *omega, const float *vort, float4 *vel_id, hipStream_t st,
                         bool padded = false, const DenseList *dl = nullptr, const SolverTape *tape = nullptr);
@@ -201,16 +226,16 @@ void launch_neighbourcells(const SolverConsts &c, const DomMany &d, const Launch
                            const int32_t *start, const unsigned long long *end64, int32_t *nbr9, hipStream_t st,
                            const DenseList *dl = nullptr);
 void launch_lambda(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec,
-                   const int32_t *nbr9, float *lambda, const PairList &pl, hipStream_t st,
+                   const uint2 *q, const int32_t *nbr9, float *lambda, const PairList &pl, hipStream_t st,
                    const DenseList *dl = nullptr);
 void launch_updatepos(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec_in,
-                      const float *lambda, const int32_t *nbr9, const PairList &pl, float4 *rec_out,
+                      const float *lambda, const int32_t *nbr9, const PairList &pl, float4 *rec_out, uint2 *q_out,
                       hipStream_t st, const DenseList *dl = nullptr, const StepUpdate *up = nullptr);
 void launch_update(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec, float4 *pos,
                    float4 *vel, float4 *vel_sorted, hipStream_t st);
 void launch_vorticity_a(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec,
-                        const int32_t *nbr9, const float4 *vel_sorted, const float4 *vel_id, float4 *vxsph,
-                        float4 *omega, float *vort, const PairList &pl, hipStream_t st,
+                        const uint2 *q, const int32_t *nbr9, const float4 *vel_sorted, const float4 *vel_id,
+                        float4 *vxsph, float4 *omega, float *vort, const PairList &pl, hipStream_t st,
                         const DenseList *dl = nullptr);
 void launch_vorticity_b(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec,
                         const int32_t *nbr9, const PairList &pl, const float4 *vxsph, const float4 *omega,
@@ -237,9 +262,10 @@ void radix_free(RadixScratch &s);
 // keys is clobbered. The permutation is written to perm (unless null), which
 // must not alias the scratch buffers. With gather_dst, the last pass also
 // writes gather_dst[k] = gather_src[perm[k]] (the sorted records), in place of
-// a separate gather.
+// a separate gather, and with q_dst, their quantized positions (scale qscale).
 void radix_sort_perm(RadixScratch &s, uint32_t *keys, int32_t n, int key_bits, uint32_t *perm, hipStream_t st,
-                     const float4 *gather_src = nullptr, float4 *gather_dst = nullptr);
+                     const float4 *gather_src = nullptr, float4 *gather_dst = nullptr, uint2 *q_dst = nullptr,
+                     float qscale = 0.0f);
 
 // ------------------------------------------------------------ errors and helpers
 void set_error(const std::string &msg);

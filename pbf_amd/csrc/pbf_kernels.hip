@@ -66,7 +66,7 @@ __device__ __forceinline__ float clamp_glsl(float x, float lo, float hi) {
 // Clamp state of one axis for the adjoint's tape: 1 inside the walls, 8 at
 // the upper wall, 0 at the lower one (shifted by the axis).
 __device__ __forceinline__ uint32_t clamp_state(float x, float lo, float hi, int axis) {
-    return ((x >= lo) & (x <= hi) ? 1u : (x > hi ? 8u : 0u)) << axis;
+    return (((x >= lo) & (x <= hi)) ? 1u : (x > hi ? 8u : 0u)) << axis;
 }
 
 // GetHash, counting.glsl:53-57: FP32 dot of the clamped integer cell.
@@ -454,6 +454,21 @@ __device__ __forceinline__ bool maybe_near(const SolverConsts &c, const float4 &
     return fmaf(dx, dx, fmaf(dy, dy, dz * dz)) <= c.h2loose;
 }
 
+// True when the candidate may satisfy sq <= h2cut, from quantized positions
+// (quantize_pos): each axis difference modulo 2^16 as a signed 16-bit value,
+// the squares summed by two saturating dot products. A pair within h differs
+// by at most h qscale (1 + 1e-5) < 4097 units per axis, so its differences
+// are exact up to the truncations (under one unit per coordinate: within
+// 2 sqrt(3) of the scaled distance), and qcut = ceil((h qscale (1 + 2e-5)
+// + 4)^2) keeps it. Far pairs may alias to near ones and are rejected by the
+// exact test.
+typedef short short2_v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ bool maybe_near_q(const SolverConsts &c, const uint2 &qi, const uint2 &qj) {
+    const short2_v dxy = __builtin_bit_cast(short2_v, qi.x) - __builtin_bit_cast(short2_v, qj.x);
+    const short2_v dz = __builtin_bit_cast(short2_v, qi.y) - __builtin_bit_cast(short2_v, qj.y);
+    return __builtin_amdgcn_sdot2(dxy, dxy, __builtin_amdgcn_sdot2(dz, dz, 0, true), true) <= c.qcut;
+}
+
 // Row word o of particle i: start + (count << 24). The count is the
 // reference's wrapped one, read back with an arithmetic shift.
 __device__ __forceinline__ int32_t row_count(int32_t word) { return word >> 24; }
@@ -487,13 +502,20 @@ __device__ __forceinline__ int32_t load_rows(const SolverConsts &c, const int32_
 // kBatch at a time with their loads issued together, so a particle with
 // hundreds of neighbours waits for one round trip per batch rather than per
 // neighbour.
-template <int NB, bool PAD, class L, class Extra, class Body>
+// QUANT: the distance test reads the quantized positions qpos (8 bytes per
+// candidate instead of 16, four integer operations instead of six float
+// ones); else the records themselves.
+template <int NB, bool PAD, bool QUANT, class L, class Extra, class Body>
 __device__ __forceinline__ void scan_neighbours(const SolverConsts &c, const Seg &sg, const float4 *__restrict__ rec,
-                                                int32_t i, const float4 &pi, L &lds, Extra &&extra, Body &&body) {
+                                                const uint2 *__restrict__ qpos, int32_t i, const float4 &pi, L &lds,
+                                                Extra &&extra, Body &&body) {
     constexpr int LC = L::kCap;
     const int t = threadIdx.x;
     const int32_t li = i - sg.off;
     const float4 *__restrict__ srec = rec + sg.off;
+    const uint2 *__restrict__ sqpos = QUANT ? qpos + sg.off : nullptr;
+    uint2 qi = make_uint2(0u, 0u);
+    if constexpr (QUANT) qi = qpos[i];
     using X = decltype(extra(0));
     int cnt = 0;
     auto flush = [&]() {
@@ -533,6 +555,7 @@ __device__ __forceinline__ void scan_neighbours(const SolverConsts &c, const Seg
         // zero record.
         const int32_t last = max(min(s0 + e, sg.n) - 1, 0);
         const float4 *__restrict__ row = srec + s0;
+        const uint2 *__restrict__ qrow = QUANT ? sqpos + s0 : nullptr;
         uint32_t boff = 2u * (uint32_t)(cnt * NB + t);
         const uint32_t bfull = 2u * (uint32_t)((LC - kBatch) * NB + t);
         for (int32_t q = 0; q < e; q += kBatch) {
@@ -548,8 +571,14 @@ __device__ __forceinline__ void scan_neighbours(const SolverConsts &c, const Seg
             }
             const int32_t j0 = s0 + q;
             float4 p[kBatch];
+            uint2 pq[kBatch];
 #pragma unroll
-            for (int u = 0; u < kBatch; u++) p[u] = PAD ? row[q + u] : srec[min(j0 + u, last)];
+            for (int u = 0; u < kBatch; u++) {
+                if constexpr (QUANT)
+                    pq[u] = PAD ? qrow[q + u] : sqpos[min(j0 + u, last)];
+                else
+                    p[u] = PAD ? row[q + u] : srec[min(j0 + u, last)];
+            }
             const uint32_t code0 = (uint32_t)((o << 8) | q);
 #pragma unroll
             for (int u = 0; u < kBatch; u += 2) {
@@ -560,9 +589,14 @@ __device__ __forceinline__ void scan_neighbours(const SolverConsts &c, const Seg
                 asm("v_mov_b32 %0, %1" : "=v"(two) : "s"(two_s));
 #pragma unroll
                 for (int v = 0; v < 2; v++) {
-                    const float4 pj =
-                        (PAD || j0 + u + v < sg.n) ? p[u + v] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                    bool f = (q + u + v < e) & maybe_near(c, pi, pj);
+                    // past the scene: the zero record, whose quantized form is zero
+                    const bool inside = PAD || j0 + u + v < sg.n;
+                    bool near;
+                    if constexpr (QUANT)
+                        near = maybe_near_q(c, qi, inside ? pq[u + v] : make_uint2(0u, 0u));
+                    else
+                        near = maybe_near(c, pi, inside ? p[u + v] : make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+                    bool f = (q + u + v < e) & near;
                     if (SELF) f = f & (q + u + v != selfq);
                     *reinterpret_cast<uint16_t *>(lbase + boff) = (uint16_t)(v ? two >> 16 : two);
                     boff += f ? 2u * NB : 0u;
@@ -627,8 +661,8 @@ __device__ __forceinline__ void replay_neighbours(const SolverConsts &c, const S
     const int t = threadIdx.x;
     const uint32_t cnt = entries ? count[i] : LIST_NONE;
     if (cnt == LIST_NONE) {
-        scan_neighbours<NB, PAD>(c, sg, rec, i, pi, lds, extra,
-                                 [&](int32_t j, uint32_t, const float4 &pj, auto x) { body(j, pj, x, false); });
+        scan_neighbours<NB, PAD, false>(c, sg, rec, nullptr, i, pi, lds, extra,
+                                        [&](int32_t j, uint32_t, const float4 &pj, auto x) { body(j, pj, x, false); });
         return;
     }
     using X = decltype(extra(0));
@@ -956,6 +990,7 @@ struct UpdatePosPass {
     const float4 *__restrict__ rec_in;
     const float *__restrict__ lambda;
     float4 *__restrict__ rec_out;
+    uint2 *__restrict__ q_out;  // quantized positions of rec_out, or none
     StepUpdate up;  // the step's closing update (last iteration), or none
     SolverTape tp;
     float4 pi;
@@ -993,6 +1028,7 @@ struct UpdatePosPass {
         z = clamp_glsl(z, sg.lo[2], sg.hi[2]);
         const float4 r = make_float4(x, y, z, pi.w);
         rec_out[i] = r;
+        if (q_out) q_out[i] = quantize_pos(r, c.qscale);
         if (up.pos) update_one(c, sg, i, r, up);
     }
 };
@@ -1102,6 +1138,7 @@ __device__ __forceinline__ bool leave_dense(int32_t cand, const DenseList &dl) {
 // calclambda.glsl:66-103; records the neighbours within h for the position update.
 template <bool H1, int NB, bool PAD, class Dom>
 __global__ __launch_bounds__(NB) void k_lambda(SolverConsts c, Dom dom, const float4 *__restrict__ rec,
+                                                     const uint2 *__restrict__ qpos,
                                                      const int32_t *__restrict__ nbr9, float *__restrict__ lambda,
                                                      uint16_t *__restrict__ list_entries,
                                                      uint16_t *__restrict__ list_count, int32_t list_cap,
@@ -1126,7 +1163,7 @@ __global__ __launch_bounds__(NB) void k_lambda(SolverConsts c, Dom dom, const fl
     const float4 pi = rec[i];
     float rho = 0.0f, sk = 0.0f, gix = 0.0f, giy = 0.0f, giz = 0.0f;
     PairListOut out = {list_entries, list_count, list_cap, 0, list_entries ? list_entries + i : nullptr};
-    scan_neighbours<NB, PAD>(c, sg, rec, i, pi, lds, [](int32_t) { return 0; },
+    scan_neighbours<NB, PAD, true>(c, sg, rec, qpos, i, pi, lds, [](int32_t) { return 0; },
                              [&](int32_t, uint32_t code, const float4 &pj, int) {
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const float sq = sq_len(dx, dy, dz);
@@ -1164,15 +1201,16 @@ __global__ __launch_bounds__(NB) void k_updatepos(SolverConsts c, Dom dom, const
                                                         const int32_t *__restrict__ nbr9,
                                                         const uint16_t *__restrict__ list_entries,
                                                         const uint16_t *__restrict__ list_count,
-                                                        float4 *__restrict__ rec_out, DenseList dl,
-                                                        uint32_t dense_blocks, StepUpdate up, SolverTape tp) {
+                                                        float4 *__restrict__ rec_out, uint2 *__restrict__ q_out,
+                                                        DenseList dl, uint32_t dense_blocks, StepUpdate up,
+                                                        SolverTape tp) {
     __shared__ union {
         NbrLds<NB, kReplayLcap> light;
         DenseLds<3, 0, NB / 64> dense;
     } u;
     {
         if (blockIdx.x < dense_blocks) {
-            UpdatePosPass<H1, PAD> ps{c, rec_in, lambda, rec_out, up, tp, {}, 0.0f};
+            UpdatePosPass<H1, PAD> ps{c, rec_in, lambda, rec_out, q_out, up, tp, {}, 0.0f};
             dense_pass<UpdatePosPass<H1, PAD>, PAD, NB / 64>(c, dom, rec_in, nbr9, dl, const_cast<uint16_t *>(list_entries),
                                                     const_cast<uint16_t *>(list_count), 0, ps, u.dense, blockIdx.x,
                                                     dense_blocks);
@@ -1215,6 +1253,7 @@ __global__ __launch_bounds__(NB) void k_updatepos(SolverConsts c, Dom dom, const
     z = clamp_glsl(z, sg.lo[2], sg.hi[2]);
     const float4 r = make_float4(x, y, z, pi.w);
     rec_out[i] = r;
+    if (q_out) q_out[i] = quantize_pos(r, c.qscale);
     if (up.pos) update_one(c, sg, i, r, up);
 }
 
@@ -1233,6 +1272,7 @@ __global__ __launch_bounds__(1024) void k_update(SolverConsts c, Dom dom, int32_
 // neighbours within h for the second half.
 template <bool H1, int NB, bool PAD, class Dom>
 __global__ __launch_bounds__(NB) void k_vorticity_a(SolverConsts c, Dom dom, const float4 *__restrict__ rec,
+                                                          const uint2 *__restrict__ qpos,
                                                           const int32_t *__restrict__ nbr9,
                                                           const float4 *__restrict__ vel_sorted,
                                                           const float4 *__restrict__ vel_id,
@@ -1262,7 +1302,7 @@ __global__ __launch_bounds__(NB) void k_vorticity_a(SolverConsts c, Dom dom, con
     float sx = 0.0f, sy = 0.0f, sz = 0.0f, wx = 0.0f, wy = 0.0f, wz = 0.0f;
     PairListOut out = {list_entries, list_count, list_cap, 0, list_entries ? list_entries + i : nullptr};
     // a zero record past the scene has id 0: imageLoad (velocitytexture, 0)
-    scan_neighbours<NB, PAD>(c, sg, rec, i, pi, lds,
+    scan_neighbours<NB, PAD, true>(c, sg, rec, qpos, i, pi, lds,
                              [&](int32_t j) { return *(j < sg.n ? vel_sorted + sg.off + j : vel_id + sg.off); },
                              [&](int32_t, uint32_t code, const float4 &pj, const float4 &vj) {
         const float px = pi.x - pj.x, py = pi.y - pj.y, pz = pi.z - pj.z;
@@ -1349,6 +1389,12 @@ __global__ __launch_bounds__(NB) void k_vorticity_b(SolverConsts c, Dom dom, con
     const float oy = v.y + sc * (nz * w.x - nx * w.z);
     const float oz = v.z + sc * (nx * w.y - ny * w.x);
     vel_id[sg.off + id_of(pi)] = make_float4(ox, oy, oz, 0.0f);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_quantize(float qscale, const float4 *__restrict__ rec,
+                                                   uint2 *__restrict__ q, int32_t count) {
+    const int32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < count) q[i] = quantize_pos(rec[i], qscale);
 }
 
 // ----------------------------------------------------------------- layout conversions
@@ -1502,7 +1548,7 @@ void update_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, con
 }
 template <class Dom>
 void lambda_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, bool pad, const float4 *rec,
-                 const int32_t *nbr9, float *lambda, const PairList &pl, hipStream_t st,
+                 const uint2 *q, const int32_t *nbr9, float *lambda, const PairList &pl, hipStream_t st,
                  const DenseList *dl = nullptr, const SolverTape *tape = nullptr) {
     if (c.n <= 0) return;
     const DenseList dlv = dl ? *dl : DenseList();
@@ -1513,17 +1559,17 @@ void lambda_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, boo
         const unsigned db = dense_blocks_of(dl, c.n, NB);
         const unsigned grid = nb_blocks(c.n, NB) + db;
         if (c.h_is_one)
-            k_lambda<true, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec, nbr9, lambda, pl.entries, pl.count, pl.cap,
-                                                              dlv, db, tp);
+            k_lambda<true, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec, q, nbr9, lambda, pl.entries, pl.count,
+                                                              pl.cap, dlv, db, tp);
         else
-            k_lambda<false, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec, nbr9, lambda, pl.entries, pl.count, pl.cap,
-                                                               dlv, db, tp);
+            k_lambda<false, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec, q, nbr9, lambda, pl.entries, pl.count,
+                                                               pl.cap, dlv, db, tp);
     });
 }
 template <class Dom>
 void updatepos_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, bool pad, const float4 *rec_in,
-                    const float *lambda, const int32_t *nbr9, const PairList &pl, float4 *rec_out, hipStream_t st,
-                    const DenseList *dl = nullptr, const StepUpdate *up = nullptr,
+                    const float *lambda, const int32_t *nbr9, const PairList &pl, float4 *rec_out, uint2 *q_out,
+                    hipStream_t st, const DenseList *dl = nullptr, const StepUpdate *up = nullptr,
                     const SolverTape *tape = nullptr) {
     if (c.n <= 0) return;
     const DenseList dlv = dl ? *dl : DenseList();
@@ -1536,15 +1582,15 @@ void updatepos_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, 
         const unsigned grid = nb_blocks(c.n, NB) + db;
         if (c.h_is_one)
             k_updatepos<true, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec_in, lambda, nbr9, pl.entries, pl.count,
-                                                                 rec_out, dlv, db, upv, tp);
+                                                                 rec_out, q_out, dlv, db, upv, tp);
         else
             k_updatepos<false, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec_in, lambda, nbr9, pl.entries, pl.count,
-                                                                  rec_out, dlv, db, upv, tp);
+                                                                  rec_out, q_out, dlv, db, upv, tp);
     });
 }
 template <class Dom>
 void vorticity_a_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, bool pad, const float4 *rec,
-                      const int32_t *nbr9, const float4 *vel_sorted, const float4 *vel_id, float4 *vxsph,
+                      const uint2 *q, const int32_t *nbr9, const float4 *vel_sorted, const float4 *vel_id, float4 *vxsph,
                       float4 *omega, float *vort, const PairList &pl, hipStream_t st, const DenseList *dl = nullptr) {
     if (c.n <= 0) return;
     const DenseList dlv = dl ? *dl : DenseList();
@@ -1554,11 +1600,12 @@ void vorticity_a_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls
         const unsigned db = dense_blocks_of(dl, c.n, NB);
         const unsigned grid = nb_blocks(c.n, NB) + db;
         if (c.h_is_one)
-            k_vorticity_a<true, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec, nbr9, vel_sorted, vel_id, vxsph, omega,
-                                                                   vort, pl.entries, pl.count, pl.cap, dlv, db);
+            k_vorticity_a<true, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec, q, nbr9, vel_sorted, vel_id, vxsph,
+                                                                   omega, vort, pl.entries, pl.count, pl.cap, dlv, db);
         else
-            k_vorticity_a<false, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec, nbr9, vel_sorted, vel_id, vxsph, omega,
-                                                                    vort, pl.entries, pl.count, pl.cap, dlv, db);
+            k_vorticity_a<false, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec, q, nbr9, vel_sorted, vel_id, vxsph,
+                                                                    omega, vort, pl.entries, pl.count, pl.cap, dlv,
+                                                                    db);
     });
 }
 template <class Dom>
@@ -1616,23 +1663,27 @@ void launch_neighbourcells(const SolverConsts &c, const float4 *sorted, const in
                            uint32_t *highlight) {
     neighbourcells_impl(c, one(c), kDefaultShape, sorted, start, end64, nbr9, st, dl, highlight);
 }
-void launch_lambda(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, float *lambda, const PairList &pl,
-                   hipStream_t st, bool padded, const DenseList *dl, const SolverTape *tape) {
-    lambda_impl(c, one(c), kDefaultShape, padded, rec, nbr9, lambda, pl, st, dl, tape);
+void launch_lambda(const SolverConsts &c, const float4 *rec, const uint2 *q, const int32_t *nbr9, float *lambda,
+                   const PairList &pl, hipStream_t st, bool padded, const DenseList *dl, const SolverTape *tape) {
+    lambda_impl(c, one(c), kDefaultShape, padded, rec, q, nbr9, lambda, pl, st, dl, tape);
 }
 void launch_updatepos(const SolverConsts &c, const float4 *rec_in, const float *lambda, const int32_t *nbr9,
-                      const PairList &pl, float4 *rec_out, hipStream_t st, bool padded, const DenseList *dl,
-                      const StepUpdate *up, const SolverTape *tape) {
-    updatepos_impl(c, one(c), kDefaultShape, padded, rec_in, lambda, nbr9, pl, rec_out, st, dl, up, tape);
+                      const PairList &pl, float4 *rec_out, uint2 *q_out, hipStream_t st, bool padded,
+                      const DenseList *dl, const StepUpdate *up, const SolverTape *tape) {
+    updatepos_impl(c, one(c), kDefaultShape, padded, rec_in, lambda, nbr9, pl, rec_out, q_out, st, dl, up, tape);
 }
 void launch_update(const SolverConsts &c, const float4 *rec, float4 *pos, float4 *vel, float4 *vel_sorted,
                    hipStream_t st) {
     update_impl(c, one(c), kDefaultShape, rec, pos, vel, vel_sorted, st);
 }
-void launch_vorticity_a(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, const float4 *vel_sorted,
-                        const float4 *vel_id, float4 *vxsph, float4 *omega, float *vort, const PairList &pl,
-                        hipStream_t st, bool padded, const DenseList *dl) {
-    vorticity_a_impl(c, one(c), kDefaultShape, padded, rec, nbr9, vel_sorted, vel_id, vxsph, omega, vort, pl, st, dl);
+void launch_vorticity_a(const SolverConsts &c, const float4 *rec, const uint2 *q, const int32_t *nbr9,
+                        const float4 *vel_sorted, const float4 *vel_id, float4 *vxsph, float4 *omega, float *vort,
+                        const PairList &pl, hipStream_t st, bool padded, const DenseList *dl) {
+    vorticity_a_impl(c, one(c), kDefaultShape, padded, rec, q, nbr9, vel_sorted, vel_id, vxsph, omega, vort, pl, st,
+                     dl);
+}
+void launch_quantize(const SolverConsts &c, const float4 *rec, uint2 *q, int32_t count, hipStream_t st) {
+    if (count > 0) k_quantize<<<blocks_for(count), BLOCK, 0, st>>>(c.qscale, rec, q, count);
 }
 void launch_vorticity_b(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, const PairList &pl,
                         const float4 *vxsph, const float4 *omega, const float *vort, float4 *vel_id, hipStream_t st,
@@ -1655,22 +1706,24 @@ void launch_neighbourcells(const SolverConsts &c, const DomMany &d, const Launch
     neighbourcells_impl(c, d, ls, sorted, start, end64, nbr9, st, dl);
 }
 void launch_lambda(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec,
-                   const int32_t *nbr9, float *lambda, const PairList &pl, hipStream_t st, const DenseList *dl) {
-    lambda_impl(c, d, ls, false, rec, nbr9, lambda, pl, st, dl);
+                   const uint2 *q, const int32_t *nbr9, float *lambda, const PairList &pl, hipStream_t st,
+                   const DenseList *dl) {
+    lambda_impl(c, d, ls, false, rec, q, nbr9, lambda, pl, st, dl);
 }
 void launch_updatepos(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec_in,
-                      const float *lambda, const int32_t *nbr9, const PairList &pl, float4 *rec_out,
+                      const float *lambda, const int32_t *nbr9, const PairList &pl, float4 *rec_out, uint2 *q_out,
                       hipStream_t st, const DenseList *dl, const StepUpdate *up) {
-    updatepos_impl(c, d, ls, false, rec_in, lambda, nbr9, pl, rec_out, st, dl, up);
+    updatepos_impl(c, d, ls, false, rec_in, lambda, nbr9, pl, rec_out, q_out, st, dl, up);
 }
 void launch_update(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec, float4 *pos,
                    float4 *vel, float4 *vel_sorted, hipStream_t st) {
     update_impl(c, d, ls, rec, pos, vel, vel_sorted, st);
 }
 void launch_vorticity_a(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec,
-                        const int32_t *nbr9, const float4 *vel_sorted, const float4 *vel_id, float4 *vxsph,
-                        float4 *omega, float *vort, const PairList &pl, hipStream_t st, const DenseList *dl) {
-    vorticity_a_impl(c, d, ls, false, rec, nbr9, vel_sorted, vel_id, vxsph, omega, vort, pl, st, dl);
+                        const uint2 *q, const int32_t *nbr9, const float4 *vel_sorted, const float4 *vel_id,
+                        float4 *vxsph, float4 *omega, float *vort, const PairList &pl, hipStream_t st,
+                        const DenseList *dl) {
+    vorticity_a_impl(c, d, ls, false, rec, q, nbr9, vel_sorted, vel_id, vxsph, omega, vort, pl, st, dl);
 }
 void launch_vorticity_b(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec,
                         const int32_t *nbr9, const PairList &pl, const float4 *vxsph, const float4 *omega,
