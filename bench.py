@@ -13,6 +13,9 @@ Prints one JSON line on stdout (everything else goes to stderr):
                launch time (HIP events on the library's stream), against 8 TB/s
   cpu_baseline the oracle (C restatement, OpenMP) timed on this host for a
                bounded sample of the same workload
+  cpu_reference the reference's own CPU path (its GLSL shaders on Mesa
+               llvmpipe), as recorded in tests/golden/llvmpipe_timing.json on
+               the machine that made the fixtures (not timed by this run)
 """
 from __future__ import annotations
 
@@ -50,12 +53,12 @@ ALG_BYTES = {
 # last updatepos launch of each step also performs the closing update (pos and
 # vel by particle id), so updatepos is the mean over launches with and without.
 MEASURED_TRAFFIC_BLOCK_1M = {
-    "lambda": 150.0,
-    "updatepos": 218.1,
-    "vorticity_a": 223.3,
-    "vorticity_b": 239.6,
-    "neighbourcells": 63.7,
-    "predict": 52.0,
+    "lambda": 166.7,
+    "updatepos": 226.5,
+    "vorticity_a": 245.2,
+    "vorticity_b": 239.9,
+    "neighbourcells": 71.0,
+    "predict": 60.0,
 }
 
 
@@ -88,6 +91,24 @@ def cpu_baseline(scene, steps: int):
             "sample": f"the first {steps} step(s) of the {scene.name} rollout ({scene.n} particles, "
                       f"{scene.params.iterations} iterations), oracle/pbf_oracle.c FP32 race-free order, "
                       f"{O.num_threads()} OpenMP threads, {dt:.2f} s"}
+
+
+def cpu_reference(scene):
+    """The reference's shaders (calclambda.glsl and the rest of SPH::Run) run
+    by Mesa's llvmpipe CPU driver through oracle/glsl_ref, timed when the golden
+    fixtures were made (tests/golden/llvmpipe_timing.py); a recorded figure, for
+    comparison beside cpu_baseline."""
+    path = os.path.join(ROOT, "tests", "golden", "llvmpipe_timing.json")
+    with open(path) as f:
+        rec = json.load(f)
+    w = rec["workloads"].get(scene.name)
+    if not w:
+        return None
+    return {"value": w["particle_steps_per_s"], "unit": "particle-steps/s", "cores": rec["threads"],
+            "kind": "reference",
+            "sample": f"{w['timed_steps']} timed steps of the {scene.name} rollout after {rec['warmup_steps']} "
+                      f"warm-up steps, the reference's GLSL shaders on {rec['driver']}, {rec['threads']} threads, "
+                      f"recorded in tests/golden/llvmpipe_timing.json (the fixture machine, not this host)"}
 
 
 def bench_batched(steps: int, tune_steps: int):
@@ -364,6 +385,7 @@ def main():
         "step_roofline": {"alg_bytes_per_particle_step": b_alg, "frac": round(step_frac, 5)},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "cpu_reference": cpu_reference(scene),
         "batched": batched,
         "adjoint": adjoint,
     }

@@ -209,6 +209,74 @@ def test_violent_rollout(gpu_lib):
     assert_bitexact(vel, wvel, "violent velocities")
 
 
+def isolated_pairs(seed=91):
+    """3,072 particles in 1,536 pairs whose members are about h apart, each pair more than 2h
+    from every other. Pair midpoints lie on a 4h lattice that includes the multiples of 16h in
+    every axis, so many pairs straddle one; separations h (1 - 10^U(-7, -2)) for three pairs in
+    four, h (1 + 10^U(-7, -2)) for the rest; random orientations and velocities."""
+    rng = np.random.default_rng(seed)
+    mx = np.arange(20.0, 77.0, 4.0)
+    my = np.arange(4.0, 29.0, 4.0)
+    mid = np.stack(np.meshgrid(mx, my, mx, indexing="ij"), axis=-1).reshape(-1, 3)[:1536]
+    u = rng.normal(size=(1536, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    d = 10.0 ** rng.uniform(-7, -2, 1536)
+    r = np.where(np.arange(1536) % 4 == 3, 1.0 + d, 1.0 - d)
+    pos = np.zeros((3072, 4), np.float32)
+    pos[0::2, :3] = mid - 0.5 * r[:, None] * u
+    pos[1::2, :3] = mid + 0.5 * r[:, None] * u
+    vel = np.zeros_like(pos)
+    vel[:, :3] = rng.normal(size=(3072, 3))
+    return S.Scene("pairs", (96, 32, 96), S.config_params(1.0, 0.008, 4), pos, vel, 1)
+
+
+def test_candidate_filter_keeps_pairs_at_h(gpu_lib):
+    """The candidate scans test distances on quantized positions before the exact test. A pair
+    it dropped would add nothing, so the scene is isolated pairs at separations next to h,
+    straddling the 16h period of the quantized coordinates: each particle's |omega| is nonzero
+    exactly when its partner is closer than h (fl(sqrt(sq)) < h, counted here in float32 the
+    reference's way), and lambda and the vorticity stage equal the oracle bit for bit."""
+    sc = isolated_pairs()
+    n = sc.n
+    cfg = O.cfg_for(sc)
+    pp = P.scene_params(sc)
+    xs = O.predict(cfg, sc.pos, sc.vel)
+    rec, _ = O.sort(cfg, xs)
+    start, end = O.findcells(cfg, rec, np.zeros(O.cells(cfg), np.int32))
+    nbr = O.neighbourcells(cfg, rec, start, end)
+    say(f"[pairs at h] n={n}")
+    st = U.stream()
+    drec, dnbr = U.dev(rec), U.dev(nbr)
+    lam = U.empty_like_host((n,), np.float32)
+    P.stage_lambda(n, sc.grid, pp, U.ptr(drec), U.ptr(dnbr), U.ptr(lam), st)
+    assert_bitexact(U.host(lam), O.lambdas(cfg, rec, nbr), "lambda of the pairs")
+    dvel = U.dev(sc.vel)
+    vort = U.empty_like_host((n,), np.float32)
+    P.stage_vorticity(n, sc.grid, pp, U.ptr(drec), U.ptr(dnbr), U.ptr(dvel), U.ptr(vort), st)
+    wv, wo = O.vorticity(cfg, rec, nbr, sc.vel, np.zeros(n, np.float32), O.ORDER_JACOBI)
+    got = U.host(vort)
+    assert_bitexact(U.host(dvel), wv, "velocities of the pairs")
+    assert_bitexact(got, wo, "|omega| of the pairs")
+    # partner of each sorted slot, and the reference's range test in float32
+    ids = rec[:, 3].view(np.int32)
+    slot = np.empty(n, np.int64)
+    slot[ids] = np.arange(n)
+    partner = slot[ids ^ 1]
+    d = rec[:, :3] - rec[partner, :3]
+    sq = (d[:, 2] * d[:, 2] + d[:, 1] * d[:, 1]) + d[:, 0] * d[:, 0]
+    # a partner at fl(sqrt(sq)) == h is in range but adds exact zeros (h - r = 0)
+    within = np.sqrt(sq) < np.float32(1.0)
+    # the partner among the particle's candidates: inside one of its nine row ranges (the
+    # reference's first and last cells are not: findcells.glsl writes no start for the cell of
+    # sorted slot 0 and no end for the last one)
+    rows = nbr.reshape(n, 12)[:, [0, 1, 2, 4, 5, 6, 8, 9, 10]]
+    rs, rc = rows & 0xFFFFFF, rows >> 24
+    listed = ((rs <= partner[:, None]) & (partner[:, None] < rs + rc)).any(axis=1)
+    assert (~listed).sum() <= 8, f"{(~listed).sum()} partners outside the rows"
+    assert np.array_equal(got != 0.0, within & listed), "a pair within h was dropped or one beyond h was kept"
+    assert within.mean() > 0.4 and (~within).mean() > 0.2, f"{within.mean():.2f} of the particles have a partner"
+
+
 @pytest.mark.parametrize("index", [0, 97, 200])
 def test_rotated_gravity_one_cell_walls_rollout(gpu_lib, index):
     """The plain handle on scenes of the batched workload: gravity turned off the y axis, walls one
