@@ -22,7 +22,7 @@
 // float (the production gradient) or double (the finite-difference check).
 // Forward states are checkpointed every K steps; the backward pass recomputes
 // each segment's steps from its checkpoint into a tape, then runs the segment
-// in reverse.
+// in reverse. The last segment is taped by the forward pass itself.
 #include <cstring>
 #include <type_traits>
 #include <new>
@@ -213,50 +213,67 @@ __device__ __forceinline__ void each_candidate(const int32_t *__restrict__ nbr9,
     }
 }
 
-// The reverse kernels run one lane per (particle, row) pair: thread t of the
-// launch takes row t / n of particle t % n (consecutive lanes, consecutive
-// particles of one row). The order of their scattered sums does not matter.
-template <class T, class F>
-__device__ __forceinline__ void row_candidates(const int32_t *__restrict__ nbr9, int32_t n, int32_t i, int32_t o,
-                                               const T4<T> *__restrict__ p, F &&body) {
-    const int32_t d = nbr9[(size_t)o * n + i];
-    const int32_t s0 = d & 0xFFFFFF, e = d >> 24;
-    const int32_t last = max(min(s0 + e, n) - 1, 0);
-    for (int32_t q = 0; q < e; q += kAdjBatch) {
+// The reverse kernels run one lane per particle over its pair terms: the
+// recorded pair list (the neighbours within h of the forward, in list order),
+// or without one (none recorded, or more than the list holds) the candidates
+// of its rows. extra(j) loads the neighbour's operands of the kernel with the
+// batch's positions, so kAdjBatch neighbours are in flight together.
+template <class T, class X, class E, class F>
+__device__ __forceinline__ void each_pair(const uint16_t *__restrict__ ents, const uint16_t *__restrict__ cnts,
+                                          const int32_t *__restrict__ nbr9, int32_t n, int32_t i,
+                                          const T4<T> *__restrict__ p, E &&extra, F &&body) {
+    const uint32_t cnt = cnts ? cnts[i] : kListNone;
+    if (cnt == kListNone) {
+        for (int o = 0; o < 9; o++) {
+            const int32_t d = nbr9[(size_t)o * n + i];
+            const int32_t s0 = d & 0xFFFFFF, e = d >> 24;
+            for (int32_t q = 0; q < e; q += kAdjBatch) {
+                int32_t j[kAdjBatch];
+                T4<T> pj[kAdjBatch];
+                X xj[kAdjBatch];
+#pragma unroll
+                for (int u = 0; u < kAdjBatch; u++) {
+                    j[u] = s0 + q + min(u, e - 1 - q);
+                    pj[u] = load_nb(p, j[u], n);
+                    xj[u] = extra(j[u]);
+                }
+#pragma unroll
+                for (int u = 0; u < kAdjBatch; u++)
+                    if (q + u < e && j[u] != i) body(j[u], pj[u], xj[u]);
+            }
+        }
+        return;
+    }
+    for (uint32_t e0 = 0; e0 < cnt; e0 += kAdjBatch) {
+        int32_t j[kAdjBatch];
         T4<T> pj[kAdjBatch];
+        X xj[kAdjBatch];
 #pragma unroll
         for (int u = 0; u < kAdjBatch; u++) {
-            const int32_t j = s0 + q + u;
-            pj[u] = p[min(j, last)];
-            if (j >= n) pj[u] = T4<T>{0, 0, 0, 0};
+            const uint32_t code = ents[(size_t)min(e0 + u, cnt - 1) * n + i];
+            j[u] = (nbr9[(size_t)(code >> 8) * n + i] & 0xFFFFFF) + (int32_t)(code & 255u);
         }
 #pragma unroll
         for (int u = 0; u < kAdjBatch; u++) {
-            const int32_t j = s0 + q + u;
-            if (q + u < e && j != i) body(j, pj[u]);
+            pj[u] = load_nb(p, j[u], n);
+            xj[u] = extra(j[u]);
         }
+#pragma unroll
+        for (int u = 0; u < kAdjBatch; u++)
+            if (e0 + u < cnt) body(j[u], pj[u], xj[u]);
     }
 }
 
-// The pair terms of particle i handled by lane plane k (0..8) of a reverse
-// kernel: with a recorded pair list (the neighbours within h of the forward,
-// in list order), entries k, k + 9, ... of it; without one (none recorded, or
-// the particle had more than the list holds), the candidates of row k.
-template <class T, class F>
-__device__ __forceinline__ void plane_pairs(const uint16_t *__restrict__ ents, const uint16_t *__restrict__ cnts,
-                                            const int32_t *__restrict__ nbr9, int32_t n, int32_t i, int32_t k,
-                                            const T4<T> *__restrict__ p, F &&body) {
-    const uint32_t cnt = cnts ? cnts[i] : kListNone;
-    if (cnt == kListNone) {
-        row_candidates(nbr9, n, i, k, p, body);
-        return;
-    }
-    for (uint32_t e = (uint32_t)k; e < cnt; e += 9) {
-        const uint32_t code = ents[(size_t)e * n + i];
-        const int32_t j = (nbr9[(size_t)(code >> 8) * n + i] & 0xFFFFFF) + (int32_t)(code & 255u);
-        body(j, load_nb(p, j, n));
-    }
-}
+// Pair terms without atomics. The neighbour side of a pair term (what lane i
+// would add to particle j) is, for every pair of these kernels, the same
+// expression as j's own side of the reversed pair with the operands swapped:
+// when j lists i whenever i lists j, lane i can gather the neighbour sides
+// meant for it from its own list instead of receiving them by atomic adds.
+// That holds for two particles whose nine rows are exactly the union of their
+// 27 cells' contents (proper, ka_proper): the range test is symmetric (the
+// squared distance has the same bits both ways) and cell adjacency is. A pair
+// with an improper particle (the reference's quirk cells, wrapped counts,
+// rows at the grid's faces) keeps the scattered form in both directions.
 
 __device__ __forceinline__ void block_add2(double a, double b, double *acc) {
     __shared__ double s[2][AB];
@@ -295,6 +312,55 @@ __global__ __launch_bounds__(AB) void ka_predict(AConsts<T> c, const T4<T> *__re
     const T a = p.x + c.dt * vx, b = p.y + c.dt * vy, d = p.z + c.dt * vz;
     xs[i] = T4<T>{a, b, d, 0};
     xs_f[i] = make_float4((float)a, (float)b, (float)d, __int_as_float(i));
+}
+
+// True cell ranges of the sorted keys, without the reference's table quirks:
+// tstart[c] and tend[c] of every occupied cell c (index x + gx (y + gy z),
+// decoded from the key x + gx z + gx gz y). Keys are exact cell hashes here
+// (the grid has fewer than 2^24 cells); tstart is -1 elsewhere.
+__global__ __launch_bounds__(AB) void ka_true_cells(int32_t n, int32_t gx, int32_t gy, int32_t gz,
+                                                    const uint32_t *__restrict__ skeys, int32_t *__restrict__ tstart,
+                                                    int32_t *__restrict__ tend) {
+    const int32_t g = blockIdx.x * AB + threadIdx.x;
+    if (g >= n) return;
+    const uint32_t k = skeys[g];
+    const uint32_t x = k % (uint32_t)gx, t = k / (uint32_t)gx, z = t % (uint32_t)gz, y = t / (uint32_t)gz;
+    if (y >= (uint32_t)gy) return;
+    const size_t c = (size_t)x + (size_t)gx * ((size_t)y + (size_t)gy * z);
+    if (g == 0 || skeys[g - 1] != k) tstart[c] = g;
+    if (g == n - 1 || skeys[g + 1] != k) tend[c] = g + 1;
+}
+
+// proper[i]: particle i (sorted slot, record rec[i]) lies in the cell its key
+// names, its 27 cells are inside the grid, and each of its nine row words is
+// the one the true cell ranges give (first start, total count below 128), so
+// its candidates are exactly the particles of its 27 cells.
+__global__ __launch_bounds__(AB) void ka_proper(int32_t n, int32_t gx, int32_t gy, int32_t gz, int32_t exact,
+                                                const float4 *__restrict__ rec, const uint32_t *__restrict__ skeys,
+                                                const int32_t *__restrict__ tstart, const int32_t *__restrict__ tend,
+                                                const int32_t *__restrict__ nbr9, uint8_t *__restrict__ proper) {
+    const int32_t i = blockIdx.x * AB + threadIdx.x;
+    if (i >= n) return;
+    const float4 r = rec[i];
+    bool ok = exact != 0 && r.x >= 1.0f && r.y >= 1.0f && r.z >= 1.0f && r.x < (float)(gx - 1) &&
+              r.y < (float)(gy - 1) && r.z < (float)(gz - 1);
+    if (ok) {
+        const int32_t px = (int32_t)r.x, py = (int32_t)r.y, pz = (int32_t)r.z;  // as the neighbour-cell pass
+        ok = skeys[i] == (uint32_t)px + (uint32_t)gx * ((uint32_t)pz + (uint32_t)gz * (uint32_t)py);
+        for (int o = 0; o < 9 && ok; o++) {
+            const int32_t y = py + o / 3 - 1, z = pz + o % 3 - 1;
+            int32_t first = -1, cnt = 0;
+            for (int dx = -1; dx <= 1; dx++) {
+                const size_t c = (size_t)(px + dx) + (size_t)gx * ((size_t)y + (size_t)gy * (size_t)z);
+                const int32_t s0 = tstart[c];
+                if (s0 == -1) continue;
+                if (first == -1) first = s0;
+                cnt += tend[c] - s0;
+            }
+            ok = cnt < 128 && nbr9[(size_t)o * n + i] == (int32_t)((uint32_t)first + ((uint32_t)cnt << 24));
+        }
+    }
+    proper[i] = ok ? 1 : 0;
 }
 
 template <class T>
@@ -483,22 +549,18 @@ __global__ __launch_bounds__(AB) void ka_vort_b(AConsts<T> c, const uint32_t *__
 }
 
 // ------------------------------------------------------------------ reverse
-// vorticity pass B reversed: from vbar (by id) to vxbar, ombar (own part),
-// obar and pbar (scattered), and d/d eps_v; eta from the forward's tape.
+// vorticity pass B reversed, per particle: from vbar (by id) to vxbar, the
+// own part of ombar, d/d eps_v and etabar = d/d eta (eta from the forward's
+// tape), for the pair pass that follows.
 template <class T>
-__global__ __launch_bounds__(AB) void kr_vort_b(AConsts<T> c, const uint32_t *__restrict__ perm,
-                                                const T4<T> *__restrict__ p, const int32_t *__restrict__ nbr9,
-                                                const uint16_t *__restrict__ ents, const uint16_t *__restrict__ cnts,
-                                                const T4<T> *__restrict__ om, const T *__restrict__ o,
-                                                const T4<T> *__restrict__ eta, const T4<T> *__restrict__ vbar,
-                                                T4<T> *__restrict__ vxbar, T4<T> *__restrict__ ombar,
-                                                T *__restrict__ obar, T4<T> *__restrict__ pbar,
-                                                double *__restrict__ acc) {
-    const int64_t tid = (int64_t)blockIdx.x * AB + threadIdx.x;
-    const int32_t row = (int32_t)(tid / c.n), i = (int32_t)(tid - (int64_t)row * c.n);
+__global__ __launch_bounds__(AB) void kr_vort_b_own(AConsts<T> c, const uint32_t *__restrict__ perm,
+                                                    const T4<T> *__restrict__ om, const T4<T> *__restrict__ eta,
+                                                    const T4<T> *__restrict__ vbar, T4<T> *__restrict__ vxbar,
+                                                    T4<T> *__restrict__ ombar, T4<T> *__restrict__ etabar,
+                                                    double *__restrict__ acc) {
+    const int32_t i = blockIdx.x * AB + threadIdx.x;
     double deps = 0.0;
-    if (row < 9) {
-        const T4<T> pi = p[i];
+    if (i < c.n) {
         const T4<T> et = eta[i];
         const T l = sqrt(sqlen(et.x, et.y, et.z));
         T Nx = 0, Ny = 0, Nz = 0;
@@ -510,40 +572,79 @@ __global__ __launch_bounds__(AB) void kr_vort_b(AConsts<T> c, const uint32_t *__
         const T4<T> w = om[i];
         const T4<T> a = vbar[perm[i]];
         const T sc = c.dt * c.vort_eps;
-        if (row == 0) {
-            vxbar[i] = T4<T>{a.x, a.y, a.z, 0};
-            // d/d eps_v of a . (dt eps (N x w))
-            const T cx = Ny * w.z - Nz * w.y, cy = Nz * w.x - Nx * w.z, cz = Nx * w.y - Ny * w.x;
-            deps = (double)(c.dt * (a.x * cx + a.y * cy + a.z * cz));
-            // a . (N x w) = w . (a x N)
-            ombar[i] = T4<T>{sc * (a.y * Nz - a.z * Ny), sc * (a.z * Nx - a.x * Nz), sc * (a.x * Ny - a.y * Nx), 0};
-        }
+        vxbar[i] = T4<T>{a.x, a.y, a.z, 0};
+        // d/d eps_v of a . (dt eps (N x w))
+        const T cx = Ny * w.z - Nz * w.y, cy = Nz * w.x - Nx * w.z, cz = Nx * w.y - Ny * w.x;
+        deps = (double)(c.dt * (a.x * cx + a.y * cy + a.z * cz));
+        // a . (N x w) = w . (a x N)
+        ombar[i] = T4<T>{sc * (a.y * Nz - a.z * Ny), sc * (a.z * Nx - a.x * Nz), sc * (a.x * Ny - a.y * Nx), 0};
         // a . (N x w) = N . (w x a); N = eta / |eta|: etabar = (Nbar - N (N . Nbar)) / |eta|
+        T4<T> eb{0, 0, 0, 0};
         if (l > (T)0) {
             const T Nbx = sc * (w.y * a.z - w.z * a.y), Nby = sc * (w.z * a.x - w.x * a.z),
                     Nbz = sc * (w.x * a.y - w.y * a.x);
             const T nn = Nx * Nbx + Ny * Nby + Nz * Nbz;
-            const T ebx = (Nbx - Nx * nn) / l, eby = (Nby - Ny * nn) / l, ebz = (Nbz - Nz * nn) / l;
-            T pbx = 0, pby = 0, pbz = 0;
-            plane_pairs(ents, cnts, nbr9, c.n, i, row, p, [&](int32_t j, const T4<T> &pj) {
-                const T ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
-                const T sq = sqlen(ex, ey, ez);
-                const T r = sqrt(sq);
-                if (r > c.h || r == (T)0) return;
-                const T s = gspiky(c, r);
-                if (j < c.n) atomicAdd(&obar[j], s * (ex * ebx + ey * eby + ez * ebz));
-                const T oj = j < c.n ? o[j] : (T)0;
-                T jx, jy, jz;
-                jspiky(c, r, ex, ey, ez, ebx, eby, ebz, jx, jy, jz);
-                pbx += oj * jx;
-                pby += oj * jy;
-                pbz += oj * jz;
-                if (j < c.n) atomic_add3(pbar, j, -oj * jx, -oj * jy, -oj * jz);
-            });
-            atomic_add3(pbar, i, pbx, pby, pbz);
+            eb = T4<T>{(Nbx - Nx * nn) / l, (Nby - Ny * nn) / l, (Nbz - Nz * nn) / l, 0};
         }
+        etabar[i] = eb;
     }
     block_add2(0.0, deps, acc);
+}
+
+template <class T>
+struct VortBX {
+    T4<T> eb;
+    T o;
+    uint8_t proper;
+};
+
+// eta_i = sum_j o_j gradW_ij reversed: obar and pbar. Own side of pair (i, j):
+// obar_j += s (d . eb_i), pbar_i += o_j J eb_i, pbar_j -= o_j J eb_i (d = p_i - p_j,
+// J the Jacobian of gradW, even in d). Gathered from (j, i): obar_i -= s (d . eb_j),
+// pbar_i -= o_i J eb_j.
+template <class T>
+__global__ __launch_bounds__(AB) void kr_vort_b(AConsts<T> c, const T4<T> *__restrict__ p,
+                                                const int32_t *__restrict__ nbr9, const uint16_t *__restrict__ ents,
+                                                const uint16_t *__restrict__ cnts, const uint8_t *__restrict__ proper,
+                                                const T *__restrict__ o, const T4<T> *__restrict__ etabar,
+                                                T *__restrict__ obar, T4<T> *__restrict__ pbar) {
+    const int32_t i = blockIdx.x * AB + threadIdx.x;
+    if (i >= c.n) return;
+    const T4<T> pi = p[i];
+    const T4<T> eb = etabar[i];
+    const T oi = o[i];
+    const bool pri = proper[i] != 0;
+    T ob = 0, pbx = 0, pby = 0, pbz = 0;
+    each_pair<T, VortBX<T>>(
+        ents, cnts, nbr9, c.n, i, p,
+        [&](int32_t j) {
+            return j < c.n ? VortBX<T>{etabar[j], o[j], proper[j]} : VortBX<T>{T4<T>{0, 0, 0, 0}, (T)0, (uint8_t)0};
+        },
+        [&](int32_t j, const T4<T> &pj, const VortBX<T> &xj) {
+            const T ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
+            const T sq = sqlen(ex, ey, ez);
+            const T r = sqrt(sq);
+            if (r > c.h || r == (T)0) return;
+            const T s = gspiky(c, r);
+            T jx, jy, jz;
+            if (pri && xj.proper) {
+                ob -= s * (ex * xj.eb.x + ey * xj.eb.y + ez * xj.eb.z);
+                jspiky(c, r, ex, ey, ez, xj.o * eb.x - oi * xj.eb.x, xj.o * eb.y - oi * xj.eb.y,
+                       xj.o * eb.z - oi * xj.eb.z, jx, jy, jz);
+                pbx += jx;
+                pby += jy;
+                pbz += jz;
+                return;
+            }
+            if (j < c.n) atomicAdd(&obar[j], s * (ex * eb.x + ey * eb.y + ez * eb.z));
+            jspiky(c, r, ex, ey, ez, eb.x, eb.y, eb.z, jx, jy, jz);
+            pbx += xj.o * jx;
+            pby += xj.o * jy;
+            pbz += xj.o * jz;
+            if (j < c.n) atomic_add3(pbar, j, -xj.o * jx, -xj.o * jy, -xj.o * jz);
+        });
+    if (ob != (T)0) atomicAdd(&obar[i], ob);
+    atomic_add3(pbar, i, pbx, pby, pbz);
 }
 
 // |omega| reversed: ombar += obar omega / |omega|.
@@ -564,69 +665,96 @@ __global__ __launch_bounds__(AB) void kr_omega_norm(int32_t n, const T4<T> *__re
     }
 }
 
-// vorticity pass A reversed: from vxbar and ombar to vsbar and pbar (both
-// scattered), and d/d xsph_c.
+template <class T>
+struct VortAX {
+    T4<T> v, a, wb;
+    uint8_t proper;
+};
+
+// vorticity pass A reversed: from vxbar and ombar to vsbar and pbar, and
+// d/d xsph_c. Own side of pair (i, j), u = v_j - v_i, g = gradW_ij:
+// ub = c W a_i + g x wb_i goes to vsbar_j (and -ub to vsbar_i); the position
+// term d = c dW/dd (u . a_i) + J (wb_i x u) to pbar_i (and -d to pbar_j).
+// Gathered from (j, i) (u, d and g change sign, W and J do not): vsbar_i gets
+// c W a_j - g x wb_j, pbar_i gets -(c dW/dd (u . a_j) - J (wb_j x u)).
 template <class T>
 __global__ __launch_bounds__(AB) void kr_vort_a(AConsts<T> c, const T4<T> *__restrict__ p,
                                                 const int32_t *__restrict__ nbr9, const uint16_t *__restrict__ ents,
-                                                const uint16_t *__restrict__ cnts, const T4<T> *__restrict__ vs,
-                                                const T4<T> *__restrict__ vxbar, const T4<T> *__restrict__ ombar,
-                                                T4<T> *__restrict__ vsbar, T4<T> *__restrict__ pbar,
-                                                double *__restrict__ acc) {
-    const int64_t tid = (int64_t)blockIdx.x * AB + threadIdx.x;
-    const int32_t row = (int32_t)(tid / c.n), i = (int32_t)(tid - (int64_t)row * c.n);
+                                                const uint16_t *__restrict__ cnts, const uint8_t *__restrict__ proper,
+                                                const T4<T> *__restrict__ vs, const T4<T> *__restrict__ vxbar,
+                                                const T4<T> *__restrict__ ombar, T4<T> *__restrict__ vsbar,
+                                                T4<T> *__restrict__ pbar, double *__restrict__ acc) {
+    const int32_t i = blockIdx.x * AB + threadIdx.x;
     double dc = 0.0;
-    if (row < 9) {
+    if (i < c.n) {
         const T4<T> pi = p[i], vi = vs[i], a = vxbar[i], wb = ombar[i];
+        const bool pri = proper[i] != 0;
         T sx = 0, sy = 0, sz = 0;
-        T vbx = row == 0 ? a.x : (T)0, vby = row == 0 ? a.y : (T)0, vbz = row == 0 ? a.z : (T)0;  // vx = v + ...
+        T vbx = a.x, vby = a.y, vbz = a.z;  // vx = v + ...
         T pbx = 0, pby = 0, pbz = 0;
-        plane_pairs(ents, cnts, nbr9, c.n, i, row, p, [&](int32_t j, const T4<T> &pj) {
-            const T ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
-            const T sq = sqlen(ex, ey, ez);
-            const T r = sqrt(sq);
-            if (r > c.h) return;
-            if (j >= c.n) return;  // zero record: its velocity is a constant here
-            const T4<T> vj = vs[j];
-            const T ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
-            const T W = wpoly(c, sq);
-            sx += ux * W;
-            sy += uy * W;
-            sz += uz * W;
-            // ubar = c W a + gradW x ombar
-            T ubx = c.xsph_c * W * a.x, uby = c.xsph_c * W * a.y, ubz = c.xsph_c * W * a.z;
-            // d(c W (u . a))/dd
-            const T dw = dwpoly(c, sq) * c.xsph_c * (ux * a.x + uy * a.y + uz * a.z);
-            T dx = dw * ex, dy = dw * ey, dz = dw * ez;
-            if (r > (T)0) {
-                const T s = gspiky(c, r);
-                const T gx = s * ex, gy = s * ey, gz = s * ez;
-                ubx += gy * wb.z - gz * wb.y;
-                uby += gz * wb.x - gx * wb.z;
-                ubz += gx * wb.y - gy * wb.x;
-                // ombar . (u x gradW) = gradW . (ombar x u)
-                const T qx = wb.y * uz - wb.z * uy, qy = wb.z * ux - wb.x * uz, qz = wb.x * uy - wb.y * ux;
-                T jx, jy, jz;
-                jspiky(c, r, ex, ey, ez, qx, qy, qz, jx, jy, jz);
-                dx += jx;
-                dy += jy;
-                dz += jz;
-            }
-            atomic_add3(vsbar, j, ubx, uby, ubz);
-            vbx -= ubx;
-            vby -= uby;
-            vbz -= ubz;
-            pbx += dx;
-            pby += dy;
-            pbz += dz;
-            atomic_add3(pbar, j, -dx, -dy, -dz);
-        });
+        each_pair<T, VortAX<T>>(
+            ents, cnts, nbr9, c.n, i, p,
+            [&](int32_t j) {
+                const T4<T> z{0, 0, 0, 0};
+                return j < c.n ? VortAX<T>{vs[j], vxbar[j], ombar[j], proper[j]} : VortAX<T>{z, z, z, (uint8_t)0};
+            },
+            [&](int32_t j, const T4<T> &pj, const VortAX<T> &xj) {
+                const T ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
+                const T sq = sqlen(ex, ey, ez);
+                const T r = sqrt(sq);
+                if (r > c.h) return;
+                if (j >= c.n) return;  // zero record: its velocity is a constant here
+                const T ux = xj.v.x - vi.x, uy = xj.v.y - vi.y, uz = xj.v.z - vi.z;
+                const T W = wpoly(c, sq);
+                sx += ux * W;
+                sy += uy * W;
+                sz += uz * W;
+                const bool sym = pri && xj.proper;
+                // the a and wb of the terms: own a_i, wb_i; with the gathered side a_i - a_j, wb_i + wb_j
+                const T dax = sym ? a.x - xj.a.x : a.x, day = sym ? a.y - xj.a.y : a.y, daz = sym ? a.z - xj.a.z : a.z;
+                const T wx = sym ? wb.x + xj.wb.x : wb.x, wy = sym ? wb.y + xj.wb.y : wb.y,
+                        wz = sym ? wb.z + xj.wb.z : wb.z;
+                // ub = c W a + g x wb, for vsbar (own side negated on i)
+                T ubx = c.xsph_c * W * dax, uby = c.xsph_c * W * day, ubz = c.xsph_c * W * daz;
+                const T dw = dwpoly(c, sq) * c.xsph_c * (ux * dax + uy * day + uz * daz);
+                T dx = dw * ex, dy = dw * ey, dz = dw * ez;
+                if (r > (T)0) {
+                    const T s = gspiky(c, r);
+                    const T gx = s * ex, gy = s * ey, gz = s * ez;
+                    ubx += gy * wz - gz * wy;
+                    uby += gz * wx - gx * wz;
+                    ubz += gx * wy - gy * wx;
+                    // wb . (u x gradW) = gradW . (wb x u)
+                    const T qx = wy * uz - wz * uy, qy = wz * ux - wx * uz, qz = wx * uy - wy * ux;
+                    T jx, jy, jz;
+                    jspiky(c, r, ex, ey, ez, qx, qy, qz, jx, jy, jz);
+                    dx += jx;
+                    dy += jy;
+                    dz += jz;
+                }
+                pbx += dx;
+                pby += dy;
+                pbz += dz;
+                if (sym) {
+                    // -(c W a_i + g x wb_i) + (c W a_j - g x wb_j)
+                    vbx -= ubx;
+                    vby -= uby;
+                    vbz -= ubz;
+                    return;
+                }
+                atomic_add3(vsbar, j, ubx, uby, ubz);
+                vbx -= ubx;
+                vby -= uby;
+                vbz -= ubz;
+                atomic_add3(pbar, j, -dx, -dy, -dz);
+            });
         atomic_add3(vsbar, i, vbx, vby, vbz);
         atomic_add3(pbar, i, pbx, pby, pbz);
-        dc = (double)(a.x * sx + a.y * sy + a.z * sz);  // this row's share of a . sum u W
+        dc = (double)(a.x * sx + a.y * sy + a.z * sz);  // a . sum u W
     }
     block_add2(dc, 0.0, acc);
 }
+
 
 // update reversed: pbar_I += xbar[id] + vsbar/dt ; xbar_old[id] = -vsbar/dt.
 // Without vorticity vsbar is the incoming vbar[id].
@@ -648,102 +776,149 @@ __global__ __launch_bounds__(AB) void kr_update(AConsts<T> c, const uint32_t *__
     xbar_old[id] = T4<T>{-vb.x / c.dt, -vb.y / c.dt, -vb.z / c.dt, 0};
 }
 
-// position update reversed: pbar_prev = mask pbar_next + pair terms;
-// lambar scattered.
+template <class T>
+struct UpdX {
+    T4<T> b;
+    T lam;
+    uint8_t mask, proper;
+};
+
+// position update reversed: pbar_prev = mask pbar_next + pair terms; lambar.
+// Own side of pair (i, j), a_i = the clamp-masked pbar_next of i:
+// gdot = (1/rho0) gradW . a_i to lambar_i and lambar_j; d = (1/rho0) f J a_i +
+// gdot ds/dd to pbar_i (-d to pbar_j), f = lam_i + lam_j + s_corr (symmetric).
+// The gathered side of (j, i) is the same with -a_j: the terms are linear in a.
 template <class T>
 __global__ __launch_bounds__(AB) void kr_updatepos(AConsts<T> c, const T4<T> *__restrict__ p,
                                                    const T *__restrict__ lam, const int32_t *__restrict__ nbr9,
                                                    const uint16_t *__restrict__ ents,
                                                    const uint16_t *__restrict__ cnts,
+                                                   const uint8_t *__restrict__ proper,
                                                    const uint8_t *__restrict__ mask, const T4<T> *__restrict__ pbn,
                                                    T4<T> *__restrict__ pbp, T *__restrict__ lambar) {
-    const int64_t tid = (int64_t)blockIdx.x * AB + threadIdx.x;
-    const int32_t row = (int32_t)(tid / c.n), i = (int32_t)(tid - (int64_t)row * c.n);
-    if (row >= 9) return;
+    const int32_t i = blockIdx.x * AB + threadIdx.x;
+    if (i >= c.n) return;
     const T4<T> pi = p[i];
     const uint8_t m = mask[i];
     const T4<T> b = pbn[i];
     const T ax = (m & 1) ? b.x : (T)0, ay = (m & 2) ? b.y : (T)0, az = (m & 4) ? b.z : (T)0;
-    T pbx = row == 0 ? ax : (T)0, pby = row == 0 ? ay : (T)0, pbz = row == 0 ? az : (T)0;  // identity part once
+    const bool pri = proper[i] != 0;
+    T pbx = ax, pby = ay, pbz = az;  // the identity part
     const T li = lam[i];
     T lb = 0;
     const T k4 = (T)-4 * c.k * c.scale * c.scale * c.scale * c.scale;
-    plane_pairs(ents, cnts, nbr9, c.n, i, row, p, [&](int32_t j, const T4<T> &pj) {
-        const T ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
-        const T sq = sqlen(ex, ey, ez);
-        const T r = sqrt(sq);
-        if (r > c.h || r == (T)0) return;
-        const T W = wpoly(c, sq);
-        T sc = c.scale * W;
-        sc *= sc;
-        sc *= sc;
-        sc = -c.k * sc;
-        const T lj = j < c.n ? lam[j] : (T)0;
-        const T f = li + lj + sc;
-        const T s = gspiky(c, r);
-        const T gdot = c.inv_rho0 * s * (ex * ax + ey * ay + ez * az);  // (1/rho0) gradW . a
-        lb += gdot;
-        if (j < c.n) atomicAdd(&lambar[j], gdot);
-        // d/dd of (1/rho0) f gradW . a: f J a + (gradW . a) ds/dd
-        T jx, jy, jz;
-        jspiky(c, r, ex, ey, ez, ax, ay, az, jx, jy, jz);
-        const T ds = k4 * W * W * W * dwpoly(c, sq);  // ds/dd = -4k scale^4 W^3 dW/dd (factor of d)
-        const T dx = c.inv_rho0 * f * jx + gdot * ds * ex;
-        const T dy = c.inv_rho0 * f * jy + gdot * ds * ey;
-        const T dz = c.inv_rho0 * f * jz + gdot * ds * ez;
-        pbx += dx;
-        pby += dy;
-        pbz += dz;
-        if (j < c.n) atomic_add3(pbp, j, -dx, -dy, -dz);
-    });
+    each_pair<T, UpdX<T>>(
+        ents, cnts, nbr9, c.n, i, p,
+        [&](int32_t j) {
+            return j < c.n ? UpdX<T>{pbn[j], lam[j], mask[j], proper[j]}
+                           : UpdX<T>{T4<T>{0, 0, 0, 0}, (T)0, (uint8_t)0, (uint8_t)0};
+        },
+        [&](int32_t j, const T4<T> &pj, const UpdX<T> &xj) {
+            const T ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
+            const T sq = sqlen(ex, ey, ez);
+            const T r = sqrt(sq);
+            if (r > c.h || r == (T)0) return;
+            const T W = wpoly(c, sq);
+            T sc = c.scale * W;
+            sc *= sc;
+            sc *= sc;
+            sc = -c.k * sc;
+            const T f = li + xj.lam + sc;
+            const T s = gspiky(c, r);
+            const bool sym = pri && xj.proper;
+            // a_i, or a_i - a_j with the gathered side
+            T vx = ax, vy = ay, vz = az;
+            if (sym) {
+                vx -= (xj.mask & 1) ? xj.b.x : (T)0;
+                vy -= (xj.mask & 2) ? xj.b.y : (T)0;
+                vz -= (xj.mask & 4) ? xj.b.z : (T)0;
+            }
+            const T gdot = c.inv_rho0 * s * (ex * vx + ey * vy + ez * vz);  // (1/rho0) gradW . v
+            lb += gdot;
+            if (!sym && j < c.n) atomicAdd(&lambar[j], gdot);
+            // d/dd of (1/rho0) f gradW . v: f J v + (gradW . v) ds/dd
+            T jx, jy, jz;
+            jspiky(c, r, ex, ey, ez, vx, vy, vz, jx, jy, jz);
+            const T ds = k4 * W * W * W * dwpoly(c, sq);  // ds/dd = -4k scale^4 W^3 dW/dd (factor of d)
+            const T dx = c.inv_rho0 * f * jx + gdot * ds * ex;
+            const T dy = c.inv_rho0 * f * jy + gdot * ds * ey;
+            const T dz = c.inv_rho0 * f * jz + gdot * ds * ez;
+            pbx += dx;
+            pby += dy;
+            pbz += dz;
+            if (!sym && j < c.n) atomic_add3(pbp, j, -dx, -dy, -dz);
+        });
     if (lb != (T)0) atomicAdd(&lambar[i], lb);
     atomic_add3(pbp, i, pbx, pby, pbz);
 }
 
-// lambda reversed: from lambar to pbar (scattered).
+template <class T>
+struct LamX {
+    T4<T> G;
+    T lb, S, lam;
+    uint8_t proper;
+};
+
+// lambda reversed: from lambar to pbar. lam = -C/S: rhobar = -lb/(rho0 S),
+// Sbar = -lam lb / S. Own side of pair (i, j): d = rb_i dW/dd + (1/rho0) J gbar_i,
+// gbar_i = 2 Sbar_i (g_ij + G_i) (g = gradW / rho0), to pbar_i (-d to pbar_j).
+// The gathered side of (j, i) has g_ji = -g_ij and dW/dd odd in d: with it,
+// pbar_i gets (rb_i + rb_j) dW/dd + (1/rho0) J (2 (Sbar_i + Sbar_j) g_ij
+// + 2 (Sbar_i G_i - Sbar_j G_j)).
 template <class T>
 __global__ __launch_bounds__(AB) void kr_lambda(AConsts<T> c, const T4<T> *__restrict__ p,
                                                 const int32_t *__restrict__ nbr9, const uint16_t *__restrict__ ents,
-                                                const uint16_t *__restrict__ cnts, const T *__restrict__ lam,
+                                                const uint16_t *__restrict__ cnts,
+                                                const uint8_t *__restrict__ proper, const T *__restrict__ lam,
                                                 const T *__restrict__ sden, const T4<T> *__restrict__ G,
                                                 const T *__restrict__ lambar, T4<T> *__restrict__ pbar) {
-    const int64_t tid = (int64_t)blockIdx.x * AB + threadIdx.x;
-    const int32_t row = (int32_t)(tid / c.n), i = (int32_t)(tid - (int64_t)row * c.n);
-    if (row >= 9) return;
+    const int32_t i = blockIdx.x * AB + threadIdx.x;
+    if (i >= c.n) return;
     const T lb = lambar[i];
-    if (lb == (T)0) return;
+    const bool pri = proper[i] != 0;
+    // with no gathered side, a particle without lambar adds nothing
+    if (lb == (T)0 && !pri) return;
     const T4<T> pi = p[i];
     const T S = sden[i];
-    // lam = -C/S: rhobar = -lb/(rho0 S), Sbar = -lam lb / S (C = -lam S)
     const T rb = -lb * c.inv_rho0 / S;
     const T sb = -lam[i] * lb / S;
     const T4<T> Gi = G[i];
     T pbx = 0, pby = 0, pbz = 0;
-    plane_pairs(ents, cnts, nbr9, c.n, i, row, p, [&](int32_t j, const T4<T> &pj) {
-        const T ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
-        const T sq = sqlen(ex, ey, ez);
-        const T r = sqrt(sq);
-        if (r > c.h) return;
-        const T dw = rb * dwpoly(c, sq);
-        T dx = dw * ex, dy = dw * ey, dz = dw * ez;
-        if (r > (T)0) {
-            const T s = gspiky(c, r);
-            // gbar = 2 Sbar (g + G), g = gradW / rho0; d/dd = J gbar / rho0
-            const T gx = 2 * sb * (s * ex * c.inv_rho0 + Gi.x), gy = 2 * sb * (s * ey * c.inv_rho0 + Gi.y),
-                    gz = 2 * sb * (s * ez * c.inv_rho0 + Gi.z);
-            T jx, jy, jz;
-            jspiky(c, r, ex, ey, ez, gx, gy, gz, jx, jy, jz);
-            dx += c.inv_rho0 * jx;
-            dy += c.inv_rho0 * jy;
-            dz += c.inv_rho0 * jz;
-        }
-        pbx += dx;
-        pby += dy;
-        pbz += dz;
-        if (j < c.n) atomic_add3(pbar, j, -dx, -dy, -dz);
-    });
+    each_pair<T, LamX<T>>(
+        ents, cnts, nbr9, c.n, i, p,
+        [&](int32_t j) {
+            return j < c.n ? LamX<T>{G[j], lambar[j], sden[j], lam[j], proper[j]}
+                           : LamX<T>{T4<T>{0, 0, 0, 0}, (T)0, (T)1, (T)0, (uint8_t)0};
+        },
+        [&](int32_t j, const T4<T> &pj, const LamX<T> &xj) {
+            const T ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
+            const T sq = sqlen(ex, ey, ez);
+            const T r = sqrt(sq);
+            if (r > c.h) return;
+            const bool sym = pri && xj.proper;
+            const T rbj = sym ? -xj.lb * c.inv_rho0 / xj.S : (T)0;
+            const T sbj = sym ? -xj.lam * xj.lb / xj.S : (T)0;
+            const T dw = (rb + rbj) * dwpoly(c, sq);
+            T dx = dw * ex, dy = dw * ey, dz = dw * ez;
+            if (r > (T)0) {
+                const T s = gspiky(c, r);
+                const T ss = 2 * (sb + sbj) * s * c.inv_rho0;
+                const T gx = ss * ex + 2 * (sb * Gi.x - sbj * xj.G.x), gy = ss * ey + 2 * (sb * Gi.y - sbj * xj.G.y),
+                        gz = ss * ez + 2 * (sb * Gi.z - sbj * xj.G.z);
+                T jx, jy, jz;
+                jspiky(c, r, ex, ey, ez, gx, gy, gz, jx, jy, jz);
+                dx += c.inv_rho0 * jx;
+                dy += c.inv_rho0 * jy;
+                dz += c.inv_rho0 * jz;
+            }
+            pbx += dx;
+            pby += dy;
+            pbz += dz;
+            if (!sym && j < c.n) atomic_add3(pbar, j, -dx, -dy, -dz);
+        });
     atomic_add3(pbar, i, pbx, pby, pbz);
 }
+
 
 // gather and predict reversed: xsbar[perm[g]] = pbar0[g];
 // xbar_old += xsbar ; vbar_old = dt xsbar (v enters the step only through x*).
@@ -815,6 +990,7 @@ struct StepTape {
     T *lam = nullptr, *sden = nullptr;  // I x n
     T4<T> *G = nullptr;      // I x n
     uint8_t *mask = nullptr; // I x n
+    const uint8_t *proper = nullptr;  // n: the step's particles whose rows are exact (ka_proper)
     T4<T> *vs = nullptr, *vx = nullptr, *om = nullptr, *eta = nullptr;
     T *o = nullptr;
     // pair lists of the FP32 step ((I + 1) lists: the iterations', then vorticity's)
@@ -850,16 +1026,21 @@ struct Adj : AdjBase {
     uint32_t *all_perm = nullptr;
     int32_t *all_nbr = nullptr;
     uint8_t *all_clamp = nullptr;            // clamp states of every step and iteration of the forward
+    uint8_t *all_proper = nullptr;           // proper flags of every step (ka_proper)
+    uint8_t *cur_proper = nullptr;           // where structure() writes them (null: not kept)
+    uint32_t *skeys = nullptr;               // keys of the sorted records
+    int32_t *tstart = nullptr, *tend = nullptr;  // true cell ranges
     unsigned long long *flips = nullptr;     // clamp decisions a frozen replay overrode
     bool have_all = false;
-    // tape of one segment
+    // tape of one segment, and the segment it holds (-1: none)
     std::vector<StepTape<T>> tape;
+    int32_t tape_segment = -1;
     T4<T> *xs = nullptr;
     T *lam_tmp = nullptr;
     uint2 *qpos = nullptr;  // quantized positions of the FP32 step's current records
     // reverse buffers
     T4<T> *xb = nullptr, *vb = nullptr, *xb2 = nullptr, *vb2 = nullptr, *pb_a = nullptr, *pb_b = nullptr;
-    T4<T> *vxb = nullptr, *omb = nullptr, *vsb = nullptr;
+    T4<T> *vxb = nullptr, *omb = nullptr, *vsb = nullptr, *etab = nullptr;
     T *ob = nullptr, *lb = nullptr;
     double *acc = nullptr;
     bool forwarded = false;
@@ -891,6 +1072,11 @@ struct Adj : AdjBase {
         dfree(all_perm);
         dfree(all_nbr);
         dfree(all_clamp);
+        dfree(all_proper);
+        dfree(skeys);
+        dfree(tstart);
+        dfree(tend);
+        dfree(etab);
         dfree(flips);
         for (auto &t : tape) {
             dfree(t.p);
@@ -968,6 +1154,9 @@ struct Adj : AdjBase {
         if ((e = dalloc(&all_perm, (size_t)steps * n)) || (e = dalloc(&all_nbr, (size_t)steps * 9 * n))) return e;
         if ((e = dalloc(&all_clamp, (size_t)steps * (iters > 0 ? iters : 1) * n)) || (e = dalloc(&flips, 1)))
             return e;
+        if ((e = dalloc(&all_proper, (size_t)steps * n)) || (e = dalloc(&skeys, n)) || (e = dalloc(&tstart, cells)) ||
+            (e = dalloc(&tend, cells)) || (e = dalloc(&etab, n)))
+            return e;
         tape.assign(every, StepTape<T>());
         for (auto &t : tape) {
             if ((e = dalloc(&t.p, (size_t)(iters + 1) * n)) || (e = dalloc(&t.lam, (size_t)iters * n)) ||
@@ -1020,7 +1209,12 @@ struct Adj : AdjBase {
         fc.xsph_c = (float)xsph;  // the plain kernels' constants (FP32 forward)
         fc.vort_eps = (float)veps;
     }
-    void set_coefficients(double cc, double eps) override { set_consts(cc, eps); }
+    // the tape was made with the previous coefficients: the next backward pass
+    // recomputes every segment
+    void set_coefficients(double cc, double eps) override {
+        set_consts(cc, eps);
+        tape_segment = -1;
+    }
     // work queued on the old stream finishes first, so calls stay in order
     void set_stream(hipStream_t s) override {
         const hipStream_t next = s ? s : own;
@@ -1043,6 +1237,14 @@ struct Adj : AdjBase {
         launch_begin_cells(start, cells, epoch, st);
         launch_findcells(fc, rec_f, start, end64, epoch, st);
         launch_neighbourcells(fc, rec_f, start, end64, nbr9, st);
+        if (cur_proper) {
+            launch_keys_of_records(fc, rec_f, skeys, st);
+            launch_fill_i32(tstart, -1, cells, st);
+            const unsigned nb = ablocks(n);
+            ka_true_cells<<<nb, AB, 0, st>>>((int32_t)n, grid[0], grid[1], grid[2], skeys, tstart, tend);
+            ka_proper<<<nb, AB, 0, st>>>((int32_t)n, grid[0], grid[1], grid[2], cells < ((size_t)1 << 24) ? 1 : 0,
+                                         rec_f, skeys, tstart, tend, nbr9, cur_proper);
+        }
     }
 
     // One step on (x, v) in place. With a tape, everything the reverse pass
@@ -1170,12 +1372,30 @@ struct Adj : AdjBase {
         PBF_HIP(hipMemsetAsync(end64, 0, cells * 8, st));
         PBF_HIP(hipMemsetAsync(epoch, 0, 4, st));
         // every step's structure is kept: the reverse pass recomputes the
-        // states and tapes of a segment from its checkpoint with it
+        // states and tapes of a segment from its checkpoint with it; the last
+        // segment, which the reverse pass starts with, is taped here
+        const int32_t nck = (steps + every - 1) / every, last0 = (nck - 1) * every;
+        tape_segment = -1;
         for (int32_t s = 0; s < steps; s++) {
             if (s % every == 0) PBF_HIP(save_ck(s / every));
-            step(nullptr, all_perm + (size_t)s * n, all_nbr + (size_t)s * 9 * n, true,
-                 all_clamp + (size_t)s * iters * n);
+            uint32_t *perm = all_perm + (size_t)s * n;
+            int32_t *nbr = all_nbr + (size_t)s * 9 * n;
+            uint8_t *clamp = all_clamp + (size_t)s * iters * n;
+            cur_proper = all_proper + (size_t)s * n;
+            if (s < last0) {
+                step(nullptr, perm, nbr, true, clamp);
+                continue;
+            }
+            StepTape<T> &t = tape[s - last0];
+            t.perm = perm;
+            t.nbr9 = nbr;
+            t.proper = cur_proper;
+            step(&t, perm, nbr, true);
+            // the tape's clamp states are the step's (frozen replays read them)
+            if (iters > 0) PBF_HIP(hipMemcpyAsync(clamp, t.mask, (size_t)iters * n, hipMemcpyDeviceToDevice, st));
         }
+        cur_proper = nullptr;
+        tape_segment = nck - 1;
         PBF_HIP(hipMemcpyAsync(xf, x, n * sizeof(T4<T>), hipMemcpyDeviceToDevice, st));
         PBF_HIP(hipMemcpyAsync(vf, v, n * sizeof(T4<T>), hipMemcpyDeviceToDevice, st));
         have_all = true;
@@ -1200,18 +1420,18 @@ struct Adj : AdjBase {
     // Reverse pass of one step from its tape: (xb, vb) after the step -> (xb, vb) before it.
     void reverse_step(const StepTape<T> &t) {
         const unsigned nb = ablocks(n);
-        const unsigned nr = ablocks((size_t)9 * n);  // one lane per (particle, row)
         const T4<T> *pI = t.p + (size_t)iters * n;
         zero(reinterpret_cast<T *>(pb_a), 4 * (size_t)n);
         const T4<T> *vsbar = nullptr;
         if (c.vorticity) {
             zero(ob, n);
             zero(reinterpret_cast<T *>(vsb), 4 * (size_t)n);
-            kr_vort_b<T><<<nr, AB, 0, st>>>(c, t.perm, pI, t.nbr9, t.list_entries(iters, n), t.list_counts(iters, n),
-                                            t.om, t.o, t.eta, vb, vxb, omb, ob, pb_a, acc);
+            kr_vort_b_own<T><<<nb, AB, 0, st>>>(c, t.perm, t.om, t.eta, vb, vxb, omb, etab, acc);
+            kr_vort_b<T><<<nb, AB, 0, st>>>(c, pI, t.nbr9, t.list_entries(iters, n), t.list_counts(iters, n),
+                                            t.proper, t.o, etab, ob, pb_a);
             kr_omega_norm<T><<<nb, AB, 0, st>>>((int32_t)n, t.om, t.o, ob, omb);
-            kr_vort_a<T><<<nr, AB, 0, st>>>(c, pI, t.nbr9, t.list_entries(iters, n), t.list_counts(iters, n), t.vs,
-                                            vxb, omb, vsb, pb_a, acc);
+            kr_vort_a<T><<<nb, AB, 0, st>>>(c, pI, t.nbr9, t.list_entries(iters, n), t.list_counts(iters, n),
+                                            t.proper, t.vs, vxb, omb, vsb, pb_a, acc);
             vsbar = vsb;
         }
         kr_update<T><<<nb, AB, 0, st>>>(c, t.perm, xb, vb, vsbar, pb_a, xb2);
@@ -1221,9 +1441,9 @@ struct Adj : AdjBase {
             const T *lam = t.lam + (size_t)it * n;
             zero(reinterpret_cast<T *>(prev), 4 * (size_t)n);
             zero(lb, n);
-            kr_updatepos<T><<<nr, AB, 0, st>>>(c, p, lam, t.nbr9, t.list_entries(it, n), t.list_counts(it, n),
-                                               t.mask + (size_t)it * n, next, prev, lb);
-            kr_lambda<T><<<nr, AB, 0, st>>>(c, p, t.nbr9, t.list_entries(it, n), t.list_counts(it, n), lam,
+            kr_updatepos<T><<<nb, AB, 0, st>>>(c, p, lam, t.nbr9, t.list_entries(it, n), t.list_counts(it, n),
+                                               t.proper, t.mask + (size_t)it * n, next, prev, lb);
+            kr_lambda<T><<<nb, AB, 0, st>>>(c, p, t.nbr9, t.list_entries(it, n), t.list_counts(it, n), t.proper, lam,
                                             t.sden + (size_t)it * n, t.G + (size_t)it * n, lb, prev);
             std::swap(next, prev);
         }
@@ -1241,14 +1461,18 @@ struct Adj : AdjBase {
         const int32_t nck = (steps + every - 1) / every;
         for (int32_t k = nck - 1; k >= 0; k--) {
             const int32_t s0 = k * every, s1 = std::min(steps, s0 + every);
-            PBF_HIP(load_ck(k));
-            for (int32_t s = s0; s < s1; s++) {
-                StepTape<T> &t = tape[s - s0];
-                uint32_t *perm = all_perm + (size_t)s * n;
-                int32_t *nbr = all_nbr + (size_t)s * 9 * n;
-                t.perm = perm;
-                t.nbr9 = nbr;
-                step(&t, perm, nbr, false);
+            if (tape_segment != k) {
+                PBF_HIP(load_ck(k));
+                for (int32_t s = s0; s < s1; s++) {
+                    StepTape<T> &t = tape[s - s0];
+                    uint32_t *perm = all_perm + (size_t)s * n;
+                    int32_t *nbr = all_nbr + (size_t)s * 9 * n;
+                    t.perm = perm;
+                    t.nbr9 = nbr;
+                    t.proper = all_proper + (size_t)s * n;
+                    step(&t, perm, nbr, false);
+                }
+                tape_segment = k;
             }
             for (int32_t s = s1 - 1; s >= s0; s--) reverse_step(tape[s - s0]);
         }
