@@ -213,44 +213,49 @@ __device__ __forceinline__ void each_candidate(const int32_t *__restrict__ nbr9,
     }
 }
 
-// The reverse kernels run one lane per particle over its pair terms: the
-// recorded pair list (the neighbours within h of the forward, in list order),
-// or without one (none recorded, or more than the list holds) the candidates
-// of its rows. extra(j) loads the neighbour's operands of the kernel with the
-// batch's positions, so kAdjBatch neighbours are in flight together.
+// The reverse kernels run a group of kAdjLanes consecutive lanes of one wave
+// per particle; lane sub of the group takes the particle's pair terms sub,
+// sub + kAdjLanes, ... of the recorded pair list (the neighbours within h of
+// the forward, in list order), or without one (none recorded, or more than
+// the list holds) the candidates sub, sub + kAdjLanes, ... of each row. The
+// group's sums are then reduced across its lanes (group_sum), so a particle's
+// own accumulators take one atomic add per launch. extra(j) loads the
+// neighbour's operands of the kernel with the batch's positions, so
+// kAdjBatch neighbours of a lane are in flight together.
+constexpr int kAdjLanes = 8;
+static_assert(64 % kAdjLanes == 0 && AB % kAdjLanes == 0, "lane groups inside a wave");
+
+template <class T>
+__device__ __forceinline__ T group_sum(T v) {
+#pragma unroll
+    for (int m = 1; m < kAdjLanes; m <<= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
 template <class T, class X, class E, class F>
 __device__ __forceinline__ void each_pair(const uint16_t *__restrict__ ents, const uint16_t *__restrict__ cnts,
-                                          const int32_t *__restrict__ nbr9, int32_t n, int32_t i,
+                                          const int32_t *__restrict__ nbr9, int32_t n, int32_t i, int32_t sub,
                                           const T4<T> *__restrict__ p, E &&extra, F &&body) {
     const uint32_t cnt = cnts ? cnts[i] : kListNone;
     if (cnt == kListNone) {
         for (int o = 0; o < 9; o++) {
             const int32_t d = nbr9[(size_t)o * n + i];
             const int32_t s0 = d & 0xFFFFFF, e = d >> 24;
-            for (int32_t q = 0; q < e; q += kAdjBatch) {
-                int32_t j[kAdjBatch];
-                T4<T> pj[kAdjBatch];
-                X xj[kAdjBatch];
-#pragma unroll
-                for (int u = 0; u < kAdjBatch; u++) {
-                    j[u] = s0 + q + min(u, e - 1 - q);
-                    pj[u] = load_nb(p, j[u], n);
-                    xj[u] = extra(j[u]);
-                }
-#pragma unroll
-                for (int u = 0; u < kAdjBatch; u++)
-                    if (q + u < e && j[u] != i) body(j[u], pj[u], xj[u]);
+            for (int32_t q = sub; q < e; q += kAdjLanes) {
+                const int32_t j = s0 + q;
+                if (j != i) body(j, load_nb(p, j, n), extra(j));
             }
         }
         return;
     }
-    for (uint32_t e0 = 0; e0 < cnt; e0 += kAdjBatch) {
+    for (uint32_t e0 = (uint32_t)sub; e0 < cnt; e0 += kAdjBatch * kAdjLanes) {
         int32_t j[kAdjBatch];
         T4<T> pj[kAdjBatch];
         X xj[kAdjBatch];
 #pragma unroll
         for (int u = 0; u < kAdjBatch; u++) {
-            const uint32_t code = ents[(size_t)min(e0 + u, cnt - 1) * n + i];
+            const uint32_t e = e0 + (uint32_t)(u * kAdjLanes);
+            const uint32_t code = ents[(size_t)(e < cnt ? e : e0) * n + i];
             j[u] = (nbr9[(size_t)(code >> 8) * n + i] & 0xFFFFFF) + (int32_t)(code & 255u);
         }
 #pragma unroll
@@ -260,7 +265,7 @@ __device__ __forceinline__ void each_pair(const uint16_t *__restrict__ ents, con
         }
 #pragma unroll
         for (int u = 0; u < kAdjBatch; u++)
-            if (e0 + u < cnt) body(j[u], pj[u], xj[u]);
+            if (e0 + (uint32_t)(u * kAdjLanes) < cnt) body(j[u], pj[u], xj[u]);
     }
 }
 
@@ -608,15 +613,16 @@ __global__ __launch_bounds__(AB) void kr_vort_b(AConsts<T> c, const T4<T> *__res
                                                 const uint16_t *__restrict__ cnts, const uint8_t *__restrict__ proper,
                                                 const T *__restrict__ o, const T4<T> *__restrict__ etabar,
                                                 T *__restrict__ obar, T4<T> *__restrict__ pbar) {
-    const int32_t i = blockIdx.x * AB + threadIdx.x;
-    if (i >= c.n) return;
+    const int64_t tid = (int64_t)blockIdx.x * AB + threadIdx.x;
+    const int32_t i = (int32_t)(tid / kAdjLanes), sub = (int32_t)(tid % kAdjLanes);
+    if (i >= c.n) return;  // whole groups
     const T4<T> pi = p[i];
     const T4<T> eb = etabar[i];
     const T oi = o[i];
     const bool pri = proper[i] != 0;
     T ob = 0, pbx = 0, pby = 0, pbz = 0;
     each_pair<T, VortBX<T>>(
-        ents, cnts, nbr9, c.n, i, p,
+        ents, cnts, nbr9, c.n, i, sub, p,
         [&](int32_t j) {
             return j < c.n ? VortBX<T>{etabar[j], o[j], proper[j]} : VortBX<T>{T4<T>{0, 0, 0, 0}, (T)0, (uint8_t)0};
         },
@@ -643,6 +649,11 @@ __global__ __launch_bounds__(AB) void kr_vort_b(AConsts<T> c, const T4<T> *__res
             pbz += xj.o * jz;
             if (j < c.n) atomic_add3(pbar, j, -xj.o * jx, -xj.o * jy, -xj.o * jz);
         });
+    ob = group_sum(ob);
+    pbx = group_sum(pbx);
+    pby = group_sum(pby);
+    pbz = group_sum(pbz);
+    if (sub != 0) return;
     if (ob != (T)0) atomicAdd(&obar[i], ob);
     atomic_add3(pbar, i, pbx, pby, pbz);
 }
@@ -684,16 +695,17 @@ __global__ __launch_bounds__(AB) void kr_vort_a(AConsts<T> c, const T4<T> *__res
                                                 const T4<T> *__restrict__ vs, const T4<T> *__restrict__ vxbar,
                                                 const T4<T> *__restrict__ ombar, T4<T> *__restrict__ vsbar,
                                                 T4<T> *__restrict__ pbar, double *__restrict__ acc) {
-    const int32_t i = blockIdx.x * AB + threadIdx.x;
+    const int64_t tid = (int64_t)blockIdx.x * AB + threadIdx.x;
+    const int32_t i = (int32_t)(tid / kAdjLanes), sub = (int32_t)(tid % kAdjLanes);
     double dc = 0.0;
-    if (i < c.n) {
+    if (i < c.n) {  // whole groups
         const T4<T> pi = p[i], vi = vs[i], a = vxbar[i], wb = ombar[i];
         const bool pri = proper[i] != 0;
         T sx = 0, sy = 0, sz = 0;
-        T vbx = a.x, vby = a.y, vbz = a.z;  // vx = v + ...
+        T vbx = sub == 0 ? a.x : (T)0, vby = sub == 0 ? a.y : (T)0, vbz = sub == 0 ? a.z : (T)0;  // vx = v + ...
         T pbx = 0, pby = 0, pbz = 0;
         each_pair<T, VortAX<T>>(
-            ents, cnts, nbr9, c.n, i, p,
+            ents, cnts, nbr9, c.n, i, sub, p,
             [&](int32_t j) {
                 const T4<T> z{0, 0, 0, 0};
                 return j < c.n ? VortAX<T>{vs[j], vxbar[j], ombar[j], proper[j]} : VortAX<T>{z, z, z, (uint8_t)0};
@@ -748,9 +760,17 @@ __global__ __launch_bounds__(AB) void kr_vort_a(AConsts<T> c, const T4<T> *__res
                 vbz -= ubz;
                 atomic_add3(pbar, j, -dx, -dy, -dz);
             });
-        atomic_add3(vsbar, i, vbx, vby, vbz);
-        atomic_add3(pbar, i, pbx, pby, pbz);
-        dc = (double)(a.x * sx + a.y * sy + a.z * sz);  // a . sum u W
+        dc = (double)(a.x * sx + a.y * sy + a.z * sz);  // this lane's share of a . sum u W
+        vbx = group_sum(vbx);
+        vby = group_sum(vby);
+        vbz = group_sum(vbz);
+        pbx = group_sum(pbx);
+        pby = group_sum(pby);
+        pbz = group_sum(pbz);
+        if (sub == 0) {
+            atomic_add3(vsbar, i, vbx, vby, vbz);
+            atomic_add3(pbar, i, pbx, pby, pbz);
+        }
     }
     block_add2(dc, 0.0, acc);
 }
@@ -796,19 +816,20 @@ __global__ __launch_bounds__(AB) void kr_updatepos(AConsts<T> c, const T4<T> *__
                                                    const uint8_t *__restrict__ proper,
                                                    const uint8_t *__restrict__ mask, const T4<T> *__restrict__ pbn,
                                                    T4<T> *__restrict__ pbp, T *__restrict__ lambar) {
-    const int32_t i = blockIdx.x * AB + threadIdx.x;
-    if (i >= c.n) return;
+    const int64_t tid = (int64_t)blockIdx.x * AB + threadIdx.x;
+    const int32_t i = (int32_t)(tid / kAdjLanes), sub = (int32_t)(tid % kAdjLanes);
+    if (i >= c.n) return;  // whole groups
     const T4<T> pi = p[i];
     const uint8_t m = mask[i];
     const T4<T> b = pbn[i];
     const T ax = (m & 1) ? b.x : (T)0, ay = (m & 2) ? b.y : (T)0, az = (m & 4) ? b.z : (T)0;
     const bool pri = proper[i] != 0;
-    T pbx = ax, pby = ay, pbz = az;  // the identity part
+    T pbx = sub == 0 ? ax : (T)0, pby = sub == 0 ? ay : (T)0, pbz = sub == 0 ? az : (T)0;  // the identity part
     const T li = lam[i];
     T lb = 0;
     const T k4 = (T)-4 * c.k * c.scale * c.scale * c.scale * c.scale;
     each_pair<T, UpdX<T>>(
-        ents, cnts, nbr9, c.n, i, p,
+        ents, cnts, nbr9, c.n, i, sub, p,
         [&](int32_t j) {
             return j < c.n ? UpdX<T>{pbn[j], lam[j], mask[j], proper[j]}
                            : UpdX<T>{T4<T>{0, 0, 0, 0}, (T)0, (uint8_t)0, (uint8_t)0};
@@ -848,6 +869,11 @@ __global__ __launch_bounds__(AB) void kr_updatepos(AConsts<T> c, const T4<T> *__
             pbz += dz;
             if (!sym && j < c.n) atomic_add3(pbp, j, -dx, -dy, -dz);
         });
+    lb = group_sum(lb);
+    pbx = group_sum(pbx);
+    pby = group_sum(pby);
+    pbz = group_sum(pbz);
+    if (sub != 0) return;
     if (lb != (T)0) atomicAdd(&lambar[i], lb);
     atomic_add3(pbp, i, pbx, pby, pbz);
 }
@@ -871,12 +897,21 @@ __global__ __launch_bounds__(AB) void kr_lambda(AConsts<T> c, const T4<T> *__res
                                                 const uint16_t *__restrict__ cnts,
                                                 const uint8_t *__restrict__ proper, const T *__restrict__ lam,
                                                 const T *__restrict__ sden, const T4<T> *__restrict__ G,
-                                                const T *__restrict__ lambar, T4<T> *__restrict__ pbar) {
-    const int32_t i = blockIdx.x * AB + threadIdx.x;
-    if (i >= c.n) return;
+                                                const T *__restrict__ lambar, T4<T> *__restrict__ pbar,
+                                                T4<T> *__restrict__ clear_p, T *__restrict__ clear_l) {
+    const int64_t tid = (int64_t)blockIdx.x * AB + threadIdx.x;
+    const int32_t i = (int32_t)(tid / kAdjLanes), sub = (int32_t)(tid % kAdjLanes);
+    if (i >= c.n) return;  // whole groups
+    // the next iteration's accumulators: the position update's input of this
+    // iteration (read by its launch, which has finished) and the other lambar
+    if (sub == 0) {
+        clear_p[i] = T4<T>{0, 0, 0, 0};
+        clear_l[i] = (T)0;
+    }
     const T lb = lambar[i];
     const bool pri = proper[i] != 0;
-    // with no gathered side, a particle without lambar adds nothing
+    // with no gathered side, a particle without lambar adds nothing (the
+    // whole group leaves)
     if (lb == (T)0 && !pri) return;
     const T4<T> pi = p[i];
     const T S = sden[i];
@@ -885,7 +920,7 @@ __global__ __launch_bounds__(AB) void kr_lambda(AConsts<T> c, const T4<T> *__res
     const T4<T> Gi = G[i];
     T pbx = 0, pby = 0, pbz = 0;
     each_pair<T, LamX<T>>(
-        ents, cnts, nbr9, c.n, i, p,
+        ents, cnts, nbr9, c.n, i, sub, p,
         [&](int32_t j) {
             return j < c.n ? LamX<T>{G[j], lambar[j], sden[j], lam[j], proper[j]}
                            : LamX<T>{T4<T>{0, 0, 0, 0}, (T)0, (T)1, (T)0, (uint8_t)0};
@@ -916,7 +951,10 @@ __global__ __launch_bounds__(AB) void kr_lambda(AConsts<T> c, const T4<T> *__res
             pbz += dz;
             if (!sym && j < c.n) atomic_add3(pbar, j, -dx, -dy, -dz);
         });
-    atomic_add3(pbar, i, pbx, pby, pbz);
+    pbx = group_sum(pbx);
+    pby = group_sum(pby);
+    pbz = group_sum(pbz);
+    if (sub == 0) atomic_add3(pbar, i, pbx, pby, pbz);
 }
 
 
@@ -959,6 +997,22 @@ __global__ __launch_bounds__(AB) void ka_from_dual(int32_t n, const T4<Dual> *__
 template <class T>
 __global__ __launch_bounds__(AB) void ka_zero(T *__restrict__ p, size_t count) {
     for (size_t k = (size_t)blockIdx.x * AB + threadIdx.x; k < count; k += (size_t)gridDim.x * AB) p[k] = (T)0;
+}
+
+// The accumulators of one reverse step cleared in one launch (per particle).
+template <class T>
+__global__ __launch_bounds__(AB) void kr_clear(int32_t n, T4<T> *__restrict__ a, T4<T> *__restrict__ b,
+                                               T4<T> *__restrict__ v, T *__restrict__ o, T *__restrict__ la,
+                                               T *__restrict__ lb) {
+    const int32_t i = blockIdx.x * AB + threadIdx.x;
+    if (i >= n) return;
+    const T4<T> z{0, 0, 0, 0};
+    a[i] = z;
+    b[i] = z;
+    v[i] = z;
+    o[i] = (T)0;
+    la[i] = (T)0;
+    lb[i] = (T)0;
 }
 
 unsigned ablocks(size_t n) { return (unsigned)((n + AB - 1) / AB); }
@@ -1041,7 +1095,7 @@ struct Adj : AdjBase {
     // reverse buffers
     T4<T> *xb = nullptr, *vb = nullptr, *xb2 = nullptr, *vb2 = nullptr, *pb_a = nullptr, *pb_b = nullptr;
     T4<T> *vxb = nullptr, *omb = nullptr, *vsb = nullptr, *etab = nullptr;
-    T *ob = nullptr, *lb = nullptr;
+    T *ob = nullptr, *lb = nullptr, *lb2 = nullptr;
     double *acc = nullptr;
     bool forwarded = false;
     // pair lists of the plain step's kernels (FP32 handles)
@@ -1106,6 +1160,7 @@ struct Adj : AdjBase {
         dfree(vsb);
         dfree(ob);
         dfree(lb);
+        dfree(lb2);
         dfree(acc);
         dfree(list_entries);
         dfree(list_count);
@@ -1172,7 +1227,8 @@ struct Adj : AdjBase {
         if ((e = dalloc(&xs, n)) || (e = dalloc(&lam_tmp, n)) || (e = dalloc(&qpos, n))) return e;
         if ((e = dalloc(&xb, n)) || (e = dalloc(&vb, n)) || (e = dalloc(&xb2, n)) || (e = dalloc(&vb2, n)) ||
             (e = dalloc(&pb_a, n)) || (e = dalloc(&pb_b, n)) || (e = dalloc(&vxb, n)) || (e = dalloc(&omb, n)) ||
-            (e = dalloc(&vsb, n)) || (e = dalloc(&ob, n)) || (e = dalloc(&lb, n)) || (e = dalloc(&acc, 2)))
+            (e = dalloc(&vsb, n)) || (e = dalloc(&ob, n)) || (e = dalloc(&lb, n)) || (e = dalloc(&lb2, n)) ||
+            (e = dalloc(&acc, 2)))
             return e;
         if (std::is_same<T, float>::value &&
             ((e = dalloc(&list_entries, (size_t)kListRows * n)) || (e = dalloc(&list_count, n))))
@@ -1420,32 +1476,33 @@ struct Adj : AdjBase {
     // Reverse pass of one step from its tape: (xb, vb) after the step -> (xb, vb) before it.
     void reverse_step(const StepTape<T> &t) {
         const unsigned nb = ablocks(n);
+        const unsigned ng = ablocks((size_t)kAdjLanes * n);  // a lane group per particle
         const T4<T> *pI = t.p + (size_t)iters * n;
-        zero(reinterpret_cast<T *>(pb_a), 4 * (size_t)n);
+        // every accumulator the first iteration needs; each iteration's lambda
+        // launch clears the next one's
+        kr_clear<T><<<nb, AB, 0, st>>>((int32_t)n, pb_a, pb_b, vsb, ob, lb, lb2);
         const T4<T> *vsbar = nullptr;
         if (c.vorticity) {
-            zero(ob, n);
-            zero(reinterpret_cast<T *>(vsb), 4 * (size_t)n);
             kr_vort_b_own<T><<<nb, AB, 0, st>>>(c, t.perm, t.om, t.eta, vb, vxb, omb, etab, acc);
-            kr_vort_b<T><<<nb, AB, 0, st>>>(c, pI, t.nbr9, t.list_entries(iters, n), t.list_counts(iters, n),
+            kr_vort_b<T><<<ng, AB, 0, st>>>(c, pI, t.nbr9, t.list_entries(iters, n), t.list_counts(iters, n),
                                             t.proper, t.o, etab, ob, pb_a);
             kr_omega_norm<T><<<nb, AB, 0, st>>>((int32_t)n, t.om, t.o, ob, omb);
-            kr_vort_a<T><<<nb, AB, 0, st>>>(c, pI, t.nbr9, t.list_entries(iters, n), t.list_counts(iters, n),
+            kr_vort_a<T><<<ng, AB, 0, st>>>(c, pI, t.nbr9, t.list_entries(iters, n), t.list_counts(iters, n),
                                             t.proper, t.vs, vxb, omb, vsb, pb_a, acc);
             vsbar = vsb;
         }
         kr_update<T><<<nb, AB, 0, st>>>(c, t.perm, xb, vb, vsbar, pb_a, xb2);
         T4<T> *next = pb_a, *prev = pb_b;
+        T *lcur = lb, *lnext = lb2;
         for (int32_t it = iters - 1; it >= 0; it--) {
             const T4<T> *p = t.p + (size_t)it * n;
             const T *lam = t.lam + (size_t)it * n;
-            zero(reinterpret_cast<T *>(prev), 4 * (size_t)n);
-            zero(lb, n);
-            kr_updatepos<T><<<nb, AB, 0, st>>>(c, p, lam, t.nbr9, t.list_entries(it, n), t.list_counts(it, n),
-                                               t.proper, t.mask + (size_t)it * n, next, prev, lb);
-            kr_lambda<T><<<nb, AB, 0, st>>>(c, p, t.nbr9, t.list_entries(it, n), t.list_counts(it, n), t.proper, lam,
-                                            t.sden + (size_t)it * n, t.G + (size_t)it * n, lb, prev);
+            kr_updatepos<T><<<ng, AB, 0, st>>>(c, p, lam, t.nbr9, t.list_entries(it, n), t.list_counts(it, n),
+                                               t.proper, t.mask + (size_t)it * n, next, prev, lcur);
+            kr_lambda<T><<<ng, AB, 0, st>>>(c, p, t.nbr9, t.list_entries(it, n), t.list_counts(it, n), t.proper, lam,
+                                            t.sden + (size_t)it * n, t.G + (size_t)it * n, lcur, prev, next, lnext);
             std::swap(next, prev);
+            std::swap(lcur, lnext);
         }
         kr_predict<T><<<nb, AB, 0, st>>>(c, t.perm, next, xb2, vb2);
         std::swap(xb, xb2);
