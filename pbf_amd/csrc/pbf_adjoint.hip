@@ -352,17 +352,31 @@ __global__ __launch_bounds__(AB) void ka_proper(int32_t n, int32_t gx, int32_t g
     if (ok) {
         const int32_t px = (int32_t)r.x, py = (int32_t)r.y, pz = (int32_t)r.z;  // as the neighbour-cell pass
         ok = skeys[i] == (uint32_t)px + (uint32_t)gx * ((uint32_t)pz + (uint32_t)gz * (uint32_t)py);
-        for (int o = 0; o < 9 && ok; o++) {
-            const int32_t y = py + o / 3 - 1, z = pz + o % 3 - 1;
+        // the 27 cells' words in two rounds of loads in flight together
+        int32_t w[9], s0[27], e0[27];
+#pragma unroll
+        for (int o = 0; o < 9; o++) w[o] = nbr9[(size_t)o * n + i];
+#pragma unroll
+        for (int q = 0; q < 27; q++) {
+            const int32_t y = py + q / 9 - 1, z = pz + (q / 3) % 3 - 1, x = px + q % 3 - 1;
+            s0[q] = tstart[(size_t)x + (size_t)gx * ((size_t)y + (size_t)gy * (size_t)z)];
+        }
+#pragma unroll
+        for (int q = 0; q < 27; q++) {
+            const int32_t y = py + q / 9 - 1, z = pz + (q / 3) % 3 - 1, x = px + q % 3 - 1;
+            e0[q] = s0[q] == -1 ? 0 : tend[(size_t)x + (size_t)gx * ((size_t)y + (size_t)gy * (size_t)z)];
+        }
+#pragma unroll
+        for (int o = 0; o < 9; o++) {  // row o: (dy, dz) = (o / 3 - 1, o % 3 - 1), cells q = 3 o + dx + 1
             int32_t first = -1, cnt = 0;
-            for (int dx = -1; dx <= 1; dx++) {
-                const size_t c = (size_t)(px + dx) + (size_t)gx * ((size_t)y + (size_t)gy * (size_t)z);
-                const int32_t s0 = tstart[c];
-                if (s0 == -1) continue;
-                if (first == -1) first = s0;
-                cnt += tend[c] - s0;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const int q = 3 * o + k;
+                if (s0[q] == -1) continue;
+                if (first == -1) first = s0[q];
+                cnt += e0[q] - s0[q];
             }
-            ok = cnt < 128 && nbr9[(size_t)o * n + i] == (int32_t)((uint32_t)first + ((uint32_t)cnt << 24));
+            ok = ok && cnt < 128 && w[o] == (int32_t)((uint32_t)first + ((uint32_t)cnt << 24));
         }
     }
     proper[i] = ok ? 1 : 0;
