@@ -309,7 +309,8 @@ void *pbf_batch_get_stream(const pbf_batch *b);
  * zero derivative where it clamps; N = eta/|eta| a zero derivative at eta = 0.
  * States are checkpointed every checkpoint_every steps and each segment's
  * states and intermediate values are recomputed from its checkpoint during the
- * reverse pass. fp64 != 0: every
+ * reverse pass, except the last segment's, which the forward pass keeps (one
+ * segment: nothing is recomputed). fp64 != 0: every
  * state, tape and adjoint in double (for finite-difference checks), and the
  * element type of every state array below is double instead of float. Arrays
  * hold 4 elements per particle (x, y, z, unused) in id order. */
@@ -321,8 +322,9 @@ int pbf_adjoint_destroy(pbf_adjoint *a);
  * from the parameters given at creation). */
 int pbf_adjoint_set_coefficients(pbf_adjoint *a, double xsph_c, double vorticity_epsilon);
 /* steps steps from (pos4, vel4), keeping the checkpoints and every step's
- * sort order and neighbour table (40 bytes per particle and step), which the
- * reverse pass and replays hold fixed. */
+ * sort order, neighbour table, clamp states and the flags of the particles whose
+ * rows are exactly their cells' contents (41 + iterations bytes per particle
+ * and step), which the reverse pass and replays hold fixed. */
 int pbf_adjoint_forward(pbf_adjoint *a, const void *pos4, const void *vel4, int host);
 int pbf_adjoint_positions(pbf_adjoint *a, void *pos4, int host);
 int pbf_adjoint_velocities(pbf_adjoint *a, void *vel4, int host);
