@@ -99,6 +99,8 @@ def cpu_reference(scene):
     fixtures were made (tests/golden/llvmpipe_timing.py); a recorded figure, for
     comparison beside cpu_baseline."""
     path = os.path.join(ROOT, "tests", "golden", "llvmpipe_timing.json")
+    if not os.path.exists(path):
+        return None
     with open(path) as f:
         rec = json.load(f)
     w = rec["workloads"].get(scene.name)
@@ -183,7 +185,7 @@ def bench_batched(steps: int, tune_steps: int):
                         f"total {total}, 3 iterations, dt 0.016, {steps} steps from the initial state",
             "value": vb, "unit": "particle-steps/s", "ms_per_step": 1e3 * tb / steps,
             "looped_plain_value": vl, "looped_plain_ms_per_step": 1e3 * tl / steps, "speedup_vs_looped": vb / vl,
-            "tuning": {k: best[k] for k in ("neighbour_block", "stream_block", "stream_ppt")},
+            "tuning": {k: best[k] for k in ("neighbour_block", "stream_block", "stream_ppt", "list_depth", "persistent")},
             "step_roofline": {"alg_bytes_per_particle_step": b_alg, "frac": vb * b_alg / (HBM_PEAK_GBS * 1e9)}}
 
 

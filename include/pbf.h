@@ -261,11 +261,16 @@ typedef struct pbf_scene_desc {
 
 /* Launch shapes the autotuner picks from: workgroup size of the neighbour
  * passes (64, 128, 256 or 512), and workgroup size (64 .. 1024) and particles per
- * thread (1 .. 4) of the streaming passes (predict, cell table, update). */
+ * thread (1 .. 4) of the streaming passes (predict, cell table, update); the
+ * LDS list depth of the candidate-scanning passes (32 or 56 entries per
+ * thread), and persistent neighbour launches (1: as many workgroups as fit on
+ * the device, each looping over the tiles; 0: one workgroup per tile). */
 typedef struct pbf_batch_tuning {
     int32_t neighbour_block;
     int32_t stream_block;
     int32_t stream_ppt;
+    int32_t list_depth;
+    int32_t persistent;
     double step_ms;         /* measured mean step time of this choice (autotune output) */
 } pbf_batch_tuning;
 
@@ -288,9 +293,12 @@ int pbf_batch_step(pbf_batch *b, int32_t nsteps);
 int pbf_batch_set_graph(pbf_batch *b, int enable);
 int pbf_batch_set_tuning(pbf_batch *b, const pbf_batch_tuning *t);
 int pbf_batch_get_tuning(const pbf_batch *b, pbf_batch_tuning *out);
-/* Times every launch shape on the current state (steps_per_trial steps each,
- * state restored afterwards) and keeps the fastest. results, when not null,
- * receives up to max_results trials; *count their number. */
+/* Times launch shapes on the current state (steps_per_trial steps each, state
+ * restored afterwards) and keeps the fastest: first every neighbour shape
+ * (workgroup size x list depth x persistent) at the current streaming shape,
+ * then every streaming shape (workgroup size x particles per thread) with the
+ * best neighbour shape. results, when not null, receives up to max_results
+ * trials; *count their number. */
 int pbf_batch_autotune(pbf_batch *b, int32_t steps_per_trial, pbf_batch_tuning *best, pbf_batch_tuning *results,
                        int32_t max_results, int32_t *count);
 /* The scenes' persistent end tables, one after another (int32 per cell). */

@@ -42,7 +42,7 @@ class PbfSceneDesc(C.Structure):
 
 class PbfBatchTuning(C.Structure):
     _fields_ = [("neighbour_block", C.c_int32), ("stream_block", C.c_int32), ("stream_ppt", C.c_int32),
-                ("step_ms", C.c_double)]
+                ("list_depth", C.c_int32), ("persistent", C.c_int32), ("step_ms", C.c_double)]
 
 
 vp = C.c_void_p

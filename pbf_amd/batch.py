@@ -85,14 +85,17 @@ class Batch:
     def tuning(self) -> dict:
         t = PbfBatchTuning()
         check(self._lib.pbf_batch_get_tuning(self._h, C.byref(t)))
-        return {"neighbour_block": t.neighbour_block, "stream_block": t.stream_block, "stream_ppt": t.stream_ppt}
+        return {"neighbour_block": t.neighbour_block, "stream_block": t.stream_block, "stream_ppt": t.stream_ppt,
+                "list_depth": t.list_depth, "persistent": t.persistent}
 
-    def set_tuning(self, neighbour_block=64, stream_block=256, stream_ppt=1):
-        t = PbfBatchTuning(int(neighbour_block), int(stream_block), int(stream_ppt), 0.0)
+    def set_tuning(self, neighbour_block=64, stream_block=256, stream_ppt=1, list_depth=56, persistent=0):
+        t = PbfBatchTuning(int(neighbour_block), int(stream_block), int(stream_ppt), int(list_depth),
+                           int(persistent), 0.0)
         check(self._lib.pbf_batch_set_tuning(self._h, C.byref(t)))
 
     def autotune(self, steps_per_trial: int = 2):
-        """Times every launch shape from the current state and keeps the fastest.
+        """Times the launch shapes from the current state (neighbour shapes, then
+        streaming shapes with the best of those) and keeps the fastest.
         Returns (best, trials) as dicts with step_ms."""
         best = PbfBatchTuning()
         res = (PbfBatchTuning * 128)()
@@ -101,7 +104,7 @@ class Batch:
 
         def d(t):
             return {"neighbour_block": t.neighbour_block, "stream_block": t.stream_block, "stream_ppt": t.stream_ppt,
-                    "step_ms": t.step_ms}
+                    "list_depth": t.list_depth, "persistent": t.persistent, "step_ms": t.step_ms}
         return d(best), [d(res[k]) for k in range(min(cnt.value, 128))]
 
     def end_tables(self) -> np.ndarray:

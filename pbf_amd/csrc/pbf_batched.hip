@@ -331,8 +331,11 @@ int pbf_batch_set_tuning(pbf_batch *b, const pbf_batch_tuning *t) {
     ls.nb = t->neighbour_block;
     ls.sb = t->stream_block;
     ls.ppt = t->stream_ppt;
+    ls.lc = t->list_depth;
+    ls.persistent = t->persistent;
     if (!valid_shape(ls))
-        return invalid("tuning: neighbour_block 64/128/256/512, stream_block a power of two in 64..1024, ppt 1..4");
+        return invalid("tuning: neighbour_block 64/128/256/512, stream_block a power of two in 64..1024, ppt 1..4, "
+                       "list_depth 32 or 56, persistent 0 or 1");
     if (b->stream) PBF_HIP(hipStreamSynchronize(b->stream));
     b->drop_graph();
     b->shape = ls;
@@ -344,6 +347,8 @@ int pbf_batch_get_tuning(const pbf_batch *b, pbf_batch_tuning *out) {
     out->neighbour_block = b->shape.nb;
     out->stream_block = b->shape.sb;
     out->stream_ppt = b->shape.ppt;
+    out->list_depth = b->shape.lc;
+    out->persistent = b->shape.persistent;
     out->step_ms = 0.0;
     return PBF_OK;
 }
@@ -399,44 +404,63 @@ int pbf_batch_autotune(pbf_batch *b, int32_t steps_per_trial, pbf_batch_tuning *
     LaunchShape best_shape = keep;
     double best_ms = 1e300;
     int32_t k = 0;
+    // one trial: a warm-up step of the shape, then steps_per_trial timed steps,
+    // each from the snapshot
+    auto trial = [&](const LaunchShape &ls) {
+        b->shape = ls;
+        float ms = 0.0f;
+        if (e == hipSuccess) e = restore();
+        if (e == hipSuccess) {
+            b->enqueue_step(st);
+            e = restore();
+        }
+        if (e == hipSuccess) e = hipEventRecord(e0, st);
+        if (e == hipSuccess) {
+            for (int32_t s = 0; s < steps_per_trial; s++) b->enqueue_step(st);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipEventRecord(e1, st);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) return;
+        const double per = ms / steps_per_trial;
+        if (results && k < max_results) {
+            results[k].neighbour_block = ls.nb;
+            results[k].stream_block = ls.sb;
+            results[k].stream_ppt = ls.ppt;
+            results[k].list_depth = ls.lc;
+            results[k].persistent = ls.persistent;
+            results[k].step_ms = per;
+        }
+        k++;
+        if (per < best_ms) {
+            best_ms = per;
+            best_shape = ls;
+        }
+    };
     static const int nbs[4] = {64, 128, 256, 512};
+    static const int lcs[2] = {32, 56};
     static const int sbs[5] = {64, 128, 256, 512, 1024};
+    // the neighbour passes' shape at the current streaming shape, then the
+    // streaming shape with the best of those
     for (int nb : nbs)
-        for (int sb : sbs)
-            for (int ppt = 1; ppt <= 4; ppt++) {
-                LaunchShape ls;
+        for (int lc : lcs)
+            for (int persistent = 0; persistent <= 1 && e == hipSuccess; persistent++) {
+                LaunchShape ls = keep;
                 ls.nb = nb;
-                ls.sb = sb;
-                ls.ppt = ppt;
-                b->shape = ls;
-                float ms = 0.0f;
-                if (e == hipSuccess) e = restore();
-                if (e == hipSuccess) {
-                    b->enqueue_step(st);  // warm-up of this shape
-                    e = restore();
-                }
-                if (e == hipSuccess) e = hipEventRecord(e0, st);
-                if (e == hipSuccess) {
-                    for (int32_t s = 0; s < steps_per_trial; s++) b->enqueue_step(st);
-                    e = hipGetLastError();
-                }
-                if (e == hipSuccess) e = hipEventRecord(e1, st);
-                if (e == hipSuccess) e = hipEventSynchronize(e1);
-                if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-                if (e != hipSuccess) break;
-                const double per = ms / steps_per_trial;
-                if (results && k < max_results) {
-                    results[k].neighbour_block = nb;
-                    results[k].stream_block = sb;
-                    results[k].stream_ppt = ppt;
-                    results[k].step_ms = per;
-                }
-                k++;
-                if (per < best_ms) {
-                    best_ms = per;
-                    best_shape = ls;
-                }
+                ls.lc = lc;
+                ls.persistent = persistent;
+                trial(ls);
             }
+    const LaunchShape nbest = best_shape;
+    for (int sb : sbs)
+        for (int ppt = 1; ppt <= 4 && e == hipSuccess; ppt++) {
+            LaunchShape ls = nbest;
+            ls.sb = sb;
+            ls.ppt = ppt;
+            if (ls.sb == nbest.sb && ls.ppt == nbest.ppt) continue;  // timed in the first round
+            trial(ls);
+        }
     if (e == hipSuccess) e = restore();
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     cleanup();
@@ -451,6 +475,8 @@ int pbf_batch_autotune(pbf_batch *b, int32_t steps_per_trial, pbf_batch_tuning *
         best->neighbour_block = best_shape.nb;
         best->stream_block = best_shape.sb;
         best->stream_ppt = best_shape.ppt;
+        best->list_depth = best_shape.lc;
+        best->persistent = best_shape.persistent;
         best->step_ms = best_ms;
     }
     return PBF_OK;

@@ -207,13 +207,16 @@ void launch_vorticity_b(const SolverConsts &c, const float4 *rec, const int32_t 
                         const float4 *vxsph, const float4 *omega, const float *vort, float4 *vel_id, hipStream_t st,
                         bool padded = false, const DenseList *dl = nullptr, const SolverTape *tape = nullptr);
 // Launch shape of the batched form: workgroup size of the neighbour passes
-// (128, 256, 512), workgroup size (64 .. 1024, a power of two) and particles
-// per thread (1 .. 4) of the streaming passes. The one-scene form uses the
+// (64, 128, 256, 512), workgroup size (64 .. 1024, a power of two) and particles
+// per thread (1 .. 4) of the streaming passes, the scanning passes' LDS list
+// depth, and persistent neighbour launches. The one-scene form uses the
 // default.
 struct LaunchShape {
     int32_t nb = 64;
     int32_t sb = 256;
     int32_t ppt = 1;
+    int32_t lc = 56;          // LDS list entries per thread of the scanning passes (32 or 56)
+    int32_t persistent = 0;   // neighbour launches of device-filling workgroups looping over tiles
 };
 bool valid_shape(const LaunchShape &ls);
 

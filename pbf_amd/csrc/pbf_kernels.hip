@@ -1136,60 +1136,73 @@ __device__ __forceinline__ bool leave_dense(int32_t cand, const DenseList &dl) {
 }
 
 // calclambda.glsl:66-103; records the neighbours within h for the position update.
-template <bool H1, int NB, bool PAD, class Dom>
+template <bool H1, int NB, bool PAD, class Dom, int LC = LCAP>
 __global__ __launch_bounds__(NB) void k_lambda(SolverConsts c, Dom dom, const float4 *__restrict__ rec,
                                                      const uint2 *__restrict__ qpos,
                                                      const int32_t *__restrict__ nbr9, float *__restrict__ lambda,
                                                      uint16_t *__restrict__ list_entries,
                                                      uint16_t *__restrict__ list_count, int32_t list_cap,
-                                                     DenseList dl, uint32_t dense_blocks, SolverTape tp) {
+                                                     DenseList dl, uint32_t dense_blocks, SolverTape tp, uint32_t ntiles) {
     __shared__ union {
-        NbrLds<NB> light;
+        NbrLds<NB, LC> light;
         DenseLds<5, 0, NB / 64> dense;
     } u;
-    {
-        if (blockIdx.x < dense_blocks) {
-            LambdaPass<H1, PAD> ps{c, rec, lambda, tp, {}};
-            dense_pass<LambdaPass<H1, PAD>, PAD, NB / 64>(c, dom, rec, nbr9, dl, list_entries, list_count, list_cap, ps,
-                                                 u.dense, blockIdx.x, dense_blocks);
-            return;
+    // tile bid of ntiles (one per workgroup, or several per workgroup of a
+    // persistent launch, whose grid is a multiple of 8 so that tile bid runs on
+    // XCD bid % 8 as xcd_block assumes)
+    auto tile = [&](const uint32_t bid) {
+        {
+            if (bid < dense_blocks) {
+                LambdaPass<H1, PAD> ps{c, rec, lambda, tp, {}};
+                dense_pass<LambdaPass<H1, PAD>, PAD, NB / 64>(c, dom, rec, nbr9, dl, list_entries, list_count, list_cap, ps,
+                                                     u.dense, bid, dense_blocks);
+                return;
+            }
         }
+        NbrLds<NB, LC> &lds = u.light;
+        const int32_t i = xcd_block<NB>(bid - dense_blocks, ntiles - dense_blocks) * NB + threadIdx.x;
+        if (i >= c.n) return;
+        if (leave_dense(load_rows<NB>(c, nbr9, i, lds), dl)) return;
+        const Seg sg = dom.at(i);
+        const float4 pi = rec[i];
+        float rho = 0.0f, sk = 0.0f, gix = 0.0f, giy = 0.0f, giz = 0.0f;
+        PairListOut out = {list_entries, list_count, list_cap, 0, list_entries ? list_entries + i : nullptr};
+        scan_neighbours<NB, PAD, true>(c, sg, rec, qpos, i, pi, lds, [](int32_t) { return 0; },
+                                 [&](int32_t, uint32_t code, const float4 &pj, int) {
+            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const float sq = sq_len(dx, dy, dz);
+            if (sq > c.h2cut) return;
+            const float r = sqrt_rn_finite(sq);
+            if (r > c.h) return;
+            out.add(i, c.n, code);
+            rho += wpoly6<H1>(c, sq, r);
+            if (r == 0.0f) return;
+            float gx, gy, gz;
+            grad_spiky<H1>(c, dx, dy, dz, r, gx, gy, gz);
+            gx = gx * c.inv_rho0;
+            gy = gy * c.inv_rho0;
+            gz = gz * c.inv_rho0;
+            sk += sq_len(gx, gy, gz);
+            gix += gx;
+            giy += gy;
+            giz += gz;
+        });
+        out.finish(i);
+        sk += sq_len(gix, giy, giz);
+        const float ci = rho * c.inv_rho0 - 1.0f;
+        lambda[i] = -ci / (sk + c.eps);
+        if (tp.sden) {
+            tp.sden[i] = sk + c.eps;
+            tp.G[i] = make_float4(gix, giy, giz, 0.0f);
+        }
+    };
+    if (std::is_same<Dom, DomOne>::value || gridDim.x >= ntiles) {
+        tile(blockIdx.x);  // one tile per workgroup (always so for one scene)
+        return;
     }
-    NbrLds<NB> &lds = u.light;
-    const int32_t i = xcd_block<NB>(blockIdx.x - dense_blocks, gridDim.x - dense_blocks) * NB + threadIdx.x;
-    if (i >= c.n) return;
-    if (leave_dense(load_rows<NB>(c, nbr9, i, lds), dl)) return;
-    const Seg sg = dom.at(i);
-    const float4 pi = rec[i];
-    float rho = 0.0f, sk = 0.0f, gix = 0.0f, giy = 0.0f, giz = 0.0f;
-    PairListOut out = {list_entries, list_count, list_cap, 0, list_entries ? list_entries + i : nullptr};
-    scan_neighbours<NB, PAD, true>(c, sg, rec, qpos, i, pi, lds, [](int32_t) { return 0; },
-                             [&](int32_t, uint32_t code, const float4 &pj, int) {
-        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        const float sq = sq_len(dx, dy, dz);
-        if (sq > c.h2cut) return;
-        const float r = sqrt_rn_finite(sq);
-        if (r > c.h) return;
-        out.add(i, c.n, code);
-        rho += wpoly6<H1>(c, sq, r);
-        if (r == 0.0f) return;
-        float gx, gy, gz;
-        grad_spiky<H1>(c, dx, dy, dz, r, gx, gy, gz);
-        gx = gx * c.inv_rho0;
-        gy = gy * c.inv_rho0;
-        gz = gz * c.inv_rho0;
-        sk += sq_len(gx, gy, gz);
-        gix += gx;
-        giy += gy;
-        giz += gz;
-    });
-    out.finish(i);
-    sk += sq_len(gix, giy, giz);
-    const float ci = rho * c.inv_rho0 - 1.0f;
-    lambda[i] = -ci / (sk + c.eps);
-    if (tp.sden) {
-        tp.sden[i] = sk + c.eps;
-        tp.G[i] = make_float4(gix, giy, giz, 0.0f);
+    for (uint32_t b = blockIdx.x; b < ntiles; b += gridDim.x) {
+        tile(b);
+        __syncthreads();  // the next tile reuses the LDS
     }
 }
 
@@ -1203,58 +1216,71 @@ __global__ __launch_bounds__(NB) void k_updatepos(SolverConsts c, Dom dom, const
                                                         const uint16_t *__restrict__ list_count,
                                                         float4 *__restrict__ rec_out, uint2 *__restrict__ q_out,
                                                         DenseList dl, uint32_t dense_blocks, StepUpdate up,
-                                                        SolverTape tp) {
+                                                        SolverTape tp, uint32_t ntiles) {
     __shared__ union {
         NbrLds<NB, kReplayLcap> light;
         DenseLds<3, 0, NB / 64> dense;
     } u;
-    {
-        if (blockIdx.x < dense_blocks) {
-            UpdatePosPass<H1, PAD> ps{c, rec_in, lambda, rec_out, q_out, up, tp, {}, 0.0f};
-            dense_pass<UpdatePosPass<H1, PAD>, PAD, NB / 64>(c, dom, rec_in, nbr9, dl, const_cast<uint16_t *>(list_entries),
-                                                    const_cast<uint16_t *>(list_count), 0, ps, u.dense, blockIdx.x,
-                                                    dense_blocks);
-            return;
+    // tile bid of ntiles (one per workgroup, or several per workgroup of a
+    // persistent launch, whose grid is a multiple of 8 so that tile bid runs on
+    // XCD bid % 8 as xcd_block assumes)
+    auto tile = [&](const uint32_t bid) {
+        {
+            if (bid < dense_blocks) {
+                UpdatePosPass<H1, PAD> ps{c, rec_in, lambda, rec_out, q_out, up, tp, {}, 0.0f};
+                dense_pass<UpdatePosPass<H1, PAD>, PAD, NB / 64>(c, dom, rec_in, nbr9, dl, const_cast<uint16_t *>(list_entries),
+                                                        const_cast<uint16_t *>(list_count), 0, ps, u.dense, bid,
+                                                        dense_blocks);
+                return;
+            }
         }
+        NbrLds<NB, kReplayLcap> &lds = u.light;
+        const int32_t i = xcd_block<NB>(bid - dense_blocks, ntiles - dense_blocks) * NB + threadIdx.x;
+        if (i >= c.n) return;
+        if (leave_dense(load_rows<NB>(c, nbr9, i, lds), dl)) return;
+        const Seg sg = dom.at(i);
+        const float4 pi = rec_in[i];
+        const float li = lambda[i];
+        float dpx = 0.0f, dpy = 0.0f, dpz = 0.0f;
+        replay_neighbours<NB, PAD>(c, sg, rec_in, list_entries, list_count, i, pi, lds,
+                          [&](int32_t j) { return val_at<PAD>(lambda, sg, j); },
+                          [&](int32_t, const float4 &pj, float lj, bool listed) {
+            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const float sq = sq_len(dx, dy, dz);
+            if (!listed && sq > c.h2cut) return;
+            const float r = sqrt_rn_finite(sq);
+            if ((!listed && r > c.h) || r == 0.0f) return;
+            float sc = c.scale * wpoly6<H1>(c, sq, r);
+            sc *= sc;
+            sc *= sc;
+            sc = -c.k * sc;
+            float gx, gy, gz;
+            grad_spiky<H1>(c, dx, dy, dz, r, gx, gy, gz);
+            const float f = li + lj + sc;
+            dpx += f * gx;
+            dpy += f * gy;
+            dpz += f * gz;
+        });
+        float x = pi.x + c.inv_rho0 * dpx, y = pi.y + c.inv_rho0 * dpy, z = pi.z + c.inv_rho0 * dpz;
+        if (tp.clamp)
+            tp.clamp[i] = (uint8_t)(clamp_state(x, sg.lo[0], sg.hi[0], 0) | clamp_state(y, sg.lo[1], sg.hi[1], 1) |
+                                    clamp_state(z, sg.lo[2], sg.hi[2], 2));
+        x = clamp_glsl(x, sg.lo[0], sg.hi[0]);
+        y = clamp_glsl(y, sg.lo[1], sg.hi[1]);
+        z = clamp_glsl(z, sg.lo[2], sg.hi[2]);
+        const float4 r = make_float4(x, y, z, pi.w);
+        rec_out[i] = r;
+        if (q_out) q_out[i] = quantize_pos(r, c.qscale);
+        if (up.pos) update_one(c, sg, i, r, up);
+    };
+    if (std::is_same<Dom, DomOne>::value || gridDim.x >= ntiles) {
+        tile(blockIdx.x);  // one tile per workgroup (always so for one scene)
+        return;
     }
-    NbrLds<NB, kReplayLcap> &lds = u.light;
-    const int32_t i = xcd_block<NB>(blockIdx.x - dense_blocks, gridDim.x - dense_blocks) * NB + threadIdx.x;
-    if (i >= c.n) return;
-    if (leave_dense(load_rows<NB>(c, nbr9, i, lds), dl)) return;
-    const Seg sg = dom.at(i);
-    const float4 pi = rec_in[i];
-    const float li = lambda[i];
-    float dpx = 0.0f, dpy = 0.0f, dpz = 0.0f;
-    replay_neighbours<NB, PAD>(c, sg, rec_in, list_entries, list_count, i, pi, lds,
-                      [&](int32_t j) { return val_at<PAD>(lambda, sg, j); },
-                      [&](int32_t, const float4 &pj, float lj, bool listed) {
-        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        const float sq = sq_len(dx, dy, dz);
-        if (!listed && sq > c.h2cut) return;
-        const float r = sqrt_rn_finite(sq);
-        if ((!listed && r > c.h) || r == 0.0f) return;
-        float sc = c.scale * wpoly6<H1>(c, sq, r);
-        sc *= sc;
-        sc *= sc;
-        sc = -c.k * sc;
-        float gx, gy, gz;
-        grad_spiky<H1>(c, dx, dy, dz, r, gx, gy, gz);
-        const float f = li + lj + sc;
-        dpx += f * gx;
-        dpy += f * gy;
-        dpz += f * gz;
-    });
-    float x = pi.x + c.inv_rho0 * dpx, y = pi.y + c.inv_rho0 * dpy, z = pi.z + c.inv_rho0 * dpz;
-    if (tp.clamp)
-        tp.clamp[i] = (uint8_t)(clamp_state(x, sg.lo[0], sg.hi[0], 0) | clamp_state(y, sg.lo[1], sg.hi[1], 1) |
-                                clamp_state(z, sg.lo[2], sg.hi[2], 2));
-    x = clamp_glsl(x, sg.lo[0], sg.hi[0]);
-    y = clamp_glsl(y, sg.lo[1], sg.hi[1]);
-    z = clamp_glsl(z, sg.lo[2], sg.hi[2]);
-    const float4 r = make_float4(x, y, z, pi.w);
-    rec_out[i] = r;
-    if (q_out) q_out[i] = quantize_pos(r, c.qscale);
-    if (up.pos) update_one(c, sg, i, r, up);
+    for (uint32_t b = blockIdx.x; b < ntiles; b += gridDim.x) {
+        tile(b);
+        __syncthreads();  // the next tile reuses the LDS
+    }
 }
 
 // update.glsl:16-28; also keeps the velocities in sorted order for the
@@ -1270,7 +1296,7 @@ __global__ __launch_bounds__(1024) void k_update(SolverConsts c, Dom dom, int32_
 // vorticity.glsl:34-60 (first half): XSPH and omega from the velocities before
 // this pass; |omega| to the vorticity buffer (vorticities[], :60). Records the
 // neighbours within h for the second half.
-template <bool H1, int NB, bool PAD, class Dom>
+template <bool H1, int NB, bool PAD, class Dom, int LC = LCAP>
 __global__ __launch_bounds__(NB) void k_vorticity_a(SolverConsts c, Dom dom, const float4 *__restrict__ rec,
                                                           const uint2 *__restrict__ qpos,
                                                           const int32_t *__restrict__ nbr9,
@@ -1279,54 +1305,67 @@ __global__ __launch_bounds__(NB) void k_vorticity_a(SolverConsts c, Dom dom, con
                                                           float4 *__restrict__ vxsph, float4 *__restrict__ omega,
                                                           float *__restrict__ vort, uint16_t *__restrict__ list_entries,
                                                           uint16_t *__restrict__ list_count, int32_t list_cap,
-                                                          DenseList dl, uint32_t dense_blocks) {
+                                                          DenseList dl, uint32_t dense_blocks, uint32_t ntiles) {
     __shared__ union {
-        NbrLds<NB> light;
+        NbrLds<NB, LC> light;
         DenseLds<6, 3, NB / 64> dense;
     } u;
-    {
-        if (blockIdx.x < dense_blocks) {
-            VorticityAPass<H1, PAD> ps{c, rec, vel_sorted, vel_id, vxsph, omega, vort, {}, {}};
-            dense_pass<VorticityAPass<H1, PAD>, PAD, NB / 64>(c, dom, rec, nbr9, dl, list_entries, list_count, list_cap, ps,
-                                                     u.dense, blockIdx.x, dense_blocks);
-            return;
+    // tile bid of ntiles (one per workgroup, or several per workgroup of a
+    // persistent launch, whose grid is a multiple of 8 so that tile bid runs on
+    // XCD bid % 8 as xcd_block assumes)
+    auto tile = [&](const uint32_t bid) {
+        {
+            if (bid < dense_blocks) {
+                VorticityAPass<H1, PAD> ps{c, rec, vel_sorted, vel_id, vxsph, omega, vort, {}, {}};
+                dense_pass<VorticityAPass<H1, PAD>, PAD, NB / 64>(c, dom, rec, nbr9, dl, list_entries, list_count, list_cap, ps,
+                                                         u.dense, bid, dense_blocks);
+                return;
+            }
         }
+        NbrLds<NB, LC> &lds = u.light;
+        const int32_t i = xcd_block<NB>(bid - dense_blocks, ntiles - dense_blocks) * NB + threadIdx.x;
+        if (i >= c.n) return;
+        if (leave_dense(load_rows<NB>(c, nbr9, i, lds), dl)) return;
+        const Seg sg = dom.at(i);
+        const float4 pi = rec[i];
+        const float4 vi = vel_sorted[i];
+        float sx = 0.0f, sy = 0.0f, sz = 0.0f, wx = 0.0f, wy = 0.0f, wz = 0.0f;
+        PairListOut out = {list_entries, list_count, list_cap, 0, list_entries ? list_entries + i : nullptr};
+        // a zero record past the scene has id 0: imageLoad (velocitytexture, 0)
+        scan_neighbours<NB, PAD, true>(c, sg, rec, qpos, i, pi, lds,
+                                 [&](int32_t j) { return *(j < sg.n ? vel_sorted + sg.off + j : vel_id + sg.off); },
+                                 [&](int32_t, uint32_t code, const float4 &pj, const float4 &vj) {
+            const float px = pi.x - pj.x, py = pi.y - pj.y, pz = pi.z - pj.z;
+            const float sq = sq_len(px, py, pz);
+            if (sq > c.h2cut) return;
+            const float r = sqrt_rn_finite(sq);
+            if (r > c.h) return;
+            out.add(i, c.n, code);
+            const float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
+            const float w = wpoly6<H1>(c, sq, r);
+            sx += ux * w;
+            sy += uy * w;
+            sz += uz * w;
+            if (r == 0.0f) return;
+            float gx, gy, gz;
+            grad_spiky<H1>(c, px, py, pz, r, gx, gy, gz);
+            wx = (wx + uy * gz) - uz * gy;
+            wy = (wy + uz * gx) - ux * gz;
+            wz = (wz + ux * gy) - uy * gx;
+        });
+        out.finish(i);
+        vxsph[i] = make_float4(vi.x + sg.xsph_c * sx, vi.y + sg.xsph_c * sy, vi.z + sg.xsph_c * sz, 0.0f);
+        omega[i] = make_float4(wx, wy, wz, 0.0f);
+        vort[i] = sqrtf(sq_len(wx, wy, wz));
+    };
+    if (std::is_same<Dom, DomOne>::value || gridDim.x >= ntiles) {
+        tile(blockIdx.x);  // one tile per workgroup (always so for one scene)
+        return;
     }
-    NbrLds<NB> &lds = u.light;
-    const int32_t i = xcd_block<NB>(blockIdx.x - dense_blocks, gridDim.x - dense_blocks) * NB + threadIdx.x;
-    if (i >= c.n) return;
-    if (leave_dense(load_rows<NB>(c, nbr9, i, lds), dl)) return;
-    const Seg sg = dom.at(i);
-    const float4 pi = rec[i];
-    const float4 vi = vel_sorted[i];
-    float sx = 0.0f, sy = 0.0f, sz = 0.0f, wx = 0.0f, wy = 0.0f, wz = 0.0f;
-    PairListOut out = {list_entries, list_count, list_cap, 0, list_entries ? list_entries + i : nullptr};
-    // a zero record past the scene has id 0: imageLoad (velocitytexture, 0)
-    scan_neighbours<NB, PAD, true>(c, sg, rec, qpos, i, pi, lds,
-                             [&](int32_t j) { return *(j < sg.n ? vel_sorted + sg.off + j : vel_id + sg.off); },
-                             [&](int32_t, uint32_t code, const float4 &pj, const float4 &vj) {
-        const float px = pi.x - pj.x, py = pi.y - pj.y, pz = pi.z - pj.z;
-        const float sq = sq_len(px, py, pz);
-        if (sq > c.h2cut) return;
-        const float r = sqrt_rn_finite(sq);
-        if (r > c.h) return;
-        out.add(i, c.n, code);
-        const float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
-        const float w = wpoly6<H1>(c, sq, r);
-        sx += ux * w;
-        sy += uy * w;
-        sz += uz * w;
-        if (r == 0.0f) return;
-        float gx, gy, gz;
-        grad_spiky<H1>(c, px, py, pz, r, gx, gy, gz);
-        wx = (wx + uy * gz) - uz * gy;
-        wy = (wy + uz * gx) - ux * gz;
-        wz = (wz + ux * gy) - uy * gx;
-    });
-    out.finish(i);
-    vxsph[i] = make_float4(vi.x + sg.xsph_c * sx, vi.y + sg.xsph_c * sy, vi.z + sg.xsph_c * sz, 0.0f);
-    omega[i] = make_float4(wx, wy, wz, 0.0f);
-    vort[i] = sqrtf(sq_len(wx, wy, wz));
+    for (uint32_t b = blockIdx.x; b < ntiles; b += gridDim.x) {
+        tile(b);
+        __syncthreads();  // the next tile reuses the LDS
+    }
 }
 
 // vorticity.glsl:65-85 (second half): confinement from this step's |omega|;
@@ -1340,55 +1379,68 @@ __global__ __launch_bounds__(NB) void k_vorticity_b(SolverConsts c, Dom dom, con
                                                           const float4 *__restrict__ omega,
                                                           const float *__restrict__ vort,
                                                           float4 *__restrict__ vel_id, DenseList dl,
-                                                          uint32_t dense_blocks, SolverTape tp) {
+                                                          uint32_t dense_blocks, SolverTape tp, uint32_t ntiles) {
     __shared__ union {
         NbrLds<NB, kReplayLcap> light;
         DenseLds<3, 0, NB / 64> dense;
     } u;
-    {
-        if (blockIdx.x < dense_blocks) {
-            VorticityBPass<H1, PAD> ps{c, rec, vxsph, omega, vort, vel_id, tp, {}};
-            dense_pass<VorticityBPass<H1, PAD>, PAD, NB / 64>(c, dom, rec, nbr9, dl, const_cast<uint16_t *>(list_entries),
-                                                     const_cast<uint16_t *>(list_count), 0, ps, u.dense, blockIdx.x,
-                                                     dense_blocks);
-            return;
+    // tile bid of ntiles (one per workgroup, or several per workgroup of a
+    // persistent launch, whose grid is a multiple of 8 so that tile bid runs on
+    // XCD bid % 8 as xcd_block assumes)
+    auto tile = [&](const uint32_t bid) {
+        {
+            if (bid < dense_blocks) {
+                VorticityBPass<H1, PAD> ps{c, rec, vxsph, omega, vort, vel_id, tp, {}};
+                dense_pass<VorticityBPass<H1, PAD>, PAD, NB / 64>(c, dom, rec, nbr9, dl, const_cast<uint16_t *>(list_entries),
+                                                         const_cast<uint16_t *>(list_count), 0, ps, u.dense, bid,
+                                                         dense_blocks);
+                return;
+            }
         }
+        NbrLds<NB, kReplayLcap> &lds = u.light;
+        const int32_t i = xcd_block<NB>(bid - dense_blocks, ntiles - dense_blocks) * NB + threadIdx.x;
+        if (i >= c.n) return;
+        if (leave_dense(load_rows<NB>(c, nbr9, i, lds), dl)) return;
+        const Seg sg = dom.at(i);
+        const float4 pi = rec[i];
+        float nx = 0.0f, ny = 0.0f, nz = 0.0f;
+        replay_neighbours<NB, PAD>(c, sg, rec, list_entries, list_count, i, pi, lds,
+                          [&](int32_t j) { return val_at<PAD>(vort, sg, j); },
+                          [&](int32_t, const float4 &pj, float om, bool listed) {
+            const float px = pi.x - pj.x, py = pi.y - pj.y, pz = pi.z - pj.z;
+            const float sq = sq_len(px, py, pz);
+            if (!listed && sq > c.h2cut) return;
+            const float r = sqrt_rn_finite(sq);
+            if ((!listed && r > c.h) || r == 0.0f) return;
+            float gx, gy, gz;
+            grad_spiky<H1>(c, px, py, pz, r, gx, gy, gz);
+            nx += om * gx;
+            ny += om * gy;
+            nz += om * gz;
+        });
+        if (tp.eta) tp.eta[i] = make_float4(nx, ny, nz, 0.0f);
+        const float l = sqrtf(sq_len(nx, ny, nz));
+        if (l > 0.0f) {
+            nx = nx / l;
+            ny = ny / l;
+            nz = nz / l;
+        }
+        const float4 w = omega[i];
+        const float4 v = vxsph[i];
+        const float sc = c.dt * c.vort_eps;
+        const float ox = v.x + sc * (ny * w.z - nz * w.y);
+        const float oy = v.y + sc * (nz * w.x - nx * w.z);
+        const float oz = v.z + sc * (nx * w.y - ny * w.x);
+        vel_id[sg.off + id_of(pi)] = make_float4(ox, oy, oz, 0.0f);
+    };
+    if (std::is_same<Dom, DomOne>::value || gridDim.x >= ntiles) {
+        tile(blockIdx.x);  // one tile per workgroup (always so for one scene)
+        return;
     }
-    NbrLds<NB, kReplayLcap> &lds = u.light;
-    const int32_t i = xcd_block<NB>(blockIdx.x - dense_blocks, gridDim.x - dense_blocks) * NB + threadIdx.x;
-    if (i >= c.n) return;
-    if (leave_dense(load_rows<NB>(c, nbr9, i, lds), dl)) return;
-    const Seg sg = dom.at(i);
-    const float4 pi = rec[i];
-    float nx = 0.0f, ny = 0.0f, nz = 0.0f;
-    replay_neighbours<NB, PAD>(c, sg, rec, list_entries, list_count, i, pi, lds,
-                      [&](int32_t j) { return val_at<PAD>(vort, sg, j); },
-                      [&](int32_t, const float4 &pj, float om, bool listed) {
-        const float px = pi.x - pj.x, py = pi.y - pj.y, pz = pi.z - pj.z;
-        const float sq = sq_len(px, py, pz);
-        if (!listed && sq > c.h2cut) return;
-        const float r = sqrt_rn_finite(sq);
-        if ((!listed && r > c.h) || r == 0.0f) return;
-        float gx, gy, gz;
-        grad_spiky<H1>(c, px, py, pz, r, gx, gy, gz);
-        nx += om * gx;
-        ny += om * gy;
-        nz += om * gz;
-    });
-    if (tp.eta) tp.eta[i] = make_float4(nx, ny, nz, 0.0f);
-    const float l = sqrtf(sq_len(nx, ny, nz));
-    if (l > 0.0f) {
-        nx = nx / l;
-        ny = ny / l;
-        nz = nz / l;
+    for (uint32_t b = blockIdx.x; b < ntiles; b += gridDim.x) {
+        tile(b);
+        __syncthreads();  // the next tile reuses the LDS
     }
-    const float4 w = omega[i];
-    const float4 v = vxsph[i];
-    const float sc = c.dt * c.vort_eps;
-    const float ox = v.x + sc * (ny * w.z - nz * w.y);
-    const float oy = v.y + sc * (nz * w.x - nx * w.z);
-    const float oz = v.z + sc * (nx * w.y - ny * w.x);
-    vel_id[sg.off + id_of(pi)] = make_float4(ox, oy, oz, 0.0f);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_quantize(float qscale, const float4 *__restrict__ rec,
@@ -1464,7 +1516,8 @@ Seg seg_of(const SolverConsts &c) {
 bool valid_shape(const LaunchShape &ls) {
     const bool nb = ls.nb == 64 || ls.nb == 128 || ls.nb == 256 || ls.nb == 512;
     const bool sb = ls.sb >= 64 && ls.sb <= 1024 && (ls.sb & (ls.sb - 1)) == 0;
-    return nb && sb && ls.ppt >= 1 && ls.ppt <= 4;
+    const bool lc = ls.lc == 32 || ls.lc == LCAP;
+    return nb && sb && lc && ls.ppt >= 1 && ls.ppt <= 4 && (ls.persistent == 0 || ls.persistent == 1);
 }
 
 namespace {
@@ -1530,6 +1583,35 @@ void neighbourcells_impl(const SolverConsts &c, const Dom &d, const LaunchShape 
     if (dl && c.n > 0) launch_dense_select(*dl, st);
 }
 
+// Grid of a neighbour launch of ntiles workgroup tiles: one workgroup per
+// tile, or for a persistent launch as many workgroups as fit on the device at
+// once (a multiple of 8), each looping over tiles.
+template <class K>
+unsigned neighbour_grid(K kernel, int nb, unsigned ntiles, bool persistent) {
+    if (!persistent) return ntiles;
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, nb, 0) != hipSuccess || per <= 0)
+        return ntiles;
+    const unsigned cap = (unsigned)(cus * per) / 8u * 8u;
+    return cap >= 8u && cap < ntiles ? cap : ntiles;
+}
+
+// Calls f with the scanning passes' LDS list depth as a compile-time constant:
+// the default for one scene; 32 or 56 entries for many (a tuning choice).
+template <class Dom, class F>
+void with_lc(int lc, F &&f) {
+    if constexpr (std::is_same<Dom, DomOne>::value) {
+        f(std::integral_constant<int, LCAP>());
+    } else {
+        if (lc == 32)
+            f(std::integral_constant<int, 32>());
+        else
+            f(std::integral_constant<int, LCAP>());
+    }
+}
+
 // Workgroups of a neighbour launch given to the dense list (none without one).
 unsigned dense_blocks_of(const DenseList *dl, int32_t n, int nb) {
     if (!dl) return 0u;
@@ -1557,13 +1639,19 @@ void lambda_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, boo
         constexpr int NB = decltype(nbc)::value;
         constexpr bool PAD = decltype(padc)::value;
         const unsigned db = dense_blocks_of(dl, c.n, NB);
-        const unsigned grid = nb_blocks(c.n, NB) + db;
-        if (c.h_is_one)
-            k_lambda<true, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec, q, nbr9, lambda, pl.entries, pl.count,
-                                                              pl.cap, dlv, db, tp);
-        else
-            k_lambda<false, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec, q, nbr9, lambda, pl.entries, pl.count,
-                                                               pl.cap, dlv, db, tp);
+        const unsigned ntiles = nb_blocks(c.n, NB) + db;
+        with_lc<Dom>(ls.lc, [&](auto lcc) {
+            constexpr int LC = decltype(lcc)::value;
+            if (c.h_is_one) {
+                auto k = k_lambda<true, NB, PAD, Dom, LC>;
+                k<<<neighbour_grid(k, NB, ntiles, ls.persistent), NB, 0, st>>>(
+                    c, d, rec, q, nbr9, lambda, pl.entries, pl.count, pl.cap, dlv, db, tp, ntiles);
+            } else {
+                auto k = k_lambda<false, NB, PAD, Dom, LC>;
+                k<<<neighbour_grid(k, NB, ntiles, ls.persistent), NB, 0, st>>>(
+                    c, d, rec, q, nbr9, lambda, pl.entries, pl.count, pl.cap, dlv, db, tp, ntiles);
+            }
+        });
     });
 }
 template <class Dom>
@@ -1579,13 +1667,16 @@ void updatepos_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, 
         constexpr int NB = decltype(nbc)::value;
         constexpr bool PAD = decltype(padc)::value;
         const unsigned db = dense_blocks_of(dl, c.n, NB);
-        const unsigned grid = nb_blocks(c.n, NB) + db;
-        if (c.h_is_one)
-            k_updatepos<true, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec_in, lambda, nbr9, pl.entries, pl.count,
-                                                                 rec_out, q_out, dlv, db, upv, tp);
-        else
-            k_updatepos<false, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec_in, lambda, nbr9, pl.entries, pl.count,
-                                                                  rec_out, q_out, dlv, db, upv, tp);
+        const unsigned ntiles = nb_blocks(c.n, NB) + db;
+        if (c.h_is_one) {
+            auto k = k_updatepos<true, NB, PAD, Dom>;
+            k<<<neighbour_grid(k, NB, ntiles, ls.persistent), NB, 0, st>>>(
+                c, d, rec_in, lambda, nbr9, pl.entries, pl.count, rec_out, q_out, dlv, db, upv, tp, ntiles);
+        } else {
+            auto k = k_updatepos<false, NB, PAD, Dom>;
+            k<<<neighbour_grid(k, NB, ntiles, ls.persistent), NB, 0, st>>>(
+                c, d, rec_in, lambda, nbr9, pl.entries, pl.count, rec_out, q_out, dlv, db, upv, tp, ntiles);
+        }
     });
 }
 template <class Dom>
@@ -1598,14 +1689,21 @@ void vorticity_a_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls
         constexpr int NB = decltype(nbc)::value;
         constexpr bool PAD = decltype(padc)::value;
         const unsigned db = dense_blocks_of(dl, c.n, NB);
-        const unsigned grid = nb_blocks(c.n, NB) + db;
-        if (c.h_is_one)
-            k_vorticity_a<true, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec, q, nbr9, vel_sorted, vel_id, vxsph,
-                                                                   omega, vort, pl.entries, pl.count, pl.cap, dlv, db);
-        else
-            k_vorticity_a<false, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec, q, nbr9, vel_sorted, vel_id, vxsph,
-                                                                    omega, vort, pl.entries, pl.count, pl.cap, dlv,
-                                                                    db);
+        const unsigned ntiles = nb_blocks(c.n, NB) + db;
+        with_lc<Dom>(ls.lc, [&](auto lcc) {
+            constexpr int LC = decltype(lcc)::value;
+            if (c.h_is_one) {
+                auto k = k_vorticity_a<true, NB, PAD, Dom, LC>;
+                k<<<neighbour_grid(k, NB, ntiles, ls.persistent), NB, 0, st>>>(
+                    c, d, rec, q, nbr9, vel_sorted, vel_id, vxsph, omega, vort, pl.entries, pl.count, pl.cap, dlv, db,
+                    ntiles);
+            } else {
+                auto k = k_vorticity_a<false, NB, PAD, Dom, LC>;
+                k<<<neighbour_grid(k, NB, ntiles, ls.persistent), NB, 0, st>>>(
+                    c, d, rec, q, nbr9, vel_sorted, vel_id, vxsph, omega, vort, pl.entries, pl.count, pl.cap, dlv, db,
+                    ntiles);
+            }
+        });
     });
 }
 template <class Dom>
@@ -1620,13 +1718,16 @@ void vorticity_b_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls
         constexpr int NB = decltype(nbc)::value;
         constexpr bool PAD = decltype(padc)::value;
         const unsigned db = dense_blocks_of(dl, c.n, NB);
-        const unsigned grid = nb_blocks(c.n, NB) + db;
-        if (c.h_is_one)
-            k_vorticity_b<true, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec, nbr9, pl.entries, pl.count, vxsph, omega,
-                                                                   vort, vel_id, dlv, db, tp);
-        else
-            k_vorticity_b<false, NB, PAD, Dom><<<grid, NB, 0, st>>>(c, d, rec, nbr9, pl.entries, pl.count, vxsph,
-                                                                    omega, vort, vel_id, dlv, db, tp);
+        const unsigned ntiles = nb_blocks(c.n, NB) + db;
+        if (c.h_is_one) {
+            auto k = k_vorticity_b<true, NB, PAD, Dom>;
+            k<<<neighbour_grid(k, NB, ntiles, ls.persistent), NB, 0, st>>>(
+                c, d, rec, nbr9, pl.entries, pl.count, vxsph, omega, vort, vel_id, dlv, db, tp, ntiles);
+        } else {
+            auto k = k_vorticity_b<false, NB, PAD, Dom>;
+            k<<<neighbour_grid(k, NB, ntiles, ls.persistent), NB, 0, st>>>(
+                c, d, rec, nbr9, pl.entries, pl.count, vxsph, omega, vort, vel_id, dlv, db, tp, ntiles);
+        }
     });
 }
 }  // namespace

@@ -134,11 +134,16 @@ def test_autotune_restores_state_and_keeps_results(gpu_lib):
     e0 = tuned.end_tables()
     best, trials = tuned.autotune(1)
     say(f"[autotune] best {best}, {len(trials)} trials")
-    assert len(trials) == 80 and all(t["step_ms"] > 0 for t in trials)
+    # 16 neighbour shapes (width x list depth x persistent), then the 19 other streaming shapes
+    assert len(trials) == 35 and all(t["step_ms"] > 0 for t in trials)
+    assert {(t["neighbour_block"], t["list_depth"], t["persistent"]) for t in trials[:16]} == {
+        (nb, lc, p) for nb in (64, 128, 256, 512) for lc in (32, 56) for p in (0, 1)}
     p1, v1 = tuned.download()
     assert same(p0, p1) and same(v0, v1) and np.array_equal(e0, tuned.end_tables())
-    assert tuned.tuning() == {k: best[k] for k in ("neighbour_block", "stream_block", "stream_ppt")}
-    for shape in [(128, 64, 4), (512, 1024, 2), (256, 256, 1)]:
+    assert tuned.tuning() == {k: best[k] for k in ("neighbour_block", "stream_block", "stream_ppt", "list_depth",
+                                                    "persistent")}
+    # shapes change no bit: widths, streaming shapes, list depths, persistent launches
+    for shape in [(128, 64, 4), (512, 1024, 2, 32), (256, 256, 1, 56, 1), (64, 128, 3, 32, 1)]:
         tuned.set_tuning(*shape)
         tuned.step(1)
         ref.step(1)
@@ -157,4 +162,8 @@ def test_batch_rejects_bad_input(gpu_lib):
     b = B.Batch([sc])
     with pytest.raises(Exception):
         b.set_tuning(100, 256, 1)
+    with pytest.raises(Exception):
+        b.set_tuning(256, 256, 1, 40)
+    with pytest.raises(Exception):
+        b.set_tuning(256, 256, 1, 56, 2)
     b.close()
