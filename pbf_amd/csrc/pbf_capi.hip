@@ -40,7 +40,7 @@ int check_params(const pbf_params *p) {
     if (!p) return invalid("null parameters");
     if (!(p->h > 0.0f) || !std::isfinite(p->h)) return invalid("h must be positive and finite");
     if (!(p->timestep != 0.0f) || !std::isfinite(p->timestep)) return invalid("timestep must be nonzero and finite");
-    if (p->iterations > 1000) return invalid("iterations above 1000");
+    if (p->iterations > (uint32_t)kMaxIterations) return invalid("iterations above 1000");
     return PBF_OK;
 }
 
@@ -217,6 +217,7 @@ struct pbf_sph {
     float4 *rec0 = nullptr;                      // sorted records at x*
     float4 *rec_a = nullptr, *rec_b = nullptr;   // sorted records of the solver iterations (Jacobi)
     uint2 *qpos = nullptr;                       // quantized positions of the current sorted records
+    uint32_t *tiny = nullptr;                    // note_tiny flags of x* and of each iteration's positions
     float4 *rec = nullptr;                       // sorted records after the last iteration (borrowed)
     uint16_t *list_entries = nullptr, *list_count = nullptr;  // neighbours within h, lambda -> update
     DenseList dense;                             // the step's particles with the most candidates
@@ -255,6 +256,7 @@ struct pbf_sph {
         dfree(rec_a);
         dfree(rec_b);
         dfree(qpos);
+        dfree(tiny);
         rec = nullptr;
         dfree(list_entries);
         dfree(list_count);
@@ -398,6 +400,7 @@ int pbf_sph_create(pbf_sph **out, uint32_t n, int32_t gx, int32_t gy, int32_t gz
     TRY(dalloc(&s->rec_a, (size_t)n + kPad));
     TRY(dalloc(&s->rec_b, (size_t)n + kPad));
     TRY(dalloc(&s->qpos, (size_t)n + kPad));
+    TRY(dalloc(&s->tiny, (size_t)kMaxIterations + 1));
     s->rec = s->rec0;
     TRY(dalloc(&s->list_entries, (size_t)kListRows * n));
     TRY(dalloc(&s->list_count, n));
@@ -498,12 +501,14 @@ int sph_enqueue(pbf_sph *s, hipStream_t st, bool events) {
     PHASE(0);
     // predict positions (src/SPH.cpp:247-261) + cell keys
     // with the clear of the highlight words (clearhighlight.glsl, src/SPH.cpp:290-293)
-    TIMED(KID_PREDICT, launch_predict(c, s->pos, s->vel, s->xs, s->keys, st, s->highlight));
+    const uint32_t iters = s->params.iterations;
+    TIMED(KID_PREDICT, launch_predict(c, s->pos, s->vel, s->xs, s->keys, st, s->highlight, s->tiny, (int32_t)iters + 1));
     PHASE(1);
     // sort (src/SPH.cpp:263-268)
     // the sort's last pass writes the sorted records (globalsort.glsl:64 as a
     // gather) and their quantized positions
-    TIMED(KID_SORT, radix_sort_perm(s->rs, s->keys, n, s->key_bits, nullptr, st, s->xs, s->rec0, s->qpos, c.qscale));
+    TIMED(KID_SORT,
+          radix_sort_perm(s->rs, s->keys, n, s->key_bits, nullptr, st, s->xs, s->rec0, s->qpos, c.qscale, s->tiny));
     PHASE(2);
     // neighbour cells (src/SPH.cpp:270-275; src/NeighbourCellFinder.cpp:113-148)
     CellTables &t = s->cells_t;
@@ -526,12 +531,13 @@ int sph_enqueue(pbf_sph *s, hipStream_t st, bool events) {
     up.vel = s->vel;
     up.vel_sorted = s->params.vorticity ? s->vel_sorted : nullptr;
     const float4 *cur = s->rec0;
-    const uint32_t iters = s->params.iterations;
     for (uint32_t it = 0; it < iters; it++) {
         float4 *dst = (it & 1) ? s->rec_b : s->rec_a;
-        TIMED(KID_LAMBDA, launch_lambda(c, cur, s->qpos, t.nbr9, s->lambda, pl, st, true, &s->dense));
+        TIMED(KID_LAMBDA,
+              launch_lambda(c, cur, s->qpos, t.nbr9, s->lambda, pl, st, true, &s->dense, nullptr, s->tiny + it));
         TIMED(KID_UPDATEPOS, launch_updatepos(c, cur, s->lambda, t.nbr9, pl, dst, s->qpos, st, true, &s->dense,
-                                              it + 1 == iters ? &up : nullptr));
+                                              it + 1 == iters ? &up : nullptr, nullptr, s->tiny + it,
+                                              s->tiny + it + 1));
         cur = dst;
     }
     s->rec = const_cast<float4 *>(cur);
@@ -541,9 +547,10 @@ int sph_enqueue(pbf_sph *s, hipStream_t st, bool events) {
     if (s->params.vorticity) {
         TIMED(KID_VORTICITY_A,
               launch_vorticity_a(c, s->rec, s->qpos, t.nbr9, s->vel_sorted, s->vel, s->vxsph, s->omega, s->vort, pl,
-                                 st, true, &s->dense));
+                                 st, true, &s->dense, s->tiny + iters));
         TIMED(KID_VORTICITY_B,
-              launch_vorticity_b(c, s->rec, t.nbr9, pl, s->vxsph, s->omega, s->vort, s->vel, st, true, &s->dense));
+              launch_vorticity_b(c, s->rec, t.nbr9, pl, s->vxsph, s->omega, s->vort, s->vel, st, true, &s->dense,
+                                 nullptr, s->tiny + iters));
     }
 #undef TIMED
     PHASE(5);

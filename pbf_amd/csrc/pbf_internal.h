@@ -96,12 +96,29 @@ __device__ __forceinline__ uint2 quantize_pos(const float4 &r, float qscale) {
     return make_uint2(quantize1(r.x, qscale) | (quantize1(r.y, qscale) << 16), quantize1(r.z, qscale));
 }
 
+// A record with a coordinate in (0, 2^-25) in magnitude. When no record of a
+// set of positions has one (every coordinate is 0 or at least 2^-25, so every
+// coordinate difference is 0 or at least 2^-48 in magnitude), then for h = 1
+// every pair within h (0 < r < 1, so h - r >= 2^-24 and the spiky factor is 0
+// or at least 2^-45 in magnitude) has gradient numerators that are 0 or at
+// least 2^-93 and a distance of at least 2^-48: inside the fast ranges of the
+// division, whose range tests the pair passes can then skip. The writers of
+// each set of positions record in a flag word whether any record is tiny.
+__device__ __forceinline__ bool tiny_coord(float x) {
+    const float a = fabsf(x);
+    return (a > 0.0f) & (a < 0x1p-25f);
+}
+__device__ __forceinline__ void note_tiny(uint32_t *flag, const float4 &r) {
+    if (flag && (tiny_coord(r.x) | tiny_coord(r.y) | tiny_coord(r.z))) *flag = 1u;
+}
+
 // ------------------------------------------------------------ launchers
 // highlight (or null): the SPH handle's highlight words by id, of which the
 // predict pass keeps bit 0 (clearhighlight.glsl) and the neighbour-cell pass
 // then marks the candidates of the flagged particles (highlight.glsl).
+// tiny (or null): ntiny flag words cleared for the step (positions_tame).
 void launch_predict(const SolverConsts &c, const float4 *pos, const float4 *vel, float4 *rec, uint32_t *keys,
-                    hipStream_t st, uint32_t *highlight = nullptr);
+                    hipStream_t st, uint32_t *highlight = nullptr, uint32_t *tiny = nullptr, int32_t ntiny = 0);
 void launch_gather_records(int32_t n, const uint32_t *perm, const float4 *rec_in, float4 *rec_out,
                            hipStream_t st);
 // out[g] = src[id of rec[g]]
@@ -136,6 +153,8 @@ constexpr uint16_t kListNone = 0xFFFF;
 // records, lambda and |omega| arrays: a neighbour row starts below n and holds
 // fewer than 128 entries, scanned in batches of at most 130, so no index reaches n + kPad.
 constexpr int32_t kPad = 256;
+// Solver iterations a parameter block may ask for.
+constexpr int32_t kMaxIterations = 1000;
 
 struct PairList {
     uint16_t *entries = nullptr;
@@ -178,9 +197,12 @@ struct SolverTape {
 // q: the quantized positions of rec, slot for slot (launch_quantize, or the
 // sort's last pass and the position update's q_out), read by the candidate
 // scans of lambda and vorticity A; padded like rec.
+// tiny (or null): the flag word of rec's positions; when it is zero and h = 1,
+// the pair passes skip the division's range tests (every operand is inside
+// the fast ranges). Writers of positions set the flag of their output set.
 void launch_lambda(const SolverConsts &c, const float4 *rec, const uint2 *q, const int32_t *nbr9, float *lambda,
                    const PairList &pl, hipStream_t st, bool padded = false, const DenseList *dl = nullptr,
-                   const SolverTape *tape = nullptr);
+                   const SolverTape *tape = nullptr, const uint32_t *tiny = nullptr);
 // The step's closing update (update.glsl:16-28: velocities from the
 // positions before and after the step, both stored by particle id; vel_sorted,
 // if set, also keeps the velocities in sorted order). Given to the last
@@ -195,17 +217,20 @@ struct StepUpdate {
 void launch_updatepos(const SolverConsts &c, const float4 *rec_in, const float *lambda, const int32_t *nbr9,
                       const PairList &pl, float4 *rec_out, uint2 *q_out, hipStream_t st, bool padded = false,
                       const DenseList *dl = nullptr, const StepUpdate *up = nullptr,
-                      const SolverTape *tape = nullptr);
+                      const SolverTape *tape = nullptr, const uint32_t *tiny_in = nullptr,
+                      uint32_t *tiny_out = nullptr);
 void launch_update(const SolverConsts &c, const float4 *rec, float4 *pos, float4 *vel, float4 *vel_sorted,
                    hipStream_t st);
 void launch_vorticity_a(const SolverConsts &c, const float4 *rec, const uint2 *q, const int32_t *nbr9,
                         const float4 *vel_sorted, const float4 *vel_id, float4 *vxsph, float4 *omega, float *vort,
-                        const PairList &pl, hipStream_t st, bool padded = false, const DenseList *dl = nullptr);
+                        const PairList &pl, hipStream_t st, bool padded = false, const DenseList *dl = nullptr,
+                        const uint32_t *tiny = nullptr);
 // q[i] = the quantized position of rec[i], i < count.
 void launch_quantize(const SolverConsts &c, const float4 *rec, uint2 *q, int32_t count, hipStream_t st);
 void launch_vorticity_b(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, const PairList &pl,
                         const float4 *vxsph, const float4 *omega, const float *vort, float4 *vel_id, hipStream_t st,
-                        bool padded = false, const DenseList *dl = nullptr, const SolverTape *tape = nullptr);
+                        bool padded = false, const DenseList *dl = nullptr, const SolverTape *tape = nullptr,
+                        const uint32_t *tiny = nullptr);
 // Launch shape of the batched form: workgroup size of the neighbour passes
 // (64, 128, 256, 512), workgroup size (64 .. 1024, a power of two) and particles
 // per thread (1 .. 4) of the streaming passes, the scanning passes' LDS list
@@ -266,9 +291,10 @@ void radix_free(RadixScratch &s);
 // must not alias the scratch buffers. With gather_dst, the last pass also
 // writes gather_dst[k] = gather_src[perm[k]] (the sorted records), in place of
 // a separate gather, and with q_dst, their quantized positions (scale qscale).
+// With tiny_dst, the flag word of the sorted records (note_tiny).
 void radix_sort_perm(RadixScratch &s, uint32_t *keys, int32_t n, int key_bits, uint32_t *perm, hipStream_t st,
                      const float4 *gather_src = nullptr, float4 *gather_dst = nullptr, uint2 *q_dst = nullptr,
-                     float qscale = 0.0f);
+                     float qscale = 0.0f, uint32_t *tiny_dst = nullptr);
 
 // ------------------------------------------------------------ errors and helpers
 void set_error(const std::string &msg);

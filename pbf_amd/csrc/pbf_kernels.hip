@@ -51,11 +51,26 @@ __device__ __forceinline__ float wpoly6(const SolverConsts &c, float sq, float r
 // no result depends on that sign.
 template <bool H1>
 __device__ __forceinline__ void grad_spiky(const SolverConsts &c, float dx, float dy, float dz, float l, float &gx,
-                                           float &gy, float &gz) {
+                                           float &gy, float &gz, bool tame = false) {
     const float t = c.h - l;
     const float s = (SPIKY_GRAD * t) * t;
     const float den = H1 ? l : (((((l * c.h) * c.h) * c.h) * c.h) * c.h) * c.h;
+    if (H1 && tame) {
+        // every operand inside the fast ranges (positions_tame): no range tests
+        const float y = rcp_rn_fast(den);
+        gx = div_markstein<false>(s * dx, den, y);
+        gy = div_markstein<false>(s * dy, den, y);
+        gz = div_markstein<false>(s * dz, den, y);
+        return;
+    }
     div3_rn<H1, false>(s * dx, s * dy, s * dz, den, gx, gy, gz);
+}
+
+// The positions read by a launch are tame (h = 1 and no tiny record, see
+// note_tiny): grad_spiky may skip the division's range tests.
+template <bool H1>
+__device__ __forceinline__ bool positions_tame(const uint32_t *flag) {
+    return H1 && flag && *flag == 0u;
 }
 
 __device__ __forceinline__ float clamp_glsl(float x, float lo, float hi) {
@@ -120,10 +135,12 @@ __device__ __forceinline__ void predict_one(const SolverConsts &c, const Dom &do
 template <class Dom>
 __global__ __launch_bounds__(1024) void k_predict(SolverConsts c, Dom dom, int32_t ppt, const float4 *__restrict__ pos,
                                                   const float4 *__restrict__ vel, float4 *__restrict__ rec,
-                                                  uint32_t *__restrict__ keys, uint32_t *__restrict__ hl) {
+                                                  uint32_t *__restrict__ keys, uint32_t *__restrict__ hl,
+                                                  uint32_t *__restrict__ tiny, int32_t ntiny) {
     PBF_STREAM_FOR(i) {
         predict_one(c, dom, i, pos, vel, rec, keys);
         if (hl) hl[i] = hl[i] & 1u;
+        for (int32_t k = i; k < ntiny; k += c.n) tiny[k] = 0u;  // the step's tiny-coordinate flags
     }
 }
 
@@ -937,6 +954,7 @@ struct LambdaPass {
     const float4 *__restrict__ rec;
     float *__restrict__ lambda;
     SolverTape tp;
+    bool tame;  // positions_tame: the division's range tests can be skipped
     float4 pi;
     __device__ __forceinline__ void begin(int32_t i, const Seg &) { pi = rec[i]; }
     __device__ __forceinline__ int extra(const Seg &, int32_t) const { return 0; }
@@ -949,7 +967,7 @@ struct LambdaPass {
         P[0] = wpoly6<H1>(c, sq, r);
         if (r == 0.0f) return true;
         float gx, gy, gz;
-        grad_spiky<H1>(c, dx, dy, dz, r, gx, gy, gz);
+        grad_spiky<H1>(c, dx, dy, dz, r, gx, gy, gz, tame);
         gx = gx * c.inv_rho0;
         gy = gy * c.inv_rho0;
         gz = gz * c.inv_rho0;
@@ -993,6 +1011,8 @@ struct UpdatePosPass {
     uint2 *__restrict__ q_out;  // quantized positions of rec_out, or none
     StepUpdate up;  // the step's closing update (last iteration), or none
     SolverTape tp;
+    bool tame;                  // positions_tame of rec_in
+    uint32_t *__restrict__ tiny_out;  // note_tiny flag of rec_out, or none
     float4 pi;
     float li;
     __device__ __forceinline__ void begin(int32_t i, const Seg &) {
@@ -1011,7 +1031,7 @@ struct UpdatePosPass {
         sc *= sc;
         sc = -c.k * sc;
         float gx, gy, gz;
-        grad_spiky<H1>(c, dx, dy, dz, r, gx, gy, gz);
+        grad_spiky<H1>(c, dx, dy, dz, r, gx, gy, gz, tame);
         const float f = li + lj + sc;
         P[0] = f * gx;
         P[1] = f * gy;
@@ -1029,6 +1049,7 @@ struct UpdatePosPass {
         const float4 r = make_float4(x, y, z, pi.w);
         rec_out[i] = r;
         if (q_out) q_out[i] = quantize_pos(r, c.qscale);
+        note_tiny(tiny_out, r);
         if (up.pos) update_one(c, sg, i, r, up);
     }
 };
@@ -1045,6 +1066,7 @@ struct VorticityAPass {
     float4 *__restrict__ vxsph;
     float4 *__restrict__ omega;
     float *__restrict__ vort;
+    bool tame;
     float4 pi, vi;
     __device__ __forceinline__ void begin(int32_t i, const Seg &) {
         pi = rec[i];
@@ -1066,7 +1088,7 @@ struct VorticityAPass {
         P[2] = uz * w;
         if (r == 0.0f) return true;
         float gx, gy, gz;
-        grad_spiky<H1>(c, px, py, pz, r, gx, gy, gz);
+        grad_spiky<H1>(c, px, py, pz, r, gx, gy, gz, tame);
         P[3] = uy * gz;
         M[0] = uz * gy;
         P[4] = uz * gx;
@@ -1094,6 +1116,7 @@ struct VorticityBPass {
     const float *__restrict__ vort;
     float4 *__restrict__ vel_id;
     SolverTape tp;
+    bool tame;
     float4 pi;
     __device__ __forceinline__ void begin(int32_t i, const Seg &) { pi = rec[i]; }
     __device__ __forceinline__ float extra(const Seg &sg, int32_t j) const { return val_at<PAD>(vort, sg, j); }
@@ -1104,7 +1127,7 @@ struct VorticityBPass {
         const float r = sqrt_rn_finite(sq);
         if ((!listed && r > c.h) || r == 0.0f) return false;
         float gx, gy, gz;
-        grad_spiky<H1>(c, px, py, pz, r, gx, gy, gz);
+        grad_spiky<H1>(c, px, py, pz, r, gx, gy, gz, tame);
         P[0] = om * gx;
         P[1] = om * gy;
         P[2] = om * gz;
@@ -1142,18 +1165,20 @@ __global__ __launch_bounds__(NB) void k_lambda(SolverConsts c, Dom dom, const fl
                                                      const int32_t *__restrict__ nbr9, float *__restrict__ lambda,
                                                      uint16_t *__restrict__ list_entries,
                                                      uint16_t *__restrict__ list_count, int32_t list_cap,
-                                                     DenseList dl, uint32_t dense_blocks, SolverTape tp, uint32_t ntiles) {
+                                                     DenseList dl, uint32_t dense_blocks, SolverTape tp, uint32_t ntiles,
+                                                     const uint32_t *__restrict__ tiny) {
     __shared__ union {
         NbrLds<NB, LC> light;
         DenseLds<5, 0, NB / 64> dense;
     } u;
+    const bool tame = positions_tame<H1>(tiny);
     // tile bid of ntiles (one per workgroup, or several per workgroup of a
     // persistent launch, whose grid is a multiple of 8 so that tile bid runs on
     // XCD bid % 8 as xcd_block assumes)
     auto tile = [&](const uint32_t bid) {
         {
             if (bid < dense_blocks) {
-                LambdaPass<H1, PAD> ps{c, rec, lambda, tp, {}};
+                LambdaPass<H1, PAD> ps{c, rec, lambda, tp, tame, {}};
                 dense_pass<LambdaPass<H1, PAD>, PAD, NB / 64>(c, dom, rec, nbr9, dl, list_entries, list_count, list_cap, ps,
                                                      u.dense, bid, dense_blocks);
                 return;
@@ -1178,7 +1203,7 @@ __global__ __launch_bounds__(NB) void k_lambda(SolverConsts c, Dom dom, const fl
             rho += wpoly6<H1>(c, sq, r);
             if (r == 0.0f) return;
             float gx, gy, gz;
-            grad_spiky<H1>(c, dx, dy, dz, r, gx, gy, gz);
+            grad_spiky<H1>(c, dx, dy, dz, r, gx, gy, gz, tame);
             gx = gx * c.inv_rho0;
             gy = gy * c.inv_rho0;
             gz = gz * c.inv_rho0;
@@ -1216,18 +1241,21 @@ __global__ __launch_bounds__(NB) void k_updatepos(SolverConsts c, Dom dom, const
                                                         const uint16_t *__restrict__ list_count,
                                                         float4 *__restrict__ rec_out, uint2 *__restrict__ q_out,
                                                         DenseList dl, uint32_t dense_blocks, StepUpdate up,
-                                                        SolverTape tp, uint32_t ntiles) {
+                                                        SolverTape tp, uint32_t ntiles,
+                                                        const uint32_t *__restrict__ tiny_in,
+                                                        uint32_t *__restrict__ tiny_out) {
     __shared__ union {
         NbrLds<NB, kReplayLcap> light;
         DenseLds<3, 0, NB / 64> dense;
     } u;
+    const bool tame = positions_tame<H1>(tiny_in);
     // tile bid of ntiles (one per workgroup, or several per workgroup of a
     // persistent launch, whose grid is a multiple of 8 so that tile bid runs on
     // XCD bid % 8 as xcd_block assumes)
     auto tile = [&](const uint32_t bid) {
         {
             if (bid < dense_blocks) {
-                UpdatePosPass<H1, PAD> ps{c, rec_in, lambda, rec_out, q_out, up, tp, {}, 0.0f};
+                UpdatePosPass<H1, PAD> ps{c, rec_in, lambda, rec_out, q_out, up, tp, tame, tiny_out, {}, 0.0f};
                 dense_pass<UpdatePosPass<H1, PAD>, PAD, NB / 64>(c, dom, rec_in, nbr9, dl, const_cast<uint16_t *>(list_entries),
                                                         const_cast<uint16_t *>(list_count), 0, ps, u.dense, bid,
                                                         dense_blocks);
@@ -1255,7 +1283,7 @@ __global__ __launch_bounds__(NB) void k_updatepos(SolverConsts c, Dom dom, const
             sc *= sc;
             sc = -c.k * sc;
             float gx, gy, gz;
-            grad_spiky<H1>(c, dx, dy, dz, r, gx, gy, gz);
+            grad_spiky<H1>(c, dx, dy, dz, r, gx, gy, gz, tame);
             const float f = li + lj + sc;
             dpx += f * gx;
             dpy += f * gy;
@@ -1271,6 +1299,7 @@ __global__ __launch_bounds__(NB) void k_updatepos(SolverConsts c, Dom dom, const
         const float4 r = make_float4(x, y, z, pi.w);
         rec_out[i] = r;
         if (q_out) q_out[i] = quantize_pos(r, c.qscale);
+        note_tiny(tiny_out, r);
         if (up.pos) update_one(c, sg, i, r, up);
     };
     if (std::is_same<Dom, DomOne>::value || gridDim.x >= ntiles) {
@@ -1305,18 +1334,20 @@ __global__ __launch_bounds__(NB) void k_vorticity_a(SolverConsts c, Dom dom, con
                                                           float4 *__restrict__ vxsph, float4 *__restrict__ omega,
                                                           float *__restrict__ vort, uint16_t *__restrict__ list_entries,
                                                           uint16_t *__restrict__ list_count, int32_t list_cap,
-                                                          DenseList dl, uint32_t dense_blocks, uint32_t ntiles) {
+                                                          DenseList dl, uint32_t dense_blocks, uint32_t ntiles,
+                                                          const uint32_t *__restrict__ tiny) {
     __shared__ union {
         NbrLds<NB, LC> light;
         DenseLds<6, 3, NB / 64> dense;
     } u;
+    const bool tame = positions_tame<H1>(tiny);
     // tile bid of ntiles (one per workgroup, or several per workgroup of a
     // persistent launch, whose grid is a multiple of 8 so that tile bid runs on
     // XCD bid % 8 as xcd_block assumes)
     auto tile = [&](const uint32_t bid) {
         {
             if (bid < dense_blocks) {
-                VorticityAPass<H1, PAD> ps{c, rec, vel_sorted, vel_id, vxsph, omega, vort, {}, {}};
+                VorticityAPass<H1, PAD> ps{c, rec, vel_sorted, vel_id, vxsph, omega, vort, tame, {}, {}};
                 dense_pass<VorticityAPass<H1, PAD>, PAD, NB / 64>(c, dom, rec, nbr9, dl, list_entries, list_count, list_cap, ps,
                                                          u.dense, bid, dense_blocks);
                 return;
@@ -1348,7 +1379,7 @@ __global__ __launch_bounds__(NB) void k_vorticity_a(SolverConsts c, Dom dom, con
             sz += uz * w;
             if (r == 0.0f) return;
             float gx, gy, gz;
-            grad_spiky<H1>(c, px, py, pz, r, gx, gy, gz);
+            grad_spiky<H1>(c, px, py, pz, r, gx, gy, gz, tame);
             wx = (wx + uy * gz) - uz * gy;
             wy = (wy + uz * gx) - ux * gz;
             wz = (wz + ux * gy) - uy * gx;
@@ -1379,18 +1410,20 @@ __global__ __launch_bounds__(NB) void k_vorticity_b(SolverConsts c, Dom dom, con
                                                           const float4 *__restrict__ omega,
                                                           const float *__restrict__ vort,
                                                           float4 *__restrict__ vel_id, DenseList dl,
-                                                          uint32_t dense_blocks, SolverTape tp, uint32_t ntiles) {
+                                                          uint32_t dense_blocks, SolverTape tp, uint32_t ntiles,
+                                                          const uint32_t *__restrict__ tiny) {
     __shared__ union {
         NbrLds<NB, kReplayLcap> light;
         DenseLds<3, 0, NB / 64> dense;
     } u;
+    const bool tame = positions_tame<H1>(tiny);
     // tile bid of ntiles (one per workgroup, or several per workgroup of a
     // persistent launch, whose grid is a multiple of 8 so that tile bid runs on
     // XCD bid % 8 as xcd_block assumes)
     auto tile = [&](const uint32_t bid) {
         {
             if (bid < dense_blocks) {
-                VorticityBPass<H1, PAD> ps{c, rec, vxsph, omega, vort, vel_id, tp, {}};
+                VorticityBPass<H1, PAD> ps{c, rec, vxsph, omega, vort, vel_id, tp, tame, {}};
                 dense_pass<VorticityBPass<H1, PAD>, PAD, NB / 64>(c, dom, rec, nbr9, dl, const_cast<uint16_t *>(list_entries),
                                                          const_cast<uint16_t *>(list_count), 0, ps, u.dense, bid,
                                                          dense_blocks);
@@ -1413,7 +1446,7 @@ __global__ __launch_bounds__(NB) void k_vorticity_b(SolverConsts c, Dom dom, con
             const float r = sqrt_rn_finite(sq);
             if ((!listed && r > c.h) || r == 0.0f) return;
             float gx, gy, gz;
-            grad_spiky<H1>(c, px, py, pz, r, gx, gy, gz);
+            grad_spiky<H1>(c, px, py, pz, r, gx, gy, gz, tame);
             nx += om * gx;
             ny += om * gy;
             nz += om * gz;
@@ -1564,8 +1597,10 @@ void with_nb(int nb, bool pad, F &&f) {
 
 template <class Dom>
 void predict_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, const float4 *pos, const float4 *vel,
-                  float4 *rec, uint32_t *keys, hipStream_t st, uint32_t *hl = nullptr) {
-    if (c.n > 0) k_predict<Dom><<<stream_blocks(c.n, ls), ls.sb, 0, st>>>(c, d, ls.ppt, pos, vel, rec, keys, hl);
+                  float4 *rec, uint32_t *keys, hipStream_t st, uint32_t *hl = nullptr, uint32_t *tiny = nullptr,
+                  int32_t ntiny = 0) {
+    if (c.n > 0)
+        k_predict<Dom><<<stream_blocks(c.n, ls), ls.sb, 0, st>>>(c, d, ls.ppt, pos, vel, rec, keys, hl, tiny, ntiny);
 }
 template <class Dom>
 void findcells_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, const float4 *sorted, int32_t *start,
@@ -1631,7 +1666,7 @@ void update_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, con
 template <class Dom>
 void lambda_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, bool pad, const float4 *rec,
                  const uint2 *q, const int32_t *nbr9, float *lambda, const PairList &pl, hipStream_t st,
-                 const DenseList *dl = nullptr, const SolverTape *tape = nullptr) {
+                 const DenseList *dl = nullptr, const SolverTape *tape = nullptr, const uint32_t *tiny = nullptr) {
     if (c.n <= 0) return;
     const DenseList dlv = dl ? *dl : DenseList();
     const SolverTape tp = tape ? *tape : SolverTape();
@@ -1645,11 +1680,11 @@ void lambda_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, boo
             if (c.h_is_one) {
                 auto k = k_lambda<true, NB, PAD, Dom, LC>;
                 k<<<neighbour_grid(k, NB, ntiles, ls.persistent), NB, 0, st>>>(
-                    c, d, rec, q, nbr9, lambda, pl.entries, pl.count, pl.cap, dlv, db, tp, ntiles);
+                    c, d, rec, q, nbr9, lambda, pl.entries, pl.count, pl.cap, dlv, db, tp, ntiles, tiny);
             } else {
                 auto k = k_lambda<false, NB, PAD, Dom, LC>;
                 k<<<neighbour_grid(k, NB, ntiles, ls.persistent), NB, 0, st>>>(
-                    c, d, rec, q, nbr9, lambda, pl.entries, pl.count, pl.cap, dlv, db, tp, ntiles);
+                    c, d, rec, q, nbr9, lambda, pl.entries, pl.count, pl.cap, dlv, db, tp, ntiles, tiny);
             }
         });
     });
@@ -1658,7 +1693,8 @@ template <class Dom>
 void updatepos_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, bool pad, const float4 *rec_in,
                     const float *lambda, const int32_t *nbr9, const PairList &pl, float4 *rec_out, uint2 *q_out,
                     hipStream_t st, const DenseList *dl = nullptr, const StepUpdate *up = nullptr,
-                    const SolverTape *tape = nullptr) {
+                    const SolverTape *tape = nullptr, const uint32_t *tiny_in = nullptr,
+                    uint32_t *tiny_out = nullptr) {
     if (c.n <= 0) return;
     const DenseList dlv = dl ? *dl : DenseList();
     const StepUpdate upv = up ? *up : StepUpdate();
@@ -1671,18 +1707,20 @@ void updatepos_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, 
         if (c.h_is_one) {
             auto k = k_updatepos<true, NB, PAD, Dom>;
             k<<<neighbour_grid(k, NB, ntiles, ls.persistent), NB, 0, st>>>(
-                c, d, rec_in, lambda, nbr9, pl.entries, pl.count, rec_out, q_out, dlv, db, upv, tp, ntiles);
+                c, d, rec_in, lambda, nbr9, pl.entries, pl.count, rec_out, q_out, dlv, db, upv, tp, ntiles, tiny_in,
+                tiny_out);
         } else {
             auto k = k_updatepos<false, NB, PAD, Dom>;
             k<<<neighbour_grid(k, NB, ntiles, ls.persistent), NB, 0, st>>>(
-                c, d, rec_in, lambda, nbr9, pl.entries, pl.count, rec_out, q_out, dlv, db, upv, tp, ntiles);
+                c, d, rec_in, lambda, nbr9, pl.entries, pl.count, rec_out, q_out, dlv, db, upv, tp, ntiles, tiny_in,
+                tiny_out);
         }
     });
 }
 template <class Dom>
 void vorticity_a_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, bool pad, const float4 *rec,
                       const uint2 *q, const int32_t *nbr9, const float4 *vel_sorted, const float4 *vel_id, float4 *vxsph,
-                      float4 *omega, float *vort, const PairList &pl, hipStream_t st, const DenseList *dl = nullptr) {
+                      float4 *omega, float *vort, const PairList &pl, hipStream_t st, const DenseList *dl = nullptr, const uint32_t *tiny = nullptr) {
     if (c.n <= 0) return;
     const DenseList dlv = dl ? *dl : DenseList();
     with_nb<Dom>(ls.nb, pad, [&](auto nbc, auto padc) {
@@ -1696,12 +1734,12 @@ void vorticity_a_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls
                 auto k = k_vorticity_a<true, NB, PAD, Dom, LC>;
                 k<<<neighbour_grid(k, NB, ntiles, ls.persistent), NB, 0, st>>>(
                     c, d, rec, q, nbr9, vel_sorted, vel_id, vxsph, omega, vort, pl.entries, pl.count, pl.cap, dlv, db,
-                    ntiles);
+                    ntiles, tiny);
             } else {
                 auto k = k_vorticity_a<false, NB, PAD, Dom, LC>;
                 k<<<neighbour_grid(k, NB, ntiles, ls.persistent), NB, 0, st>>>(
                     c, d, rec, q, nbr9, vel_sorted, vel_id, vxsph, omega, vort, pl.entries, pl.count, pl.cap, dlv, db,
-                    ntiles);
+                    ntiles, tiny);
             }
         });
     });
@@ -1710,7 +1748,7 @@ template <class Dom>
 void vorticity_b_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, bool pad, const float4 *rec,
                       const int32_t *nbr9, const PairList &pl, const float4 *vxsph, const float4 *omega,
                       const float *vort, float4 *vel_id, hipStream_t st, const DenseList *dl = nullptr,
-                      const SolverTape *tape = nullptr) {
+                      const SolverTape *tape = nullptr, const uint32_t *tiny = nullptr) {
     if (c.n <= 0) return;
     const DenseList dlv = dl ? *dl : DenseList();
     const SolverTape tp = tape ? *tape : SolverTape();
@@ -1722,11 +1760,11 @@ void vorticity_b_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls
         if (c.h_is_one) {
             auto k = k_vorticity_b<true, NB, PAD, Dom>;
             k<<<neighbour_grid(k, NB, ntiles, ls.persistent), NB, 0, st>>>(
-                c, d, rec, nbr9, pl.entries, pl.count, vxsph, omega, vort, vel_id, dlv, db, tp, ntiles);
+                c, d, rec, nbr9, pl.entries, pl.count, vxsph, omega, vort, vel_id, dlv, db, tp, ntiles, tiny);
         } else {
             auto k = k_vorticity_b<false, NB, PAD, Dom>;
             k<<<neighbour_grid(k, NB, ntiles, ls.persistent), NB, 0, st>>>(
-                c, d, rec, nbr9, pl.entries, pl.count, vxsph, omega, vort, vel_id, dlv, db, tp, ntiles);
+                c, d, rec, nbr9, pl.entries, pl.count, vxsph, omega, vort, vel_id, dlv, db, tp, ntiles, tiny);
         }
     });
 }
@@ -1734,8 +1772,8 @@ void vorticity_b_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls
 
 // one scene
 void launch_predict(const SolverConsts &c, const float4 *pos, const float4 *vel, float4 *rec, uint32_t *keys,
-                    hipStream_t st, uint32_t *highlight) {
-    predict_impl(c, one(c), kDefaultShape, pos, vel, rec, keys, st, highlight);
+                    hipStream_t st, uint32_t *highlight, uint32_t *tiny, int32_t ntiny) {
+    predict_impl(c, one(c), kDefaultShape, pos, vel, rec, keys, st, highlight, tiny, ntiny);
 }
 void launch_keys_of_records(const SolverConsts &c, const float4 *rec, uint32_t *keys, hipStream_t st) {
     if (c.n > 0) k_keys<DomOne><<<blocks_for(c.n), BLOCK, 0, st>>>(c, one(c), rec, keys);
@@ -1765,13 +1803,16 @@ void launch_neighbourcells(const SolverConsts &c, const float4 *sorted, const in
     neighbourcells_impl(c, one(c), kDefaultShape, sorted, start, end64, nbr9, st, dl, highlight);
 }
 void launch_lambda(const SolverConsts &c, const float4 *rec, const uint2 *q, const int32_t *nbr9, float *lambda,
-                   const PairList &pl, hipStream_t st, bool padded, const DenseList *dl, const SolverTape *tape) {
-    lambda_impl(c, one(c), kDefaultShape, padded, rec, q, nbr9, lambda, pl, st, dl, tape);
+                   const PairList &pl, hipStream_t st, bool padded, const DenseList *dl, const SolverTape *tape,
+                   const uint32_t *tiny) {
+    lambda_impl(c, one(c), kDefaultShape, padded, rec, q, nbr9, lambda, pl, st, dl, tape, tiny);
 }
 void launch_updatepos(const SolverConsts &c, const float4 *rec_in, const float *lambda, const int32_t *nbr9,
                       const PairList &pl, float4 *rec_out, uint2 *q_out, hipStream_t st, bool padded,
-                      const DenseList *dl, const StepUpdate *up, const SolverTape *tape) {
-    updatepos_impl(c, one(c), kDefaultShape, padded, rec_in, lambda, nbr9, pl, rec_out, q_out, st, dl, up, tape);
+                      const DenseList *dl, const StepUpdate *up, const SolverTape *tape, const uint32_t *tiny_in,
+                      uint32_t *tiny_out) {
+    updatepos_impl(c, one(c), kDefaultShape, padded, rec_in, lambda, nbr9, pl, rec_out, q_out, st, dl, up, tape,
+                   tiny_in, tiny_out);
 }
 void launch_update(const SolverConsts &c, const float4 *rec, float4 *pos, float4 *vel, float4 *vel_sorted,
                    hipStream_t st) {
@@ -1779,17 +1820,17 @@ void launch_update(const SolverConsts &c, const float4 *rec, float4 *pos, float4
 }
 void launch_vorticity_a(const SolverConsts &c, const float4 *rec, const uint2 *q, const int32_t *nbr9,
                         const float4 *vel_sorted, const float4 *vel_id, float4 *vxsph, float4 *omega, float *vort,
-                        const PairList &pl, hipStream_t st, bool padded, const DenseList *dl) {
+                        const PairList &pl, hipStream_t st, bool padded, const DenseList *dl, const uint32_t *tiny) {
     vorticity_a_impl(c, one(c), kDefaultShape, padded, rec, q, nbr9, vel_sorted, vel_id, vxsph, omega, vort, pl, st,
-                     dl);
+                     dl, tiny);
 }
 void launch_quantize(const SolverConsts &c, const float4 *rec, uint2 *q, int32_t count, hipStream_t st) {
     if (count > 0) k_quantize<<<blocks_for(count), BLOCK, 0, st>>>(c.qscale, rec, q, count);
 }
 void launch_vorticity_b(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, const PairList &pl,
                         const float4 *vxsph, const float4 *omega, const float *vort, float4 *vel_id, hipStream_t st,
-                        bool padded, const DenseList *dl, const SolverTape *tape) {
-    vorticity_b_impl(c, one(c), kDefaultShape, padded, rec, nbr9, pl, vxsph, omega, vort, vel_id, st, dl, tape);
+                        bool padded, const DenseList *dl, const SolverTape *tape, const uint32_t *tiny) {
+    vorticity_b_impl(c, one(c), kDefaultShape, padded, rec, nbr9, pl, vxsph, omega, vort, vel_id, st, dl, tape, tiny);
 }
 
 // many scenes

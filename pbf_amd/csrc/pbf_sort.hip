@@ -107,7 +107,8 @@ __global__ __launch_bounds__(RS_THREADS) void k_downsweep(const uint32_t *__rest
                                                           uint32_t *__restrict__ vals_out,
                                                           const float4 *__restrict__ gather_src,
                                                           float4 *__restrict__ gather_dst,
-                                                          uint2 *__restrict__ q_dst, float qscale) {
+                                                          uint2 *__restrict__ q_dst, float qscale,
+                                                          uint32_t *__restrict__ tiny_dst) {
     __shared__ uint32_t cnt[RS_ITEMS * RS_WAVES][256];
     __shared__ uint32_t wsum[RS_WAVES];
     for (int s = 0; s < RS_ITEMS * RS_WAVES; s++) cnt[s][threadIdx.x] = 0;
@@ -154,6 +155,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_downsweep(const uint32_t *__rest
             const float4 r = gather_src[val[k]];
             gather_dst[dst] = r;
             if (q_dst) q_dst[dst] = quantize_pos(r, qscale);
+            note_tiny(tiny_dst, r);
         }
     }
 }
@@ -184,7 +186,7 @@ void radix_free(RadixScratch &s) {
 }
 
 void radix_sort_perm(RadixScratch &s, uint32_t *keys, int32_t n, int key_bits, uint32_t *perm, hipStream_t st,
-                     const float4 *gather_src, float4 *gather_dst, uint2 *q_dst, float qscale) {
+                     const float4 *gather_src, float4 *gather_dst, uint2 *q_dst, float qscale, uint32_t *tiny_dst) {
     if (n <= 0) return;
     const int32_t tiles = (n + RS_TILE - 1) / RS_TILE;
     const int passes = key_bits <= 8 ? 1 : (key_bits + 7) / 8;
@@ -198,7 +200,7 @@ void radix_sort_perm(RadixScratch &s, uint32_t *keys, int32_t n, int key_bits, u
         hipLaunchKernelGGL(k_scan_rows, dim3(256), dim3(RS_THREADS), 0, st, s.counts, tiles, s.totals);
         hipLaunchKernelGGL(k_downsweep, dim3(tiles), dim3(RS_THREADS), 0, st, kin, vin, n, 8 * p, tiles,
                            s.counts, s.totals, last ? nullptr : kout, vout, last ? gather_src : nullptr,
-                           last ? gather_dst : nullptr, last ? q_dst : nullptr, qscale);
+                           last ? gather_dst : nullptr, last ? q_dst : nullptr, qscale, last ? tiny_dst : nullptr);
         vin = vout;
         uint32_t *t = kin;
         kin = kout;

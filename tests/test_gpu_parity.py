@@ -277,6 +277,29 @@ def test_candidate_filter_keeps_pairs_at_h(gpu_lib):
     assert within.mean() > 0.4 and (~within).mean() > 0.2, f"{within.mean():.2f} of the particles have a partner"
 
 
+def test_tiny_coordinates_rollout(gpu_lib):
+    """The pair passes skip the division's range tests only when no position of the set they read
+    has a coordinate in (0, 2^-25) in magnitude. Here, without gravity or initial velocity (x* = x),
+    a layer of the block sits at y = 1e-9 .. 2.9e-8 above the floor and a column at z = 1e-12 in a
+    grid whose walls start at 0, so the first iterations read tiny coordinates; 6 steps must equal
+    the oracle bit for bit."""
+    sc = S.dam_break((16, 8, 16), (64, 32, 64), steps=6, name="tiny")
+    sc.params.gravity = 0.0
+    sc.params.wall_lo = (0.0, 0.0, 0.0)
+    sc.params.wall_hi_margin = (0.0, 0.0, 0.0)
+    sc.pos = sc.pos.copy()
+    low = sc.pos[:, 1] < 0.5
+    sc.pos[low, 1] = np.linspace(1e-9, 2.9e-8, int(low.sum())).astype(np.float32)
+    col = np.arange(sc.n) % 97 == 0
+    sc.pos[col, 2] = np.float32(1e-12)
+    assert np.any((np.abs(sc.pos[:, :3]) > 0) & (np.abs(sc.pos[:, :3]) < 2.0 ** -25))
+    say(f"[tiny coordinates] n={sc.n}")
+    _, pos, vel = run_sph(sc, 6)
+    wpos, wvel, _ = oracle_rollout(sc, 6)
+    assert_bitexact(pos, wpos, "tiny-coordinate positions")
+    assert_bitexact(vel, wvel, "tiny-coordinate velocities")
+
+
 @pytest.mark.parametrize("index", [0, 97, 200])
 def test_rotated_gravity_one_cell_walls_rollout(gpu_lib, index):
     """The plain handle on scenes of the batched workload: gravity turned off the y axis, walls one
