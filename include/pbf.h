@@ -384,7 +384,8 @@ int pbf_stage_vorticity(uint32_t n, const int32_t grid[3], const pbf_params *p, 
  * cannot allocate device memory (code under HIP graph capture, serving loops,
  * harnesses that forbid hipMalloc). Same arguments and results; it allocates
  * and frees nothing and does not synchronize the stream. Its scratch (the
- * [9][n] neighbour table and three float4 arrays, 84 bytes per particle) is
+ * [9][n] neighbour table, four float4 arrays and the quantized positions,
+ * 108 bytes per particle) is
  * one fixed device array of the library per GPU, held until the process ends,
  * so n is at most PBF_NOALLOC_MAX_N (PBF_ERR_INVALID beyond). Calls that
  * overlap on one GPU share that array and race; order them on one stream. */

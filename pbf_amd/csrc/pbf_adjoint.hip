@@ -1112,8 +1112,9 @@ struct Adj : AdjBase {
     T *ob = nullptr, *lb = nullptr, *lb2 = nullptr;
     double *acc = nullptr;
     bool forwarded = false;
-    // pair lists of the plain step's kernels (FP32 handles)
+    // pair lists and neighbour records of the plain step's kernels (FP32 handles)
     uint16_t *list_entries = nullptr, *list_count = nullptr;
+    float4 *nrec = nullptr;
     // dual-number state of pbf_adjoint_jvp (FP64 handles, allocated on first use)
     T4<Dual> *jx = nullptr, *jv = nullptr, *jxs = nullptr, *jpa = nullptr, *jpb = nullptr, *jvs = nullptr,
              *jvx = nullptr, *jom = nullptr;
@@ -1178,6 +1179,7 @@ struct Adj : AdjBase {
         dfree(acc);
         dfree(list_entries);
         dfree(list_count);
+        dfree(nrec);
         dfree(jx);
         dfree(jv);
         dfree(jxs);
@@ -1245,7 +1247,8 @@ struct Adj : AdjBase {
             (e = dalloc(&acc, 2)))
             return e;
         if (std::is_same<T, float>::value &&
-            ((e = dalloc(&list_entries, (size_t)kListRows * n)) || (e = dalloc(&list_count, n))))
+            ((e = dalloc(&list_entries, (size_t)kListRows * n)) || (e = dalloc(&list_count, n)) ||
+             (e = dalloc(&nrec, n))))
             return e;
         return hipSuccess;
     }
@@ -1392,9 +1395,9 @@ struct Adj : AdjBase {
                 pl.entries = tp->ents + (size_t)it * kListRows * n;
                 pl.count = tp->cnts + (size_t)it * n;
             }
-            launch_lambda(fc, cur, qpos, nbr9, lam, pl, st, false, nullptr, &tl);
-            launch_updatepos(fc, cur, lam, nbr9, pl, nxt, qpos, st, false, nullptr, it + 1 == iters ? &up : nullptr,
-                             &tu);
+            launch_lambda(fc, cur, qpos, nbr9, lam, nrec, pl, st, false, nullptr, &tl);
+            launch_updatepos(fc, cur, lam, nrec, nbr9, pl, nxt, qpos, st, false, nullptr,
+                             it + 1 == iters ? &up : nullptr, &tu);
             cur = nxt;
         }
         if (iters == 0) launch_update(fc, cur, up.pos, up.vel, vs, st);
@@ -1408,8 +1411,8 @@ struct Adj : AdjBase {
                 pl.entries = tp->ents + (size_t)iters * kListRows * n;
                 pl.count = tp->cnts + (size_t)iters * n;
             }
-            launch_vorticity_a(fc, cur, qpos, nbr9, vs, up.vel, vx, om, o, pl, st, false);
-            launch_vorticity_b(fc, cur, nbr9, pl, vx, om, o, up.vel, st, false, nullptr, &te);
+            launch_vorticity_a(fc, cur, qpos, nbr9, vs, up.vel, vx, om, o, nrec, pl, st, false);
+            launch_vorticity_b(fc, cur, nbr9, pl, vx, om, nrec, up.vel, st, false, nullptr, &te);
         }
     }
 

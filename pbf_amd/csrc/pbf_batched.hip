@@ -40,6 +40,7 @@ struct pbf_batch {
     uint2 *qpos = nullptr;  // quantized positions of the current sorted records
     float4 *vel_sorted = nullptr, *vxsph = nullptr, *omega = nullptr;
     float *lambda = nullptr, *vort = nullptr;
+    float4 *nrec = nullptr;  // neighbour records: position and lambda, then |omega|
     uint32_t *keys = nullptr, *perm = nullptr;
     uint16_t *list_entries = nullptr, *list_count = nullptr;
     int32_t *start = nullptr, *nbr9 = nullptr;
@@ -75,6 +76,7 @@ struct pbf_batch {
         dfree(omega);
         dfree(lambda);
         dfree(vort);
+        dfree(nrec);
         dfree(keys);
         dfree(perm);
         dfree(list_entries);
@@ -113,15 +115,16 @@ struct pbf_batch {
         const float4 *cur = rec0;
         for (uint32_t it = 0; it < params.iterations; it++) {
             float4 *dst = (it & 1) ? rec_b : rec_a;
-            launch_lambda(c, d, shape, cur, qpos, nbr9, lambda, pl, st, &dense);
-            launch_updatepos(c, d, shape, cur, lambda, nbr9, pl, dst, qpos, st, &dense,
+            launch_lambda(c, d, shape, cur, qpos, nbr9, lambda, nrec, pl, st, &dense);
+            launch_updatepos(c, d, shape, cur, lambda, nrec, nbr9, pl, dst, qpos, st, &dense,
                              it + 1 == params.iterations ? &up : nullptr);
             cur = dst;
         }
         if (params.iterations == 0) launch_update(c, d, shape, cur, pos, vel, up.vel_sorted, st);
         if (params.vorticity) {
-            launch_vorticity_a(c, d, shape, cur, qpos, nbr9, vel_sorted, vel, vxsph, omega, vort, pl, st, &dense);
-            launch_vorticity_b(c, d, shape, cur, nbr9, pl, vxsph, omega, vort, vel, st, &dense);
+            launch_vorticity_a(c, d, shape, cur, qpos, nbr9, vel_sorted, vel, vxsph, omega, vort, nrec, pl, st,
+                               &dense);
+            launch_vorticity_b(c, d, shape, cur, nbr9, pl, vxsph, omega, nrec, vel, st, &dense);
         }
     }
 };
@@ -232,6 +235,7 @@ int pbf_batch_create(pbf_batch **out, uint32_t nscenes, const pbf_scene_desc *sc
     TRY(dalloc(&b->omega, n));
     TRY(dalloc(&b->lambda, n));
     TRY(dalloc(&b->vort, n));
+    TRY(dalloc(&b->nrec, n));
     TRY(dalloc(&b->keys, n));
     TRY(dalloc(&b->perm, n));
     TRY(dalloc(&b->list_entries, (size_t)kListRows * n));

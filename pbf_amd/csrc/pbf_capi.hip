@@ -223,6 +223,7 @@ struct pbf_sph {
     DenseList dense;                             // the step's particles with the most candidates
     float4 *vel_sorted = nullptr, *vxsph = nullptr, *omega = nullptr;
     float *lambda = nullptr, *vort = nullptr;
+    float4 *nrec = nullptr;                      // neighbour records: position and lambda, then |omega|
     uint32_t *keys = nullptr, *perm = nullptr;
     uint32_t *highlight = nullptr;               // highlight words by id (SPH::GetHighlightBuffer)
     CellTables cells_t;
@@ -268,6 +269,7 @@ struct pbf_sph {
         dfree(omega);
         dfree(lambda);
         dfree(vort);
+        dfree(nrec);
         dfree(keys);
         dfree(perm);
         dfree(highlight);
@@ -394,8 +396,9 @@ int pbf_sph_create(pbf_sph **out, uint32_t n, int32_t gx, int32_t gy, int32_t gz
     TRY(dalloc(&s->pos, n));
     TRY(dalloc(&s->vel, n));
     TRY(dalloc(&s->xs, n));
-    // sorted records, lambda and |omega| keep kPad zero entries after the n
-    // particles (the solver kernels read them for indices past n)
+    // sorted records, lambda, |omega| and the neighbour records keep kPad zero
+    // entries after the n particles (the solver kernels read them for indices
+    // past n)
     TRY(dalloc(&s->rec0, (size_t)n + kPad));
     TRY(dalloc(&s->rec_a, (size_t)n + kPad));
     TRY(dalloc(&s->rec_b, (size_t)n + kPad));
@@ -413,6 +416,7 @@ int pbf_sph_create(pbf_sph **out, uint32_t n, int32_t gx, int32_t gy, int32_t gz
     TRY(dalloc(&s->omega, n));
     TRY(dalloc(&s->lambda, (size_t)n + kPad));
     TRY(dalloc(&s->vort, (size_t)n + kPad));
+    TRY(dalloc(&s->nrec, (size_t)n + kPad));
     TRY(dalloc(&s->keys, n));
     TRY(dalloc(&s->perm, n));
     TRY(dalloc(&s->highlight, n));
@@ -430,6 +434,7 @@ int pbf_sph_create(pbf_sph **out, uint32_t n, int32_t gx, int32_t gy, int32_t gz
     TRY(hipMemset(s->qpos, 0, ((size_t)n + kPad) * sizeof(uint2)));
     TRY(hipMemset(s->lambda, 0, ((size_t)n + kPad) * sizeof(float)));
     TRY(hipMemset(s->vort, 0, ((size_t)n + kPad) * sizeof(float)));
+    TRY(hipMemset(s->nrec, 0, ((size_t)n + kPad) * sizeof(float4)));
     TRY(hipDeviceSynchronize());
 #undef TRY
     *out = s;
@@ -534,9 +539,10 @@ int sph_enqueue(pbf_sph *s, hipStream_t st, bool events) {
     for (uint32_t it = 0; it < iters; it++) {
         float4 *dst = (it & 1) ? s->rec_b : s->rec_a;
         TIMED(KID_LAMBDA,
-              launch_lambda(c, cur, s->qpos, t.nbr9, s->lambda, pl, st, true, &s->dense, nullptr, s->tiny + it));
-        TIMED(KID_UPDATEPOS, launch_updatepos(c, cur, s->lambda, t.nbr9, pl, dst, s->qpos, st, true, &s->dense,
-                                              it + 1 == iters ? &up : nullptr, nullptr, s->tiny + it,
+              launch_lambda(c, cur, s->qpos, t.nbr9, s->lambda, s->nrec, pl, st, true, &s->dense, nullptr,
+                            s->tiny + it));
+        TIMED(KID_UPDATEPOS, launch_updatepos(c, cur, s->lambda, s->nrec, t.nbr9, pl, dst, s->qpos, st, true,
+                                              &s->dense, it + 1 == iters ? &up : nullptr, nullptr, s->tiny + it,
                                               s->tiny + it + 1));
         cur = dst;
     }
@@ -546,10 +552,10 @@ int sph_enqueue(pbf_sph *s, hipStream_t st, bool events) {
     if (iters == 0) TIMED(KID_UPDATE, launch_update(c, s->rec, s->pos, s->vel, up.vel_sorted, st));
     if (s->params.vorticity) {
         TIMED(KID_VORTICITY_A,
-              launch_vorticity_a(c, s->rec, s->qpos, t.nbr9, s->vel_sorted, s->vel, s->vxsph, s->omega, s->vort, pl,
-                                 st, true, &s->dense, s->tiny + iters));
+              launch_vorticity_a(c, s->rec, s->qpos, t.nbr9, s->vel_sorted, s->vel, s->vxsph, s->omega, s->vort,
+                                 s->nrec, pl, st, true, &s->dense, s->tiny + iters));
         TIMED(KID_VORTICITY_B,
-              launch_vorticity_b(c, s->rec, t.nbr9, pl, s->vxsph, s->omega, s->vort, s->vel, st, true, &s->dense,
+              launch_vorticity_b(c, s->rec, t.nbr9, pl, s->vxsph, s->omega, s->nrec, s->vel, st, true, &s->dense,
                                  nullptr, s->tiny + iters));
     }
 #undef TIMED
@@ -1005,23 +1011,24 @@ static int nbr_scratch(uint32_t n, const void *nbr12, int32_t **nbr9, hipStream_
 
 // The two vorticity passes of the stage on caller memory, with the [9][n]
 // table in nbr9 (already converted) and kVortStageBytes per particle of
-// scratch in tmp: three float4 arrays, then the quantized positions.
+// scratch in tmp: four float4 arrays (sorted velocities, XSPH velocities,
+// omega, neighbour records), then the quantized positions.
 static void launch_vorticity_stage(uint32_t n, const int32_t grid[3], const pbf_params &p, const void *rec4,
                                    void *vel4, void *vort_out, const int32_t *nbr9, float4 *tmp, hipStream_t st) {
     const SolverConsts c = make_consts(n, grid, p, key_bits_of(grid));
     const float4 *rec = (const float4 *)rec4;
     float4 *vel = (float4 *)vel4;
-    float4 *vs = tmp, *vx = tmp + n, *om = tmp + 2 * (size_t)n;
-    uint2 *q = reinterpret_cast<uint2 *>(tmp + 3 * (size_t)n);
+    float4 *vs = tmp, *vx = tmp + n, *om = tmp + 2 * (size_t)n, *nrec = tmp + 3 * (size_t)n;
+    uint2 *q = reinterpret_cast<uint2 *>(tmp + 4 * (size_t)n);
     // sorted copy of the velocities, vs[g] = vel[id(rec[g])] (SPH::Run keeps
     // one from the update pass)
     launch_gather_by_id((int32_t)n, rec, vel, vs, st);
     launch_quantize(c, rec, q, (int32_t)n, st);
-    launch_vorticity_a(c, rec, q, nbr9, vs, vel, vx, om, (float *)vort_out, PairList(), st);
-    launch_vorticity_b(c, rec, nbr9, PairList(), vx, om, (const float *)vort_out, vel, st);
+    launch_vorticity_a(c, rec, q, nbr9, vs, vel, vx, om, (float *)vort_out, nrec, PairList(), st);
+    launch_vorticity_b(c, rec, nbr9, PairList(), vx, om, nrec, vel, st);
 }
 
-constexpr size_t kVortStageBytes = 3 * sizeof(float4) + sizeof(uint2);
+constexpr size_t kVortStageBytes = 4 * sizeof(float4) + sizeof(uint2);
 
 static int release_scratch(void *p, hipStream_t st) {
     hipError_t e = hipStreamSynchronize(st);
@@ -1040,16 +1047,20 @@ int pbf_stage_lambda(uint32_t n, const int32_t grid[3], const pbf_params *p, con
     int32_t *nbr9 = nullptr;
     if ((rc = nbr_scratch(n, nbr12, &nbr9, st))) return rc;
     const SolverConsts c = make_consts(n, grid, *p, key_bits_of(grid));
-    uint2 *q = nullptr;  // quantized positions of the caller's records
-    const hipError_t e = dalloc(&q, n);
+    uint2 *q = nullptr;        // quantized positions of the caller's records
+    float4 *nrec = nullptr;    // neighbour records the pass writes beside lambda (unused here)
+    hipError_t e = dalloc(&q, n);
+    if (e == hipSuccess) e = dalloc(&nrec, n);
     if (e != hipSuccess) {
         (void)hipFree(nbr9);
+        (void)hipFree(q);
         return hip_fail(e, "pbf_stage_lambda");
     }
     launch_quantize(c, (const float4 *)rec4, q, (int32_t)n, st);
-    launch_lambda(c, (const float4 *)rec4, q, nbr9, (float *)lambda_out, PairList(), st);
+    launch_lambda(c, (const float4 *)rec4, q, nbr9, (float *)lambda_out, nrec, PairList(), st);
     rc = release_scratch(nbr9, st);
     (void)hipFree(q);
+    (void)hipFree(nrec);
     return rc;
 }
 
@@ -1063,9 +1074,18 @@ int pbf_stage_updatepos(uint32_t n, const int32_t grid[3], const pbf_params *p, 
     int32_t *nbr9 = nullptr;
     if ((rc = nbr_scratch(n, nbr12, &nbr9, st))) return rc;
     const SolverConsts c = make_consts(n, grid, *p, key_bits_of(grid));
-    launch_updatepos(c, (const float4 *)rec4_in, (const float *)lambda, nbr9, PairList(), (float4 *)rec4_out,
+    float4 *nrec = nullptr;  // the caller's positions with their lambda, read per neighbour
+    const hipError_t e = dalloc(&nrec, n);
+    if (e != hipSuccess) {
+        (void)hipFree(nbr9);
+        return hip_fail(e, "pbf_stage_updatepos");
+    }
+    launch_pack_records((int32_t)n, (const float4 *)rec4_in, (const float *)lambda, nrec, st);
+    launch_updatepos(c, (const float4 *)rec4_in, (const float *)lambda, nrec, nbr9, PairList(), (float4 *)rec4_out,
                      nullptr, st);
-    return release_scratch(nbr9, st);
+    rc = release_scratch(nbr9, st);
+    (void)hipFree(nrec);
+    return rc;
 }
 
 int pbf_stage_update(uint32_t n, const pbf_params *p, const void *rec4, void *pos4, void *vel4, void *stream) {
@@ -1085,7 +1105,7 @@ int pbf_stage_vorticity(uint32_t n, const int32_t grid[3], const pbf_params *p, 
     if (!rec4 || !nbr12 || !vel4 || !vort_out) return invalid("null pointer");
     hipStream_t st = (hipStream_t)stream;
     int32_t *nbr9 = nullptr;
-    float4 *tmp = nullptr;  // sorted velocities, XSPH velocities, omega, quantized positions
+    float4 *tmp = nullptr;  // sorted velocities, XSPH velocities, omega, neighbour records, quantized positions
     if ((rc = nbr_scratch(n, nbr12, &nbr9, st))) return rc;
     hipError_t e = dalloc(&tmp, ((size_t)kVortStageBytes * n + sizeof(float4) - 1) / sizeof(float4));
     if (e != hipSuccess) {
@@ -1104,8 +1124,8 @@ int pbf_stage_vorticity(uint32_t n, const int32_t grid[3], const pbf_params *p, 
 
 // ------------------------------------------------------ vorticity, no allocation
 // Scratch of pbf_stage_vorticity_noalloc: the [9][n] neighbour table (36 B per
-// particle, then aligned) followed by the sorted, XSPH and omega velocities
-// and the quantized positions (56 B per particle), for at most
+// particle, then aligned) followed by the sorted, XSPH and omega velocities,
+// the neighbour records and the quantized positions (72 B per particle), for at most
 // PBF_NOALLOC_MAX_N particles. Each GPU the
 // library runs on holds one copy until the process ends. In host code the
 // name `scratch` is a host-side stand-in, not a device address; the address

@@ -200,9 +200,13 @@ struct SolverTape {
 // tiny (or null): the flag word of rec's positions; when it is zero and h = 1,
 // the pair passes skip the division's range tests (every operand is inside
 // the fast ranges). Writers of positions set the flag of their output set.
+// nrec: neighbour records for the replaying pass, nrec[i] = (position of
+// rec[i], the pass's per-neighbour operand): lambda_i from launch_lambda,
+// |omega_i| from launch_vorticity_a. The replaying pass reads its neighbours
+// from it, one 16-byte load per pair. Padded like rec (the padding zero).
 void launch_lambda(const SolverConsts &c, const float4 *rec, const uint2 *q, const int32_t *nbr9, float *lambda,
-                   const PairList &pl, hipStream_t st, bool padded = false, const DenseList *dl = nullptr,
-                   const SolverTape *tape = nullptr, const uint32_t *tiny = nullptr);
+                   float4 *nrec, const PairList &pl, hipStream_t st, bool padded = false,
+                   const DenseList *dl = nullptr, const SolverTape *tape = nullptr, const uint32_t *tiny = nullptr);
 // The step's closing update (update.glsl:16-28: velocities from the
 // positions before and after the step, both stored by particle id; vel_sorted,
 // if set, also keeps the velocities in sorted order). Given to the last
@@ -214,8 +218,9 @@ struct StepUpdate {
     float4 *vel_sorted = nullptr;
 };
 // q_out: the quantized positions of rec_out, written beside it (null: none).
-void launch_updatepos(const SolverConsts &c, const float4 *rec_in, const float *lambda, const int32_t *nbr9,
-                      const PairList &pl, float4 *rec_out, uint2 *q_out, hipStream_t st, bool padded = false,
+void launch_updatepos(const SolverConsts &c, const float4 *rec_in, const float *lambda, const float4 *nrec,
+                      const int32_t *nbr9, const PairList &pl, float4 *rec_out, uint2 *q_out, hipStream_t st,
+                      bool padded = false,
                       const DenseList *dl = nullptr, const StepUpdate *up = nullptr,
                       const SolverTape *tape = nullptr, const uint32_t *tiny_in = nullptr,
                       uint32_t *tiny_out = nullptr);
@@ -223,12 +228,15 @@ void launch_update(const SolverConsts &c, const float4 *rec, float4 *pos, float4
                    hipStream_t st);
 void launch_vorticity_a(const SolverConsts &c, const float4 *rec, const uint2 *q, const int32_t *nbr9,
                         const float4 *vel_sorted, const float4 *vel_id, float4 *vxsph, float4 *omega, float *vort,
-                        const PairList &pl, hipStream_t st, bool padded = false, const DenseList *dl = nullptr,
-                        const uint32_t *tiny = nullptr);
+                        float4 *nrec, const PairList &pl, hipStream_t st, bool padded = false,
+                        const DenseList *dl = nullptr, const uint32_t *tiny = nullptr);
+// (x, y, z of rec[i], v[i]) to out[i], i < count: the neighbour records of a
+// replaying pass from a caller's records and per-particle operand.
+void launch_pack_records(int32_t count, const float4 *rec, const float *v, float4 *out, hipStream_t st);
 // q[i] = the quantized position of rec[i], i < count.
 void launch_quantize(const SolverConsts &c, const float4 *rec, uint2 *q, int32_t count, hipStream_t st);
 void launch_vorticity_b(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, const PairList &pl,
-                        const float4 *vxsph, const float4 *omega, const float *vort, float4 *vel_id, hipStream_t st,
+                        const float4 *vxsph, const float4 *omega, const float4 *nrec, float4 *vel_id, hipStream_t st,
                         bool padded = false, const DenseList *dl = nullptr, const SolverTape *tape = nullptr,
                         const uint32_t *tiny = nullptr);
 // Launch shape of the batched form: workgroup size of the neighbour passes
@@ -254,20 +262,20 @@ void launch_neighbourcells(const SolverConsts &c, const DomMany &d, const Launch
                            const int32_t *start, const unsigned long long *end64, int32_t *nbr9, hipStream_t st,
                            const DenseList *dl = nullptr);
 void launch_lambda(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec,
-                   const uint2 *q, const int32_t *nbr9, float *lambda, const PairList &pl, hipStream_t st,
+                   const uint2 *q, const int32_t *nbr9, float *lambda, float4 *nrec, const PairList &pl, hipStream_t st,
                    const DenseList *dl = nullptr);
 void launch_updatepos(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec_in,
-                      const float *lambda, const int32_t *nbr9, const PairList &pl, float4 *rec_out, uint2 *q_out,
-                      hipStream_t st, const DenseList *dl = nullptr, const StepUpdate *up = nullptr);
+                      const float *lambda, const float4 *nrec, const int32_t *nbr9, const PairList &pl, float4 *rec_out,
+                      uint2 *q_out, hipStream_t st, const DenseList *dl = nullptr, const StepUpdate *up = nullptr);
 void launch_update(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec, float4 *pos,
                    float4 *vel, float4 *vel_sorted, hipStream_t st);
 void launch_vorticity_a(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec,
                         const uint2 *q, const int32_t *nbr9, const float4 *vel_sorted, const float4 *vel_id,
-                        float4 *vxsph, float4 *omega, float *vort, const PairList &pl, hipStream_t st,
+                        float4 *vxsph, float4 *omega, float *vort, float4 *nrec, const PairList &pl, hipStream_t st,
                         const DenseList *dl = nullptr);
 void launch_vorticity_b(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec,
                         const int32_t *nbr9, const PairList &pl, const float4 *vxsph, const float4 *omega,
-                        const float *vort, float4 *vel_id, hipStream_t st, const DenseList *dl = nullptr);
+                        const float4 *nrec, float4 *vel_id, hipStream_t st, const DenseList *dl = nullptr);
 void launch_nbr9_to_12(int32_t n, const int32_t *nbr9, int32_t *nbr12, hipStream_t st);
 void launch_nbr12_to_9(int32_t n, const int32_t *nbr12, int32_t *nbr9, hipStream_t st);
 void launch_end64_to_i32(size_t cells, const unsigned long long *end64, int32_t *end, hipStream_t st);

@@ -169,6 +169,15 @@ __global__ __launch_bounds__(BLOCK) void k_gather_by_id(int32_t n, const float4 
     out[g] = src[id_of(rec[g])];
 }
 
+// Neighbour records of a replaying pass: (x, y, z of rec[g], v[g]).
+__global__ __launch_bounds__(BLOCK) void k_pack_records(int32_t n, const float4 *__restrict__ rec,
+                                                        const float *__restrict__ v, float4 *__restrict__ out) {
+    const int32_t g = blockIdx.x * BLOCK + threadIdx.x;
+    if (g >= n) return;
+    const float4 r = rec[g];
+    out[g] = make_float4(r.x, r.y, r.z, v[g]);
+}
+
 __global__ __launch_bounds__(BLOCK) void k_fill_i32(int32_t *__restrict__ p, int32_t v, size_t count) {
     const size_t stride = (size_t)gridDim.x * BLOCK * 4;
     for (size_t i = ((size_t)blockIdx.x * BLOCK + threadIdx.x) * 4; i < count; i += stride) {
@@ -410,12 +419,6 @@ __device__ __forceinline__ float4 rec_at(const float4 *__restrict__ rec, const S
     const float4 r = rec[sg.off + min(j, sg.n - 1)];
     const int32_t m = j < sg.n ? -1 : 0;
     return make_float4(mask_bits(r.x, m), mask_bits(r.y, m), mask_bits(r.z, m), mask_bits(r.w, m));
-}
-template <bool PAD>
-__device__ __forceinline__ float val_at(const float *__restrict__ a, const Seg &sg, int32_t j) {
-    if (PAD) return a[j];
-    const float v = a[sg.off + min(j, sg.n - 1)];
-    return mask_bits(v, j < sg.n ? -1 : 0);
 }
 
 // Neighbour iteration in two phases. FOR_EACH_NEIGHBOUR (foreachneighbour.glsl:
@@ -953,6 +956,7 @@ struct LambdaPass {
     const SolverConsts &c;
     const float4 *__restrict__ rec;
     float *__restrict__ lambda;
+    float4 *__restrict__ nrec;  // neighbour records of the position update
     SolverTape tp;
     bool tame;  // positions_tame: the division's range tests can be skipped
     float4 pi;
@@ -980,7 +984,9 @@ struct LambdaPass {
     __device__ __forceinline__ void finish(int32_t i, const Seg &, const float *a) const {
         const float sk = a[1] + sq_len(a[2], a[3], a[4]);
         const float ci = a[0] * c.inv_rho0 - 1.0f;
-        lambda[i] = -ci / (sk + c.eps);
+        const float l = -ci / (sk + c.eps);
+        lambda[i] = l;
+        nrec[i] = make_float4(pi.x, pi.y, pi.z, l);
         if (tp.sden) {
             tp.sden[i] = sk + c.eps;
             tp.G[i] = make_float4(a[2], a[3], a[4], 0.0f);
@@ -1019,8 +1025,10 @@ struct UpdatePosPass {
         pi = rec_in[i];
         li = lambda[i];
     }
-    __device__ __forceinline__ float extra(const Seg &sg, int32_t j) const { return val_at<PAD>(lambda, sg, j); }
-    __device__ __forceinline__ bool eval(const float4 &pj, float lj, bool listed, float *P, float *) const {
+    // neighbours come from the lambda pass's records: lambda_j in pj.w
+    __device__ __forceinline__ int extra(const Seg &, int32_t) const { return 0; }
+    __device__ __forceinline__ bool eval(const float4 &pj, int, bool listed, float *P, float *) const {
+        const float lj = pj.w;
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const float sq = sq_len(dx, dy, dz);
         if (!listed && sq > c.h2cut) return false;
@@ -1066,6 +1074,7 @@ struct VorticityAPass {
     float4 *__restrict__ vxsph;
     float4 *__restrict__ omega;
     float *__restrict__ vort;
+    float4 *__restrict__ nrec;  // neighbour records of the confinement pass
     bool tame;
     float4 pi, vi;
     __device__ __forceinline__ void begin(int32_t i, const Seg &) {
@@ -1100,7 +1109,9 @@ struct VorticityAPass {
     __device__ __forceinline__ void finish(int32_t i, const Seg &sg, const float *a) const {
         vxsph[i] = make_float4(vi.x + sg.xsph_c * a[0], vi.y + sg.xsph_c * a[1], vi.z + sg.xsph_c * a[2], 0.0f);
         omega[i] = make_float4(a[3], a[4], a[5], 0.0f);
-        vort[i] = sqrtf(sq_len(a[3], a[4], a[5]));
+        const float o = sqrtf(sq_len(a[3], a[4], a[5]));
+        vort[i] = o;
+        nrec[i] = make_float4(pi.x, pi.y, pi.z, o);
     }
 };
 
@@ -1113,14 +1124,15 @@ struct VorticityBPass {
     const float4 *__restrict__ rec;
     const float4 *__restrict__ vxsph;
     const float4 *__restrict__ omega;
-    const float *__restrict__ vort;
     float4 *__restrict__ vel_id;
     SolverTape tp;
     bool tame;
     float4 pi;
     __device__ __forceinline__ void begin(int32_t i, const Seg &) { pi = rec[i]; }
-    __device__ __forceinline__ float extra(const Seg &sg, int32_t j) const { return val_at<PAD>(vort, sg, j); }
-    __device__ __forceinline__ bool eval(const float4 &pj, float om, bool listed, float *P, float *) const {
+    // neighbours come from the first half's records: |omega_j| in pj.w
+    __device__ __forceinline__ int extra(const Seg &, int32_t) const { return 0; }
+    __device__ __forceinline__ bool eval(const float4 &pj, int, bool listed, float *P, float *) const {
+        const float om = pj.w;
         const float px = pi.x - pj.x, py = pi.y - pj.y, pz = pi.z - pj.z;
         const float sq = sq_len(px, py, pz);
         if (!listed && sq > c.h2cut) return false;
@@ -1163,6 +1175,7 @@ template <bool H1, int NB, bool PAD, class Dom, int LC = LCAP>
 __global__ __launch_bounds__(NB) void k_lambda(SolverConsts c, Dom dom, const float4 *__restrict__ rec,
                                                      const uint2 *__restrict__ qpos,
                                                      const int32_t *__restrict__ nbr9, float *__restrict__ lambda,
+                                                     float4 *__restrict__ nrec,
                                                      uint16_t *__restrict__ list_entries,
                                                      uint16_t *__restrict__ list_count, int32_t list_cap,
                                                      DenseList dl, uint32_t dense_blocks, SolverTape tp, uint32_t ntiles,
@@ -1178,7 +1191,7 @@ __global__ __launch_bounds__(NB) void k_lambda(SolverConsts c, Dom dom, const fl
     auto tile = [&](const uint32_t bid) {
         {
             if (bid < dense_blocks) {
-                LambdaPass<H1, PAD> ps{c, rec, lambda, tp, tame, {}};
+                LambdaPass<H1, PAD> ps{c, rec, lambda, nrec, tp, tame, {}};
                 dense_pass<LambdaPass<H1, PAD>, PAD, NB / 64>(c, dom, rec, nbr9, dl, list_entries, list_count, list_cap, ps,
                                                      u.dense, bid, dense_blocks);
                 return;
@@ -1215,7 +1228,9 @@ __global__ __launch_bounds__(NB) void k_lambda(SolverConsts c, Dom dom, const fl
         out.finish(i);
         sk += sq_len(gix, giy, giz);
         const float ci = rho * c.inv_rho0 - 1.0f;
-        lambda[i] = -ci / (sk + c.eps);
+        const float l = -ci / (sk + c.eps);
+        lambda[i] = l;
+        nrec[i] = make_float4(pi.x, pi.y, pi.z, l);
         if (tp.sden) {
             tp.sden[i] = sk + c.eps;
             tp.G[i] = make_float4(gix, giy, giz, 0.0f);
@@ -1231,11 +1246,14 @@ __global__ __launch_bounds__(NB) void k_lambda(SolverConsts c, Dom dom, const fl
     }
 }
 
-// updatepos.glsl:42-105, race-free: neighbours read from rec_in, result to
-// rec_out; replays the lambda pass's neighbour list (same positions).
+// updatepos.glsl:42-105, race-free: neighbours read from the lambda pass's
+// records nrec (the positions of rec_in with lambda in w, one load per
+// neighbour), result to rec_out; replays the lambda pass's neighbour list
+// (same positions).
 template <bool H1, int NB, bool PAD, class Dom>
 __global__ __launch_bounds__(NB) void k_updatepos(SolverConsts c, Dom dom, const float4 *__restrict__ rec_in,
                                                         const float *__restrict__ lambda,
+                                                        const float4 *__restrict__ nrec,
                                                         const int32_t *__restrict__ nbr9,
                                                         const uint16_t *__restrict__ list_entries,
                                                         const uint16_t *__restrict__ list_count,
@@ -1256,7 +1274,7 @@ __global__ __launch_bounds__(NB) void k_updatepos(SolverConsts c, Dom dom, const
         {
             if (bid < dense_blocks) {
                 UpdatePosPass<H1, PAD> ps{c, rec_in, lambda, rec_out, q_out, up, tp, tame, tiny_out, {}, 0.0f};
-                dense_pass<UpdatePosPass<H1, PAD>, PAD, NB / 64>(c, dom, rec_in, nbr9, dl, const_cast<uint16_t *>(list_entries),
+                dense_pass<UpdatePosPass<H1, PAD>, PAD, NB / 64>(c, dom, nrec, nbr9, dl, const_cast<uint16_t *>(list_entries),
                                                         const_cast<uint16_t *>(list_count), 0, ps, u.dense, bid,
                                                         dense_blocks);
                 return;
@@ -1270,9 +1288,9 @@ __global__ __launch_bounds__(NB) void k_updatepos(SolverConsts c, Dom dom, const
         const float4 pi = rec_in[i];
         const float li = lambda[i];
         float dpx = 0.0f, dpy = 0.0f, dpz = 0.0f;
-        replay_neighbours<NB, PAD>(c, sg, rec_in, list_entries, list_count, i, pi, lds,
-                          [&](int32_t j) { return val_at<PAD>(lambda, sg, j); },
-                          [&](int32_t, const float4 &pj, float lj, bool listed) {
+        replay_neighbours<NB, PAD>(c, sg, nrec, list_entries, list_count, i, pi, lds, [](int32_t) { return 0; },
+                          [&](int32_t, const float4 &pj, int, bool listed) {
+            const float lj = pj.w;
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const float sq = sq_len(dx, dy, dz);
             if (!listed && sq > c.h2cut) return;
@@ -1332,7 +1350,8 @@ __global__ __launch_bounds__(NB) void k_vorticity_a(SolverConsts c, Dom dom, con
                                                           const float4 *__restrict__ vel_sorted,
                                                           const float4 *__restrict__ vel_id,
                                                           float4 *__restrict__ vxsph, float4 *__restrict__ omega,
-                                                          float *__restrict__ vort, uint16_t *__restrict__ list_entries,
+                                                          float *__restrict__ vort, float4 *__restrict__ nrec,
+                                                          uint16_t *__restrict__ list_entries,
                                                           uint16_t *__restrict__ list_count, int32_t list_cap,
                                                           DenseList dl, uint32_t dense_blocks, uint32_t ntiles,
                                                           const uint32_t *__restrict__ tiny) {
@@ -1347,7 +1366,7 @@ __global__ __launch_bounds__(NB) void k_vorticity_a(SolverConsts c, Dom dom, con
     auto tile = [&](const uint32_t bid) {
         {
             if (bid < dense_blocks) {
-                VorticityAPass<H1, PAD> ps{c, rec, vel_sorted, vel_id, vxsph, omega, vort, tame, {}, {}};
+                VorticityAPass<H1, PAD> ps{c, rec, vel_sorted, vel_id, vxsph, omega, vort, nrec, tame, {}, {}};
                 dense_pass<VorticityAPass<H1, PAD>, PAD, NB / 64>(c, dom, rec, nbr9, dl, list_entries, list_count, list_cap, ps,
                                                          u.dense, bid, dense_blocks);
                 return;
@@ -1387,7 +1406,9 @@ __global__ __launch_bounds__(NB) void k_vorticity_a(SolverConsts c, Dom dom, con
         out.finish(i);
         vxsph[i] = make_float4(vi.x + sg.xsph_c * sx, vi.y + sg.xsph_c * sy, vi.z + sg.xsph_c * sz, 0.0f);
         omega[i] = make_float4(wx, wy, wz, 0.0f);
-        vort[i] = sqrtf(sq_len(wx, wy, wz));
+        const float o = sqrtf(sq_len(wx, wy, wz));
+        vort[i] = o;
+        nrec[i] = make_float4(pi.x, pi.y, pi.z, o);
     };
     if (std::is_same<Dom, DomOne>::value || gridDim.x >= ntiles) {
         tile(blockIdx.x);  // one tile per workgroup (always so for one scene)
@@ -1399,7 +1420,8 @@ __global__ __launch_bounds__(NB) void k_vorticity_a(SolverConsts c, Dom dom, con
     }
 }
 
-// vorticity.glsl:65-85 (second half): confinement from this step's |omega|;
+// vorticity.glsl:65-85 (second half): confinement from this step's |omega|,
+// read with the neighbour's position from the first half's records nrec;
 // replays the first half's neighbour list.
 template <bool H1, int NB, bool PAD, class Dom>
 __global__ __launch_bounds__(NB) void k_vorticity_b(SolverConsts c, Dom dom, const float4 *__restrict__ rec,
@@ -1408,7 +1430,7 @@ __global__ __launch_bounds__(NB) void k_vorticity_b(SolverConsts c, Dom dom, con
                                                           const uint16_t *__restrict__ list_count,
                                                           const float4 *__restrict__ vxsph,
                                                           const float4 *__restrict__ omega,
-                                                          const float *__restrict__ vort,
+                                                          const float4 *__restrict__ nrec,
                                                           float4 *__restrict__ vel_id, DenseList dl,
                                                           uint32_t dense_blocks, SolverTape tp, uint32_t ntiles,
                                                           const uint32_t *__restrict__ tiny) {
@@ -1423,8 +1445,8 @@ __global__ __launch_bounds__(NB) void k_vorticity_b(SolverConsts c, Dom dom, con
     auto tile = [&](const uint32_t bid) {
         {
             if (bid < dense_blocks) {
-                VorticityBPass<H1, PAD> ps{c, rec, vxsph, omega, vort, vel_id, tp, tame, {}};
-                dense_pass<VorticityBPass<H1, PAD>, PAD, NB / 64>(c, dom, rec, nbr9, dl, const_cast<uint16_t *>(list_entries),
+                VorticityBPass<H1, PAD> ps{c, rec, vxsph, omega, vel_id, tp, tame, {}};
+                dense_pass<VorticityBPass<H1, PAD>, PAD, NB / 64>(c, dom, nrec, nbr9, dl, const_cast<uint16_t *>(list_entries),
                                                          const_cast<uint16_t *>(list_count), 0, ps, u.dense, bid,
                                                          dense_blocks);
                 return;
@@ -1437,9 +1459,9 @@ __global__ __launch_bounds__(NB) void k_vorticity_b(SolverConsts c, Dom dom, con
         const Seg sg = dom.at(i);
         const float4 pi = rec[i];
         float nx = 0.0f, ny = 0.0f, nz = 0.0f;
-        replay_neighbours<NB, PAD>(c, sg, rec, list_entries, list_count, i, pi, lds,
-                          [&](int32_t j) { return val_at<PAD>(vort, sg, j); },
-                          [&](int32_t, const float4 &pj, float om, bool listed) {
+        replay_neighbours<NB, PAD>(c, sg, nrec, list_entries, list_count, i, pi, lds, [](int32_t) { return 0; },
+                          [&](int32_t, const float4 &pj, int, bool listed) {
+            const float om = pj.w;
             const float px = pi.x - pj.x, py = pi.y - pj.y, pz = pi.z - pj.z;
             const float sq = sq_len(px, py, pz);
             if (!listed && sq > c.h2cut) return;
@@ -1665,7 +1687,7 @@ void update_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, con
 }
 template <class Dom>
 void lambda_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, bool pad, const float4 *rec,
-                 const uint2 *q, const int32_t *nbr9, float *lambda, const PairList &pl, hipStream_t st,
+                 const uint2 *q, const int32_t *nbr9, float *lambda, float4 *nrec, const PairList &pl, hipStream_t st,
                  const DenseList *dl = nullptr, const SolverTape *tape = nullptr, const uint32_t *tiny = nullptr) {
     if (c.n <= 0) return;
     const DenseList dlv = dl ? *dl : DenseList();
@@ -1680,18 +1702,18 @@ void lambda_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, boo
             if (c.h_is_one) {
                 auto k = k_lambda<true, NB, PAD, Dom, LC>;
                 k<<<neighbour_grid(k, NB, ntiles, ls.persistent), NB, 0, st>>>(
-                    c, d, rec, q, nbr9, lambda, pl.entries, pl.count, pl.cap, dlv, db, tp, ntiles, tiny);
+                    c, d, rec, q, nbr9, lambda, nrec, pl.entries, pl.count, pl.cap, dlv, db, tp, ntiles, tiny);
             } else {
                 auto k = k_lambda<false, NB, PAD, Dom, LC>;
                 k<<<neighbour_grid(k, NB, ntiles, ls.persistent), NB, 0, st>>>(
-                    c, d, rec, q, nbr9, lambda, pl.entries, pl.count, pl.cap, dlv, db, tp, ntiles, tiny);
+                    c, d, rec, q, nbr9, lambda, nrec, pl.entries, pl.count, pl.cap, dlv, db, tp, ntiles, tiny);
             }
         });
     });
 }
 template <class Dom>
 void updatepos_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, bool pad, const float4 *rec_in,
-                    const float *lambda, const int32_t *nbr9, const PairList &pl, float4 *rec_out, uint2 *q_out,
+                    const float *lambda, const float4 *nrec, const int32_t *nbr9, const PairList &pl, float4 *rec_out, uint2 *q_out,
                     hipStream_t st, const DenseList *dl = nullptr, const StepUpdate *up = nullptr,
                     const SolverTape *tape = nullptr, const uint32_t *tiny_in = nullptr,
                     uint32_t *tiny_out = nullptr) {
@@ -1707,12 +1729,12 @@ void updatepos_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, 
         if (c.h_is_one) {
             auto k = k_updatepos<true, NB, PAD, Dom>;
             k<<<neighbour_grid(k, NB, ntiles, ls.persistent), NB, 0, st>>>(
-                c, d, rec_in, lambda, nbr9, pl.entries, pl.count, rec_out, q_out, dlv, db, upv, tp, ntiles, tiny_in,
+                c, d, rec_in, lambda, nrec, nbr9, pl.entries, pl.count, rec_out, q_out, dlv, db, upv, tp, ntiles, tiny_in,
                 tiny_out);
         } else {
             auto k = k_updatepos<false, NB, PAD, Dom>;
             k<<<neighbour_grid(k, NB, ntiles, ls.persistent), NB, 0, st>>>(
-                c, d, rec_in, lambda, nbr9, pl.entries, pl.count, rec_out, q_out, dlv, db, upv, tp, ntiles, tiny_in,
+                c, d, rec_in, lambda, nrec, nbr9, pl.entries, pl.count, rec_out, q_out, dlv, db, upv, tp, ntiles, tiny_in,
                 tiny_out);
         }
     });
@@ -1720,7 +1742,8 @@ void updatepos_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, 
 template <class Dom>
 void vorticity_a_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, bool pad, const float4 *rec,
                       const uint2 *q, const int32_t *nbr9, const float4 *vel_sorted, const float4 *vel_id, float4 *vxsph,
-                      float4 *omega, float *vort, const PairList &pl, hipStream_t st, const DenseList *dl = nullptr, const uint32_t *tiny = nullptr) {
+                      float4 *omega, float *vort, float4 *nrec, const PairList &pl, hipStream_t st,
+                      const DenseList *dl = nullptr, const uint32_t *tiny = nullptr) {
     if (c.n <= 0) return;
     const DenseList dlv = dl ? *dl : DenseList();
     with_nb<Dom>(ls.nb, pad, [&](auto nbc, auto padc) {
@@ -1733,12 +1756,12 @@ void vorticity_a_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls
             if (c.h_is_one) {
                 auto k = k_vorticity_a<true, NB, PAD, Dom, LC>;
                 k<<<neighbour_grid(k, NB, ntiles, ls.persistent), NB, 0, st>>>(
-                    c, d, rec, q, nbr9, vel_sorted, vel_id, vxsph, omega, vort, pl.entries, pl.count, pl.cap, dlv, db,
+                    c, d, rec, q, nbr9, vel_sorted, vel_id, vxsph, omega, vort, nrec, pl.entries, pl.count, pl.cap, dlv, db,
                     ntiles, tiny);
             } else {
                 auto k = k_vorticity_a<false, NB, PAD, Dom, LC>;
                 k<<<neighbour_grid(k, NB, ntiles, ls.persistent), NB, 0, st>>>(
-                    c, d, rec, q, nbr9, vel_sorted, vel_id, vxsph, omega, vort, pl.entries, pl.count, pl.cap, dlv, db,
+                    c, d, rec, q, nbr9, vel_sorted, vel_id, vxsph, omega, vort, nrec, pl.entries, pl.count, pl.cap, dlv, db,
                     ntiles, tiny);
             }
         });
@@ -1747,7 +1770,7 @@ void vorticity_a_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls
 template <class Dom>
 void vorticity_b_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls, bool pad, const float4 *rec,
                       const int32_t *nbr9, const PairList &pl, const float4 *vxsph, const float4 *omega,
-                      const float *vort, float4 *vel_id, hipStream_t st, const DenseList *dl = nullptr,
+                      const float4 *nrec, float4 *vel_id, hipStream_t st, const DenseList *dl = nullptr,
                       const SolverTape *tape = nullptr, const uint32_t *tiny = nullptr) {
     if (c.n <= 0) return;
     const DenseList dlv = dl ? *dl : DenseList();
@@ -1760,11 +1783,11 @@ void vorticity_b_impl(const SolverConsts &c, const Dom &d, const LaunchShape &ls
         if (c.h_is_one) {
             auto k = k_vorticity_b<true, NB, PAD, Dom>;
             k<<<neighbour_grid(k, NB, ntiles, ls.persistent), NB, 0, st>>>(
-                c, d, rec, nbr9, pl.entries, pl.count, vxsph, omega, vort, vel_id, dlv, db, tp, ntiles, tiny);
+                c, d, rec, nbr9, pl.entries, pl.count, vxsph, omega, nrec, vel_id, dlv, db, tp, ntiles, tiny);
         } else {
             auto k = k_vorticity_b<false, NB, PAD, Dom>;
             k<<<neighbour_grid(k, NB, ntiles, ls.persistent), NB, 0, st>>>(
-                c, d, rec, nbr9, pl.entries, pl.count, vxsph, omega, vort, vel_id, dlv, db, tp, ntiles, tiny);
+                c, d, rec, nbr9, pl.entries, pl.count, vxsph, omega, nrec, vel_id, dlv, db, tp, ntiles, tiny);
         }
     });
 }
@@ -1783,6 +1806,9 @@ void launch_gather_records(int32_t n, const uint32_t *perm, const float4 *in, fl
 }
 void launch_gather_by_id(int32_t n, const float4 *rec, const float4 *src, float4 *out, hipStream_t st) {
     if (n > 0) k_gather_by_id<<<blocks_for(n), BLOCK, 0, st>>>(n, rec, src, out);
+}
+void launch_pack_records(int32_t n, const float4 *rec, const float *v, float4 *out, hipStream_t st) {
+    if (n > 0) k_pack_records<<<blocks_for(n), BLOCK, 0, st>>>(n, rec, v, out);
 }
 void launch_fill_i32(int32_t *p, int32_t value, size_t count, hipStream_t st) {
     if (count > 0) k_fill_i32<<<grid_stride_blocks((count + 3) / 4), BLOCK, 0, st>>>(p, value, count);
@@ -1803,15 +1829,15 @@ void launch_neighbourcells(const SolverConsts &c, const float4 *sorted, const in
     neighbourcells_impl(c, one(c), kDefaultShape, sorted, start, end64, nbr9, st, dl, highlight);
 }
 void launch_lambda(const SolverConsts &c, const float4 *rec, const uint2 *q, const int32_t *nbr9, float *lambda,
-                   const PairList &pl, hipStream_t st, bool padded, const DenseList *dl, const SolverTape *tape,
-                   const uint32_t *tiny) {
-    lambda_impl(c, one(c), kDefaultShape, padded, rec, q, nbr9, lambda, pl, st, dl, tape, tiny);
+                   float4 *nrec, const PairList &pl, hipStream_t st, bool padded, const DenseList *dl,
+                   const SolverTape *tape, const uint32_t *tiny) {
+    lambda_impl(c, one(c), kDefaultShape, padded, rec, q, nbr9, lambda, nrec, pl, st, dl, tape, tiny);
 }
-void launch_updatepos(const SolverConsts &c, const float4 *rec_in, const float *lambda, const int32_t *nbr9,
-                      const PairList &pl, float4 *rec_out, uint2 *q_out, hipStream_t st, bool padded,
-                      const DenseList *dl, const StepUpdate *up, const SolverTape *tape, const uint32_t *tiny_in,
-                      uint32_t *tiny_out) {
-    updatepos_impl(c, one(c), kDefaultShape, padded, rec_in, lambda, nbr9, pl, rec_out, q_out, st, dl, up, tape,
+void launch_updatepos(const SolverConsts &c, const float4 *rec_in, const float *lambda, const float4 *nrec,
+                      const int32_t *nbr9, const PairList &pl, float4 *rec_out, uint2 *q_out, hipStream_t st,
+                      bool padded, const DenseList *dl, const StepUpdate *up, const SolverTape *tape,
+                      const uint32_t *tiny_in, uint32_t *tiny_out) {
+    updatepos_impl(c, one(c), kDefaultShape, padded, rec_in, lambda, nrec, nbr9, pl, rec_out, q_out, st, dl, up, tape,
                    tiny_in, tiny_out);
 }
 void launch_update(const SolverConsts &c, const float4 *rec, float4 *pos, float4 *vel, float4 *vel_sorted,
@@ -1820,17 +1846,18 @@ void launch_update(const SolverConsts &c, const float4 *rec, float4 *pos, float4
 }
 void launch_vorticity_a(const SolverConsts &c, const float4 *rec, const uint2 *q, const int32_t *nbr9,
                         const float4 *vel_sorted, const float4 *vel_id, float4 *vxsph, float4 *omega, float *vort,
-                        const PairList &pl, hipStream_t st, bool padded, const DenseList *dl, const uint32_t *tiny) {
-    vorticity_a_impl(c, one(c), kDefaultShape, padded, rec, q, nbr9, vel_sorted, vel_id, vxsph, omega, vort, pl, st,
-                     dl, tiny);
+                        float4 *nrec, const PairList &pl, hipStream_t st, bool padded, const DenseList *dl,
+                        const uint32_t *tiny) {
+    vorticity_a_impl(c, one(c), kDefaultShape, padded, rec, q, nbr9, vel_sorted, vel_id, vxsph, omega, vort, nrec, pl,
+                     st, dl, tiny);
 }
 void launch_quantize(const SolverConsts &c, const float4 *rec, uint2 *q, int32_t count, hipStream_t st) {
     if (count > 0) k_quantize<<<blocks_for(count), BLOCK, 0, st>>>(c.qscale, rec, q, count);
 }
 void launch_vorticity_b(const SolverConsts &c, const float4 *rec, const int32_t *nbr9, const PairList &pl,
-                        const float4 *vxsph, const float4 *omega, const float *vort, float4 *vel_id, hipStream_t st,
+                        const float4 *vxsph, const float4 *omega, const float4 *nrec, float4 *vel_id, hipStream_t st,
                         bool padded, const DenseList *dl, const SolverTape *tape, const uint32_t *tiny) {
-    vorticity_b_impl(c, one(c), kDefaultShape, padded, rec, nbr9, pl, vxsph, omega, vort, vel_id, st, dl, tape, tiny);
+    vorticity_b_impl(c, one(c), kDefaultShape, padded, rec, nbr9, pl, vxsph, omega, nrec, vel_id, st, dl, tape, tiny);
 }
 
 // many scenes
@@ -1848,14 +1875,14 @@ void launch_neighbourcells(const SolverConsts &c, const DomMany &d, const Launch
     neighbourcells_impl(c, d, ls, sorted, start, end64, nbr9, st, dl);
 }
 void launch_lambda(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec,
-                   const uint2 *q, const int32_t *nbr9, float *lambda, const PairList &pl, hipStream_t st,
+                   const uint2 *q, const int32_t *nbr9, float *lambda, float4 *nrec, const PairList &pl, hipStream_t st,
                    const DenseList *dl) {
-    lambda_impl(c, d, ls, false, rec, q, nbr9, lambda, pl, st, dl);
+    lambda_impl(c, d, ls, false, rec, q, nbr9, lambda, nrec, pl, st, dl);
 }
 void launch_updatepos(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec_in,
-                      const float *lambda, const int32_t *nbr9, const PairList &pl, float4 *rec_out, uint2 *q_out,
-                      hipStream_t st, const DenseList *dl, const StepUpdate *up) {
-    updatepos_impl(c, d, ls, false, rec_in, lambda, nbr9, pl, rec_out, q_out, st, dl, up);
+                      const float *lambda, const float4 *nrec, const int32_t *nbr9, const PairList &pl, float4 *rec_out,
+                      uint2 *q_out, hipStream_t st, const DenseList *dl, const StepUpdate *up) {
+    updatepos_impl(c, d, ls, false, rec_in, lambda, nrec, nbr9, pl, rec_out, q_out, st, dl, up);
 }
 void launch_update(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec, float4 *pos,
                    float4 *vel, float4 *vel_sorted, hipStream_t st) {
@@ -1863,14 +1890,14 @@ void launch_update(const SolverConsts &c, const DomMany &d, const LaunchShape &l
 }
 void launch_vorticity_a(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec,
                         const uint2 *q, const int32_t *nbr9, const float4 *vel_sorted, const float4 *vel_id,
-                        float4 *vxsph, float4 *omega, float *vort, const PairList &pl, hipStream_t st,
+                        float4 *vxsph, float4 *omega, float *vort, float4 *nrec, const PairList &pl, hipStream_t st,
                         const DenseList *dl) {
-    vorticity_a_impl(c, d, ls, false, rec, q, nbr9, vel_sorted, vel_id, vxsph, omega, vort, pl, st, dl);
+    vorticity_a_impl(c, d, ls, false, rec, q, nbr9, vel_sorted, vel_id, vxsph, omega, vort, nrec, pl, st, dl);
 }
 void launch_vorticity_b(const SolverConsts &c, const DomMany &d, const LaunchShape &ls, const float4 *rec,
                         const int32_t *nbr9, const PairList &pl, const float4 *vxsph, const float4 *omega,
-                        const float *vort, float4 *vel_id, hipStream_t st, const DenseList *dl) {
-    vorticity_b_impl(c, d, ls, false, rec, nbr9, pl, vxsph, omega, vort, vel_id, st, dl);
+                        const float4 *nrec, float4 *vel_id, hipStream_t st, const DenseList *dl) {
+    vorticity_b_impl(c, d, ls, false, rec, nbr9, pl, vxsph, omega, nrec, vel_id, st, dl);
 }
 void launch_nbr9_to_12(int32_t n, const int32_t *nbr9, int32_t *nbr12, hipStream_t st) {
     if (n > 0) k_nbr9_to_12<<<blocks_for(n), BLOCK, 0, st>>>(n, nbr9, nbr12);
