@@ -53,10 +53,10 @@ ALG_BYTES = {
 # last updatepos launch of each step also performs the closing update (pos and
 # vel by particle id), so updatepos is the mean over launches with and without.
 MEASURED_TRAFFIC_BLOCK_1M = {
-    "lambda": 166.7,
-    "updatepos": 226.5,
-    "vorticity_a": 245.2,
-    "vorticity_b": 239.9,
+    "lambda": 183.6,
+    "updatepos": 237.2,
+    "vorticity_a": 261.9,
+    "vorticity_b": 245.4,
     "neighbourcells": 71.0,
     "predict": 60.0,
 }
