@@ -966,7 +966,7 @@ struct LambdaPass {
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const float sq = sq_len(dx, dy, dz);
         if (sq > c.h2cut) return false;
-        const float r = sqrt_rn_finite(sq);
+        const float r = sqrt_rn_dist(sq, tame);
         if (r > c.h) return false;
         P[0] = wpoly6<H1>(c, sq, r);
         if (r == 0.0f) return true;
@@ -1032,7 +1032,7 @@ struct UpdatePosPass {
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const float sq = sq_len(dx, dy, dz);
         if (!listed && sq > c.h2cut) return false;
-        const float r = sqrt_rn_finite(sq);
+        const float r = sqrt_rn_dist(sq, tame);
         if ((!listed && r > c.h) || r == 0.0f) return false;
         float sc = c.scale * wpoly6<H1>(c, sq, r);
         sc *= sc;
@@ -1088,7 +1088,7 @@ struct VorticityAPass {
         const float px = pi.x - pj.x, py = pi.y - pj.y, pz = pi.z - pj.z;
         const float sq = sq_len(px, py, pz);
         if (sq > c.h2cut) return false;
-        const float r = sqrt_rn_finite(sq);
+        const float r = sqrt_rn_dist(sq, tame);
         if (r > c.h) return false;
         const float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
         const float w = wpoly6<H1>(c, sq, r);
@@ -1136,7 +1136,7 @@ struct VorticityBPass {
         const float px = pi.x - pj.x, py = pi.y - pj.y, pz = pi.z - pj.z;
         const float sq = sq_len(px, py, pz);
         if (!listed && sq > c.h2cut) return false;
-        const float r = sqrt_rn_finite(sq);
+        const float r = sqrt_rn_dist(sq, tame);
         if ((!listed && r > c.h) || r == 0.0f) return false;
         float gx, gy, gz;
         grad_spiky<H1>(c, px, py, pz, r, gx, gy, gz, tame);
@@ -1210,7 +1210,7 @@ __global__ __launch_bounds__(NB) void k_lambda(SolverConsts c, Dom dom, const fl
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const float sq = sq_len(dx, dy, dz);
             if (sq > c.h2cut) return;
-            const float r = sqrt_rn_finite(sq);
+            const float r = sqrt_rn_dist(sq, tame);
             if (r > c.h) return;
             out.add(i, c.n, code);
             rho += wpoly6<H1>(c, sq, r);
@@ -1294,7 +1294,7 @@ __global__ __launch_bounds__(NB) void k_updatepos(SolverConsts c, Dom dom, const
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const float sq = sq_len(dx, dy, dz);
             if (!listed && sq > c.h2cut) return;
-            const float r = sqrt_rn_finite(sq);
+            const float r = sqrt_rn_dist(sq, tame);
             if ((!listed && r > c.h) || r == 0.0f) return;
             float sc = c.scale * wpoly6<H1>(c, sq, r);
             sc *= sc;
@@ -1388,7 +1388,7 @@ __global__ __launch_bounds__(NB) void k_vorticity_a(SolverConsts c, Dom dom, con
             const float px = pi.x - pj.x, py = pi.y - pj.y, pz = pi.z - pj.z;
             const float sq = sq_len(px, py, pz);
             if (sq > c.h2cut) return;
-            const float r = sqrt_rn_finite(sq);
+            const float r = sqrt_rn_dist(sq, tame);
             if (r > c.h) return;
             out.add(i, c.n, code);
             const float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
@@ -1465,7 +1465,7 @@ __global__ __launch_bounds__(NB) void k_vorticity_b(SolverConsts c, Dom dom, con
             const float px = pi.x - pj.x, py = pi.y - pj.y, pz = pi.z - pj.z;
             const float sq = sq_len(px, py, pz);
             if (!listed && sq > c.h2cut) return;
-            const float r = sqrt_rn_finite(sq);
+            const float r = sqrt_rn_dist(sq, tame);
             if ((!listed && r > c.h) || r == 0.0f) return;
             float gx, gy, gz;
             grad_spiky<H1>(c, px, py, pz, r, gx, gy, gz, tame);

@@ -73,6 +73,22 @@ __device__ __forceinline__ float sqrt_rn_finite(float x) {
     return r;
 }
 
+// sqrt_rn_finite for squared distances of positions that may be tame: with
+// TAME set every operand is 0 or at least 2^-96 (the squares of coordinate
+// differences that are 0 or at least 2^-48), and the fast form alone is
+// exact for both (at 0 both neighbours' residuals fail their tests, and the
+// hardware root of 0 is returned).
+__device__ __forceinline__ float sqrt_rn_dist(float x, bool tame) {
+    float r = sqrt_rn_fast(x);
+    if (!tame) {
+        if (lanes(x >= kSqrtLo) != active_lanes()) {
+            asm volatile("" ::: "memory");
+            if (!(x >= kSqrtLo)) r = sqrtf(x);
+        }
+    }
+    return r;
+}
+
 // RN(1 / b) for |b| in [2^-60, 2^60): the hardware reciprocal refined by one
 // Newton step in FMA form. The self-test checks it against the IEEE division
 // for every float of that range.
