@@ -53,10 +53,10 @@ ALG_BYTES = {
 # last updatepos launch of each step also performs the closing update (pos and
 # vel by particle id), so updatepos is the mean over launches with and without.
 MEASURED_TRAFFIC_BLOCK_1M = {
-    "lambda": 183.6,
+    "lambda": 183.8,
     "updatepos": 237.2,
-    "vorticity_a": 261.9,
-    "vorticity_b": 245.4,
+    "vorticity_a": 262.1,
+    "vorticity_b": 245.1,
     "neighbourcells": 71.0,
     "predict": 60.0,
 }
