@@ -162,6 +162,26 @@ def test_block_1m_one_step(gpu_lib):
     assert np.array_equal(np.sort(ids), np.arange(sc.n)), "sorted ids are a permutation"
 
 
+def test_block_1m_graph_rollout(gpu_lib):
+    """The timed configuration itself: the 1M block's first 20 steps (the window that
+    bench.py --steps 20 times) as hipGraph replays (the lattice phase, whose positions are tame and whose neighbour
+    passes read the packed neighbour records), bit-exact against the oracle, with the
+    never-cleared end table."""
+    sc = S.cfg_block_1m()
+    steps = 20
+    say(f"[block_1m graph] n={sc.n} steps={steps}")
+    sph = P.SPH(sc.n, sc.grid)
+    sph.set_params(P.scene_params(sc))
+    sph.upload(sc.pos, sc.vel)
+    sph.set_graph(True)
+    sph.step(steps)
+    pos, vel = sph.download()
+    wpos, wvel, wend = oracle_rollout(sc, steps)
+    assert_bitexact(pos, wpos, f"block_1m positions after {steps} graph steps")
+    assert_bitexact(vel, wvel, f"block_1m velocities after {steps} graph steps")
+    assert_bitexact(sph.end_table(), wend, f"block_1m end table after {steps} graph steps")
+
+
 def test_stress_4m_one_step(gpu_lib):
     """The 4,194,304-particle stress scene (grid 384x192x384: 26-bit keys, four
     sort passes, keys above 2^24 that the reference's FP32 hash rounds; a dense
