@@ -37,6 +37,15 @@ __device__ __forceinline__ float wpoly6(const SolverConsts &c, float sq, float r
     const float w = (t * t) * (t * POLY6);
     return H1 ? w : w / c.h9;
 }
+// wpoly6 for a pair known to lie within h (r <= h): the same value without
+// the range test (pairs of a list recorded at the same positions, or pairs
+// the caller has already tested).
+template <bool H1>
+__device__ __forceinline__ float wpoly6_within(const SolverConsts &c, float sq) {
+    const float t = c.hh - sq;
+    const float w = (t * t) * (t * POLY6);
+    return H1 ? w : w / c.h9;
+}
 
 // gradWspiky (calclambda.glsl:57-64) for l = sqrt(sq) with 0 < l <= h:
 // (-3 * 15/pi * t * t) * r / (l * h*h*h*h*h*h), t = h - l; the three
@@ -1034,7 +1043,7 @@ struct UpdatePosPass {
         if (!listed && sq > c.h2cut) return false;
         const float r = sqrt_rn_dist(sq, tame);
         if ((!listed && r > c.h) || r == 0.0f) return false;
-        float sc = c.scale * wpoly6<H1>(c, sq, r);
+        float sc = c.scale * wpoly6_within<H1>(c, sq);  // r <= h here, listed or tested
         sc *= sc;
         sc *= sc;
         sc = -c.k * sc;
@@ -1296,7 +1305,7 @@ __global__ __launch_bounds__(NB) void k_updatepos(SolverConsts c, Dom dom, const
             if (!listed && sq > c.h2cut) return;
             const float r = sqrt_rn_dist(sq, tame);
             if ((!listed && r > c.h) || r == 0.0f) return;
-            float sc = c.scale * wpoly6<H1>(c, sq, r);
+            float sc = c.scale * wpoly6_within<H1>(c, sq);  // r <= h here, listed or tested
             sc *= sc;
             sc *= sc;
             sc = -c.k * sc;
