@@ -28,18 +28,11 @@ constexpr int BLOCK = 256;
 
 __device__ __forceinline__ float sq_len(float x, float y, float z) { return (z * z + y * y) + x * x; }
 
-// Wpoly6 (calclambda.glsl:41-47) of r = sqrt(sq). H1: h == 1, where the
-// reference's division by h^9 is a division by 1.
-template <bool H1>
-__device__ __forceinline__ float wpoly6(const SolverConsts &c, float sq, float r) {
-    if (r > c.h) return 0.0f;
-    const float t = c.hh - sq;
-    const float w = (t * t) * (t * POLY6);
-    return H1 ? w : w / c.h9;
-}
-// wpoly6 for a pair known to lie within h (r <= h): the same value without
-// the range test (pairs of a list recorded at the same positions, or pairs
-// the caller has already tested).
+// Wpoly6 (calclambda.glsl:41-47) of r = sqrt(sq) for a pair known to lie
+// within h (r <= h: the callers test it first, or replay a list recorded at
+// the same positions, which only holds such pairs), so the shader's
+// `r > h` branch is not taken. H1: h == 1, where the reference's division by
+// h^9 is a division by 1.
 template <bool H1>
 __device__ __forceinline__ float wpoly6_within(const SolverConsts &c, float sq) {
     const float t = c.hh - sq;
@@ -977,7 +970,7 @@ struct LambdaPass {
         if (sq > c.h2cut) return false;
         const float r = sqrt_rn_dist(sq, tame);
         if (r > c.h) return false;
-        P[0] = wpoly6<H1>(c, sq, r);
+        P[0] = wpoly6_within<H1>(c, sq);
         if (r == 0.0f) return true;
         float gx, gy, gz;
         grad_spiky<H1>(c, dx, dy, dz, r, gx, gy, gz, tame);
@@ -1100,7 +1093,7 @@ struct VorticityAPass {
         const float r = sqrt_rn_dist(sq, tame);
         if (r > c.h) return false;
         const float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
-        const float w = wpoly6<H1>(c, sq, r);
+        const float w = wpoly6_within<H1>(c, sq);
         P[0] = ux * w;
         P[1] = uy * w;
         P[2] = uz * w;
@@ -1222,7 +1215,7 @@ __global__ __launch_bounds__(NB) void k_lambda(SolverConsts c, Dom dom, const fl
             const float r = sqrt_rn_dist(sq, tame);
             if (r > c.h) return;
             out.add(i, c.n, code);
-            rho += wpoly6<H1>(c, sq, r);
+            rho += wpoly6_within<H1>(c, sq);
             if (r == 0.0f) return;
             float gx, gy, gz;
             grad_spiky<H1>(c, dx, dy, dz, r, gx, gy, gz, tame);
@@ -1401,7 +1394,7 @@ __global__ __launch_bounds__(NB) void k_vorticity_a(SolverConsts c, Dom dom, con
             if (r > c.h) return;
             out.add(i, c.n, code);
             const float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
-            const float w = wpoly6<H1>(c, sq, r);
+            const float w = wpoly6_within<H1>(c, sq);
             sx += ux * w;
             sy += uy * w;
             sz += uz * w;
