@@ -56,15 +56,26 @@ __global__ __launch_bounds__(RS_THREADS) void k_upsweep(const uint32_t *__restri
     counts[(size_t)threadIdx.x * tiles + blockIdx.x] = hist[threadIdx.x];
 }
 
+// Inclusive scan over the 64 lanes of a wave (all active) with DPP moves, no
+// LDS: Hillis-Steele within each 16-lane row (row_shr 1, 2, 4, 8; a lane whose
+// source lies before its row reads 0), then row 0's total into row 1 and row
+// 2's into row 3 (row_bcast 15 on rows 1 and 3), then the first two rows'
+// total into rows 2 and 3 (row_bcast 31 on rows 2 and 3); rows outside a row
+// mask add 0.
+__device__ __forceinline__ uint32_t wave_scan_dpp(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);   // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true);   // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true);   // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true);   // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return v;
+}
+
 // Inclusive scan of one value per thread over a 256-thread block; returns the
 // inclusive prefix, and the block total in *total (LDS scratch of 4 words).
 __device__ __forceinline__ uint32_t block_scan256(uint32_t v, uint32_t *wsum, uint32_t *total) {
-    uint32_t inc = v;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t t = __shfl_up(inc, off, 64);
-        if (lane_id() >= (unsigned)off) inc += t;
-    }
+    const uint32_t inc = wave_scan_dpp(v);
     const int w = threadIdx.x >> 6;
     if (lane_id() == 63) wsum[w] = inc;
     __syncthreads();
