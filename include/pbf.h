@@ -363,7 +363,11 @@ void *pbf_adjoint_get_stream(const pbf_adjoint *a);
 /* ---------------------------------------------------------------- stages
  * Single shader dispatches of SPH::Run on caller memory (device pointers,
  * reference layouts), so each stage can be checked alone. n is a multiple of
- * 256 (the reference's BLOCKSIZE dispatch); nbr12 is the GetResult layout. */
+ * 256 (the reference's BLOCKSIZE dispatch); nbr12 is the GetResult layout.
+ * The neighbour stages allocate their scratch (the [9][n] table, quantized
+ * positions, the packed neighbour records the pair passes read) with
+ * hipMalloc and free it before returning, which synchronizes the stream;
+ * pbf_stage_vorticity_noalloc below does neither. */
 /* predictpos.glsl:18-38: rec4_out[i] = {x + dt v', i}. */
 int pbf_stage_predict(uint32_t n, const int32_t grid[3], const pbf_params *p, const void *pos4, const void *vel4,
                       void *rec4_out, void *stream);
