@@ -406,8 +406,9 @@ __global__ __launch_bounds__(kDenseSelectThreads) void k_dense_select(DenseList 
 
 // ----------------------------------------------------------------- solver
 // Neighbour records j (scene-local) of the packed ranges; indices past the
-// scene's n (a wrapped count) read a zero record, as the reference's robust
-// buffer access does.
+// scene's n (a wrapped count) read a zero record (all four components, so a
+// packed operand in w reads 0 too), as the reference's robust buffer access
+// does.
 // PAD: the arrays hold at least kPad zero entries after the last particle
 // (one scene), so indices past n read the zero record without a test.
 // Without padding the load is of a clamped index and the zero made after it
@@ -673,8 +674,10 @@ struct PairListOut {
 // body(j, rec[j], extra(j), listed) for i's neighbours within h (listed: the
 // recording pass saw these exact positions, so the range tests are known to
 // pass) or for candidates that may be within h (scanned); j is scene-local;
-// extra(j) gathers the kernel's per-neighbour operand so that, on the list
-// path, the loads of four entries are in flight together.
+// extra(j) gathers any further per-neighbour operand, so that on the list path
+// the loads of a batch of entries are in flight together. The position update
+// and vorticity B pass the packed neighbour records as rec (the operand in w)
+// and no extra.
 template <int NB, bool PAD, class L, class Extra, class Body>
 __device__ __forceinline__ void replay_neighbours(const SolverConsts &c, const Seg &sg, const float4 *__restrict__ rec,
                                                   const uint16_t *__restrict__ entries,
