@@ -39,7 +39,7 @@ struct pbf_batch {
     float4 *pos = nullptr, *vel = nullptr, *xs = nullptr, *rec0 = nullptr, *rec_a = nullptr, *rec_b = nullptr;
     uint2 *qpos = nullptr;  // quantized positions of the current sorted records
     float4 *vel_sorted = nullptr, *vxsph = nullptr, *omega = nullptr;
-    float *lambda = nullptr, *vort = nullptr;
+    float *lambda = nullptr;
     float4 *nrec = nullptr;  // neighbour records: position and lambda, then |omega|
     uint32_t *keys = nullptr, *perm = nullptr;
     uint16_t *list_entries = nullptr, *list_count = nullptr;
@@ -75,7 +75,6 @@ struct pbf_batch {
         dfree(vxsph);
         dfree(omega);
         dfree(lambda);
-        dfree(vort);
         dfree(nrec);
         dfree(keys);
         dfree(perm);
@@ -122,7 +121,7 @@ struct pbf_batch {
         }
         if (params.iterations == 0) launch_update(c, d, shape, cur, pos, vel, up.vel_sorted, st);
         if (params.vorticity) {
-            launch_vorticity_a(c, d, shape, cur, qpos, nbr9, vel_sorted, vel, vxsph, omega, vort, nrec, pl, st,
+            launch_vorticity_a(c, d, shape, cur, qpos, nbr9, vel_sorted, vel, vxsph, omega, nullptr, nrec, pl, st,
                                &dense);
             launch_vorticity_b(c, d, shape, cur, nbr9, pl, vxsph, omega, nrec, vel, st, &dense);
         }
@@ -234,7 +233,6 @@ int pbf_batch_create(pbf_batch **out, uint32_t nscenes, const pbf_scene_desc *sc
     TRY(dalloc(&b->vxsph, n));
     TRY(dalloc(&b->omega, n));
     TRY(dalloc(&b->lambda, n));
-    TRY(dalloc(&b->vort, n));
     TRY(dalloc(&b->nrec, n));
     TRY(dalloc(&b->keys, n));
     TRY(dalloc(&b->perm, n));

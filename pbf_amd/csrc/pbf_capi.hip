@@ -222,7 +222,7 @@ struct pbf_sph {
     uint16_t *list_entries = nullptr, *list_count = nullptr;  // neighbours within h, lambda -> update
     DenseList dense;                             // the step's particles with the most candidates
     float4 *vel_sorted = nullptr, *vxsph = nullptr, *omega = nullptr;
-    float *lambda = nullptr, *vort = nullptr;
+    float *lambda = nullptr;
     float4 *nrec = nullptr;                      // neighbour records: position and lambda, then |omega|
     uint32_t *keys = nullptr, *perm = nullptr;
     uint32_t *highlight = nullptr;               // highlight words by id (SPH::GetHighlightBuffer)
@@ -268,7 +268,6 @@ struct pbf_sph {
         dfree(vxsph);
         dfree(omega);
         dfree(lambda);
-        dfree(vort);
         dfree(nrec);
         dfree(keys);
         dfree(perm);
@@ -396,9 +395,8 @@ int pbf_sph_create(pbf_sph **out, uint32_t n, int32_t gx, int32_t gy, int32_t gz
     TRY(dalloc(&s->pos, n));
     TRY(dalloc(&s->vel, n));
     TRY(dalloc(&s->xs, n));
-    // sorted records, lambda, |omega| and the neighbour records keep kPad zero
-    // entries after the n particles (the solver kernels read them for indices
-    // past n)
+    // sorted records, lambda and the neighbour records keep kPad zero entries
+    // after the n particles (the solver kernels read them for indices past n)
     TRY(dalloc(&s->rec0, (size_t)n + kPad));
     TRY(dalloc(&s->rec_a, (size_t)n + kPad));
     TRY(dalloc(&s->rec_b, (size_t)n + kPad));
@@ -415,7 +413,6 @@ int pbf_sph_create(pbf_sph **out, uint32_t n, int32_t gx, int32_t gy, int32_t gz
     TRY(dalloc(&s->vxsph, n));
     TRY(dalloc(&s->omega, n));
     TRY(dalloc(&s->lambda, (size_t)n + kPad));
-    TRY(dalloc(&s->vort, (size_t)n + kPad));
     TRY(dalloc(&s->nrec, (size_t)n + kPad));
     TRY(dalloc(&s->keys, n));
     TRY(dalloc(&s->perm, n));
@@ -433,7 +430,6 @@ int pbf_sph_create(pbf_sph **out, uint32_t n, int32_t gx, int32_t gy, int32_t gz
     TRY(hipMemset(s->rec_b, 0, ((size_t)n + kPad) * sizeof(float4)));
     TRY(hipMemset(s->qpos, 0, ((size_t)n + kPad) * sizeof(uint2)));
     TRY(hipMemset(s->lambda, 0, ((size_t)n + kPad) * sizeof(float)));
-    TRY(hipMemset(s->vort, 0, ((size_t)n + kPad) * sizeof(float)));
     TRY(hipMemset(s->nrec, 0, ((size_t)n + kPad) * sizeof(float4)));
     TRY(hipDeviceSynchronize());
 #undef TRY
@@ -552,7 +548,7 @@ int sph_enqueue(pbf_sph *s, hipStream_t st, bool events) {
     if (iters == 0) TIMED(KID_UPDATE, launch_update(c, s->rec, s->pos, s->vel, up.vel_sorted, st));
     if (s->params.vorticity) {
         TIMED(KID_VORTICITY_A,
-              launch_vorticity_a(c, s->rec, s->qpos, t.nbr9, s->vel_sorted, s->vel, s->vxsph, s->omega, s->vort,
+              launch_vorticity_a(c, s->rec, s->qpos, t.nbr9, s->vel_sorted, s->vel, s->vxsph, s->omega, nullptr,
                                  s->nrec, pl, st, true, &s->dense, s->tiny + iters));
         TIMED(KID_VORTICITY_B,
               launch_vorticity_b(c, s->rec, t.nbr9, pl, s->vxsph, s->omega, s->nrec, s->vel, st, true, &s->dense,

@@ -226,6 +226,8 @@ void launch_updatepos(const SolverConsts &c, const float4 *rec_in, const float *
                       uint32_t *tiny_out = nullptr);
 void launch_update(const SolverConsts &c, const float4 *rec, float4 *pos, float4 *vel, float4 *vel_sorted,
                    hipStream_t st);
+// vort (or null): |omega| by sorted slot (the reference's vorticities[]);
+// the second pass reads it from nrec.
 void launch_vorticity_a(const SolverConsts &c, const float4 *rec, const uint2 *q, const int32_t *nbr9,
                         const float4 *vel_sorted, const float4 *vel_id, float4 *vxsph, float4 *omega, float *vort,
                         float4 *nrec, const PairList &pl, hipStream_t st, bool padded = false,

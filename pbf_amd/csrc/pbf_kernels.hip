@@ -1078,7 +1078,7 @@ struct VorticityAPass {
     const float4 *__restrict__ vel_id;
     float4 *__restrict__ vxsph;
     float4 *__restrict__ omega;
-    float *__restrict__ vort;
+    float *__restrict__ vort;   // |omega| by sorted slot, or none
     float4 *__restrict__ nrec;  // neighbour records of the confinement pass
     bool tame;
     float4 pi, vi;
@@ -1115,7 +1115,7 @@ struct VorticityAPass {
         vxsph[i] = make_float4(vi.x + sg.xsph_c * a[0], vi.y + sg.xsph_c * a[1], vi.z + sg.xsph_c * a[2], 0.0f);
         omega[i] = make_float4(a[3], a[4], a[5], 0.0f);
         const float o = sqrtf(sq_len(a[3], a[4], a[5]));
-        vort[i] = o;
+        if (vort) vort[i] = o;
         nrec[i] = make_float4(pi.x, pi.y, pi.z, o);
     }
 };
@@ -1412,7 +1412,7 @@ __global__ __launch_bounds__(NB) void k_vorticity_a(SolverConsts c, Dom dom, con
         vxsph[i] = make_float4(vi.x + sg.xsph_c * sx, vi.y + sg.xsph_c * sy, vi.z + sg.xsph_c * sz, 0.0f);
         omega[i] = make_float4(wx, wy, wz, 0.0f);
         const float o = sqrtf(sq_len(wx, wy, wz));
-        vort[i] = o;
+        if (vort) vort[i] = o;
         nrec[i] = make_float4(pi.x, pi.y, pi.z, o);
     };
     if (std::is_same<Dom, DomOne>::value || gridDim.x >= ntiles) {
