@@ -294,8 +294,8 @@ def main():
                          "the known-size kernel that scales FETCH_SIZE / WRITE_SIZE counters to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
-        log("bench.py measures one GPU only: this build has no multi-GPU path (scenes would shard "
-            "by slabs or by scene; see DESIGN.md). Run with --gpus 1.")
+        log("bench.py measures one GPU only: the library has no multi-GPU path (a scene would shard "
+            "by slabs of the sorted cell order with halo exchange, a batch by scene). Run with --gpus 1.")
         sys.exit(2)
 
     import torch
