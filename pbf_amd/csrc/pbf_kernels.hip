@@ -410,7 +410,10 @@ __global__ __launch_bounds__(kDenseSelectThreads) void k_dense_select(DenseList 
 // packed operand in w reads 0 too), as the reference's robust buffer access
 // does.
 // PAD: the arrays hold at least kPad zero entries after the last particle
-// (one scene), so indices past n read the zero record without a test.
+// (one scene), so indices past n read the zero record without a test; the
+// index is non-negative and below 2^28, so its byte offset is formed in 32
+// bits and the load takes the uniform base in scalar registers (no 64-bit
+// address arithmetic per neighbour).
 // Without padding the load is of a clamped index and the zero made after it
 // by masking the bits (+0.0): a load under the condition would be a branch,
 // which keeps the loads of a batch from being in flight together (and the
@@ -418,7 +421,7 @@ __global__ __launch_bounds__(kDenseSelectThreads) void k_dense_select(DenseList 
 __device__ __forceinline__ float mask_bits(float v, int32_t m) { return __int_as_float(__float_as_int(v) & m); }
 template <bool PAD>
 __device__ __forceinline__ float4 rec_at(const float4 *__restrict__ rec, const Seg &sg, int32_t j) {
-    if (PAD) return rec[j];
+    if (PAD) return *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(rec) + (uint32_t)j * 16u);
     const float4 r = rec[sg.off + min(j, sg.n - 1)];
     const int32_t m = j < sg.n ? -1 : 0;
     return make_float4(mask_bits(r.x, m), mask_bits(r.y, m), mask_bits(r.z, m), mask_bits(r.w, m));
