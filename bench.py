@@ -54,9 +54,9 @@ ALG_BYTES = {
 # vel by particle id), so updatepos is the mean over launches with and without.
 MEASURED_TRAFFIC_BLOCK_1M = {
     "lambda": 183.8,
-    "updatepos": 237.2,
-    "vorticity_a": 262.1,
-    "vorticity_b": 245.1,
+    "updatepos": 237.1,
+    "vorticity_a": 257.8,
+    "vorticity_b": 245.3,
     "neighbourcells": 71.0,
     "predict": 60.0,
 }
