@@ -204,6 +204,28 @@ def test_stress_4m_one_step(gpu_lib):
     sph.close()
 
 
+def test_stress_4m_graph_rollout(gpu_lib):
+    """The 4,194,304-particle stress scene (a dense block under uniform-random particles, whose
+    densest particles take the dense-particle path) for 3 hipGraph-replayed steps, bit-exact
+    against the oracle with the never-cleared end table."""
+    sc = S.cfg_stress_4m()
+    steps = 3
+    say(f"[stress_4m graph] n={sc.n} steps={steps}")
+    sph = P.SPH(sc.n, sc.grid)
+    sph.set_params(P.scene_params(sc))
+    sph.upload(sc.pos, sc.vel)
+    sph.set_graph(True)
+    sph.step(steps)
+    pos, vel = sph.download()
+    wpos, wvel, wend = oracle_rollout(sc, steps)
+    assert_bitexact(pos, wpos, f"stress_4m positions after {steps} graph steps")
+    assert_bitexact(vel, wvel, f"stress_4m velocities after {steps} graph steps")
+    assert_bitexact(sph.end_table(), wend, f"stress_4m end table after {steps} graph steps")
+    th, cnt = sph.dense_info()
+    say(f"[stress_4m graph] dense threshold {th}, {cnt} dense particles")
+    sph.close()
+
+
 def test_error_paths(gpu_lib):
     """C-ABI errors: bad sizes and odd sort blocks fail with the reference's error kinds."""
     with pytest.raises(P._lib.PbfError):
