@@ -47,37 +47,16 @@ __device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
 __device__ __forceinline__ uint64_t lanes(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 __device__ __forceinline__ uint64_t active_lanes() { return __builtin_amdgcn_read_exec(); }
 
-// The empty volatile asm keeps the compiler from speculating the general
-// sequence into every call (it would otherwise if-convert the rare branch and
-// run both square roots for every operand).
-__device__ __forceinline__ float sqrt_rn(float x) {
-    float r = sqrt_rn_fast(x);
-    // some lane has x outside [kSqrtLo, inf): the fast lanes' masks do not
-    // cover every active lane
-    if ((lanes(x >= kSqrtLo) & lanes(x < __builtin_inff())) != active_lanes()) {
-        asm volatile("" ::: "memory");
-        if (!(x >= kSqrtLo) | !(x < __builtin_inff())) r = sqrtf(x);
-    }
-    return r;
-}
-
-// sqrt_rn for callers that know x is not +inf (a squared distance already
-// compared with a finite bound): only the lower end of the fast range is
-// tested; NaN fails that test and takes the general sequence.
-__device__ __forceinline__ float sqrt_rn_finite(float x) {
-    float r = sqrt_rn_fast(x);
-    if (lanes(x >= kSqrtLo) != active_lanes()) {
-        asm volatile("" ::: "memory");
-        if (!(x >= kSqrtLo)) r = sqrtf(x);
-    }
-    return r;
-}
-
-// sqrt_rn_finite for squared distances of positions that may be tame: with
-// TAME set every operand is 0 or at least 2^-96 (the squares of coordinate
-// differences that are 0 or at least 2^-48), and the fast form alone is
-// exact for both (at 0 both neighbours' residuals fail their tests, and the
-// hardware root of 0 is returned).
+// RN(sqrt(x)) of a squared distance x (not +inf: already compared with a
+// finite bound). Operands below the fast range (and NaN) take the general
+// sequence, in the lanes that have them, under a wave-uniform branch; the
+// empty volatile asm keeps the compiler from speculating that sequence into
+// every call (it would otherwise if-convert the rare branch and run both
+// square roots for every operand). TAME: every operand is 0 or at least 2^-96
+// (the squares of coordinate differences that are 0 or at least 2^-48), and
+// the fast form alone is exact for both (at 0 both neighbours' residuals fail
+// their tests and the hardware root of 0 is returned; the self-test checks
+// it), so the test is skipped.
 __device__ __forceinline__ float sqrt_rn_dist(float x, bool tame) {
     float r = sqrt_rn_fast(x);
     if (!tame) {
