@@ -5,6 +5,8 @@ alone, bit for bit (positions, velocities, end tables), over several steps;
 a hipGraph-captured step equals the eagerly launched one on every replay, with
 eager work and synchronisations between replays; the autotuner leaves the
 state where it found it and its choice changes no bit of the results."""
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -97,6 +99,35 @@ def test_full_256_scene_mix_against_oracle(gpu_lib):
         moved += int(np.abs(p[:, :3] - sc.pos[:, :3]).max() > 1e-3)
     say(f"[full mix] all {len(scenes)} scenes bit-exact; {moved} moved by more than 1e-3 h")
     assert moved == len(scenes)
+
+
+def test_zero_iterations_against_oracle(gpu_lib):
+    """A batch whose steps run no solver iteration (src/SPH.cpp:303-311 with the count 0): the
+    closing update reads the sorted predicted positions and the first vorticity pass alone writes
+    the neighbour records the second reads. Four scenes for 4 steps, each bit-exact against the
+    oracle's step with that scene's gravity, walls and XSPH coefficient."""
+    scenes = small_batch(nscenes=4, seed=9)
+    for sc in scenes:
+        sc.params = dataclasses.replace(sc.params, iterations=0)
+    steps = 4
+    say(f"[batched zero iterations] scenes={[s.n for s in scenes]} steps={steps}")
+    b = B.Batch(scenes)
+    b.upload()
+    b.step(steps)
+    pos, vel = b.download()
+    ends = b.end_tables()
+    b.close()
+    cell_off = np.concatenate([[0], np.cumsum([int(np.prod(s.grid)) for s in scenes])])
+    for k, sc in enumerate(scenes):
+        cfg = O.make_cfg(sc.n, sc.grid, sc.params)
+        assert cfg.iterations == 0 and cfg.vorticity == 1
+        p, v, e, vo = sc.pos, sc.vel, None, None
+        for _ in range(steps):
+            p, v, e, vo = O.step(cfg, p, v, e, vo, O.ORDER_JACOBI)
+        o = slice(b.offsets[k], b.offsets[k + 1])
+        assert same(pos[o], p), f"scene {k} (n={sc.n}) positions"
+        assert same(vel[o], v), f"scene {k} (n={sc.n}) velocities"
+        assert np.array_equal(ends[cell_off[k]:cell_off[k + 1]], e), f"scene {k} end table"
 
 
 def test_graph_replays_equal_eager_steps(gpu_lib):

@@ -6,6 +6,8 @@ stages follow the reference's operation order without fused multiply-add, so
 the race-free stages equal the fixtures bit for bit and the race-free
 replacements of the reference's in-place passes (updatepos, vorticity) equal
 the oracle's race-free order bit for bit."""
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -136,6 +138,24 @@ def test_rollout_matches_oracle(gpu_lib, name, steps):
     assert_bitexact(pos, wpos, f"{name} positions after {steps} steps")
     assert_bitexact(vel, wvel, f"{name} velocities after {steps} steps")
     assert_bitexact(sph.end_table(), wend, f"{name} end table after {steps} steps")
+
+
+@pytest.mark.parametrize("iterations,vorticity", [(0, True), (0, False), (1, True)])
+def test_iteration_counts_rollout(gpu_lib, iterations, vorticity):
+    """Zero solver iterations (the loop of src/SPH.cpp:303-311 runs no lambda or position update:
+    the closing update of src/SPH.cpp:315-318 reads the sorted predicted positions, and the first
+    vorticity pass alone writes the neighbour records the second reads) and a single one (the
+    closing update done by the only position update), on a 4,096-particle dam break for 8 steps:
+    positions, velocities and end table bit-exact against the oracle."""
+    sc = S.dam_break((16, 16, 16), (64, 32, 64), steps=8, name="iters")
+    sc.params = dataclasses.replace(sc.params, iterations=iterations, vorticity=vorticity)
+    say(f"[iterations {iterations} vorticity {vorticity}] n={sc.n}")
+    sph, pos, vel = run_sph(sc, 8)
+    wpos, wvel, wend = oracle_rollout(sc, 8)
+    assert_bitexact(pos, wpos, f"positions ({iterations} iterations)")
+    assert_bitexact(vel, wvel, f"velocities ({iterations} iterations)")
+    assert_bitexact(sph.end_table(), wend, f"end table ({iterations} iterations)")
+    assert np.abs(pos[:, :3] - sc.pos[:, :3]).max() > 1e-2
 
 
 def test_dam_32k_rollout(gpu_lib):
