@@ -11,6 +11,8 @@ The adjoint workload uses the reference's default relaxation epsilon 5
 (src/SPH.cpp:138): with 0.01 the 16,384-particle block explodes within a few
 steps, and central differences of its 32-step loss do not converge at any step
 size (the map is not usefully differentiable there)."""
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -122,6 +124,53 @@ def test_adjoint_matches_finite_differences_short(gpu_lib):
     assert err.max() < 1e-5
     assert abs(gc - fc) <= 1e-5 * abs(fc)
     assert abs(ge - fe) <= 1e-5 * abs(fe)
+
+
+def test_vorticity_off(gpu_lib):
+    """Vorticity off (the reference's default, src/SPH.cpp:25: no XSPH or confinement pass, so
+    neither coefficient enters the step): the FP32 forward equals the plain step bit for bit over
+    8 steps; over the same 8 steps the FP64 reverse pass equals forward-mode differentiation
+    (dual numbers) on 16 velocity components, and both coefficient gradients are exactly zero
+    by either mode."""
+    sc = adjoint_scene()
+    sc.params = dataclasses.replace(sc.params, vorticity=False)
+    steps = 8
+    say(f"[adjoint vorticity off] n={sc.n} steps={steps}")
+    r = A.Rollout.for_scene(sc, steps=steps, checkpoint_every=4)
+    x = r.forward(sc.pos, sc.vel)
+    v = r.velocities()
+    r.close()
+    sph = P.SPH(sc.n, sc.grid)
+    sph.set_params(P.scene_params(sc))
+    sph.upload(sc.pos, sc.vel)
+    for _ in range(steps):
+        sph.Run()
+    wp, wv = sph.download()
+    sph.close()
+    assert np.array_equal(x[:, :3].view(np.int32), wp[:, :3].view(np.int32))
+    assert np.array_equal(v[:, :3].view(np.int32), wv[:, :3].view(np.int32))
+    r = A.Rollout.for_scene(sc, steps=steps, checkpoint_every=4, fp64=True)
+    t = r.forward(sc.pos, adjoint_scene(seed=40).vel)
+    x = r.forward(sc.pos, sc.vel)
+    g = mse_grad(x, t)
+    gv, gc, ge = r.backward(g)
+    v64 = sc.vel.astype(np.float64)
+    comps = np.random.default_rng(0).choice(sc.n * 3, 16, replace=False)
+    jv = np.empty(16)
+    for k, cidx in enumerate(comps):
+        i, a = divmod(int(cidx), 3)
+        dv = np.zeros_like(v64)
+        dv[i, a] = 1.0
+        jv[k] = float(np.sum(g[:, :3] * r.jvp(sc.pos, v64, dv)[1][:, :3]))
+    zero = np.zeros_like(v64)
+    jc = float(np.sum(g[:, :3] * r.jvp(sc.pos, v64, zero, 1.0, 0.0)[1][:, :3]))
+    je = float(np.sum(g[:, :3] * r.jvp(sc.pos, v64, zero, 0.0, 1.0)[1][:, :3]))
+    r.close()
+    ad = np.array([gv[int(c) // 3, int(c) % 3] for c in comps])
+    err = np.abs(ad - jv) / np.abs(jv)
+    say(f"[adjoint vorticity off] components max rel {err.max():.2e}; c {gc} ({jc}); eps {ge} ({je})")
+    assert err.max() <= 1e-8
+    assert gc == 0.0 and ge == 0.0 and jc == 0.0 and je == 0.0
 
 
 def test_adjoint_matches_forward_mode_32_steps(gpu_lib):

@@ -140,13 +140,15 @@ def test_rollout_matches_oracle(gpu_lib, name, steps):
     assert_bitexact(sph.end_table(), wend, f"{name} end table after {steps} steps")
 
 
-@pytest.mark.parametrize("iterations,vorticity", [(0, True), (0, False), (1, True)])
+@pytest.mark.parametrize("iterations,vorticity", [(0, True), (0, False), (1, True), (2, False), (3, False)])
 def test_iteration_counts_rollout(gpu_lib, iterations, vorticity):
     """Zero solver iterations (the loop of src/SPH.cpp:303-311 runs no lambda or position update:
     the closing update of src/SPH.cpp:315-318 reads the sorted predicted positions, and the first
-    vorticity pass alone writes the neighbour records the second reads) and a single one (the
-    closing update done by the only position update), on a 4,096-particle dam break for 8 steps:
-    positions, velocities and end table bit-exact against the oracle."""
+    vorticity pass alone writes the neighbour records the second reads), a single one (the
+    closing update done by the only position update), and vorticity off with even and odd
+    counts (the reference's default, src/SPH.cpp:25: no vorticity pass, so the closing update
+    keeps no sorted velocities), on a 4,096-particle dam break for 8 steps: positions,
+    velocities and end table bit-exact against the oracle."""
     sc = S.dam_break((16, 16, 16), (64, 32, 64), steps=8, name="iters")
     sc.params = dataclasses.replace(sc.params, iterations=iterations, vorticity=vorticity)
     say(f"[iterations {iterations} vorticity {vorticity}] n={sc.n}")
