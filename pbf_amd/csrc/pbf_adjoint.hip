@@ -309,7 +309,12 @@ __global__ __launch_bounds__(AB) void ka_predict(AConsts<T> c, const T4<T> *__re
     const T4<T> p = x[i], u = v[i];
     T vx = u.x, vy = u.y, vz = u.z;
     if (c.extforce && p.z > (T)c.gz / (T)2) {
-        vz = vz + ((T)2 * c.gravity * (T)-1) * c.dt;
+        // velocity += 2 gravity (0, 0, -1) dt on every component
+        // (predictpos.glsl:26-27): the zero terms turn a -0 velocity into +0
+        const T g2 = (T)2 * c.gravity;
+        vx = vx + (g2 * (T)0) * c.dt;
+        vy = vy + (g2 * (T)0) * c.dt;
+        vz = vz + (g2 * (T)-1) * c.dt;
     }
     vx = vx + (c.gravity * c.gdir[0]) * c.dt;
     vy = vy + (c.gravity * c.gdir[1]) * c.dt;

@@ -173,6 +173,68 @@ def test_vorticity_off(gpu_lib):
     assert gc == 0.0 and ge == 0.0 and jc == 0.0 and je == 0.0
 
 
+def test_external_force(gpu_lib):
+    """The external force on (predictpos.glsl:26-27: particles above half the grid height gain
+    twice gravity along -z), the block raised by 8 h so that about half of it is pushed: the
+    FP32 forward equals the plain step bit for bit over 8 steps; the FP64 step equals the
+    oracle's FP64 replay; over 4 steps the FP64 reverse pass equals forward-mode
+    differentiation on 8 velocity components and both coefficients."""
+    sc = adjoint_scene()
+    sc.pos = sc.pos.copy()
+    sc.pos[:, 2] += np.float32(8.0)
+    sc.params = dataclasses.replace(sc.params, extforce=True)
+    above = float((sc.pos[:, 2] > sc.grid[2] / 2).mean())
+    say(f"[adjoint external force] n={sc.n}, {above:.2f} of the particles above half height")
+    assert 0.3 < above < 0.7
+    r = A.Rollout.for_scene(sc, steps=8, checkpoint_every=4)
+    x = r.forward(sc.pos, sc.vel)
+    v = r.velocities()
+    r.close()
+    sph = P.SPH(sc.n, sc.grid)
+    sph.set_params(P.scene_params(sc))
+    sph.upload(sc.pos, sc.vel)
+    for _ in range(8):
+        sph.Run()
+    wp, wv = sph.download()
+    sph.close()
+    assert np.array_equal(x[:, :3].view(np.int32), wp[:, :3].view(np.int32))
+    assert np.array_equal(v[:, :3].view(np.int32), wv[:, :3].view(np.int32))
+    r = A.Rollout.for_scene(sc, steps=1, checkpoint_every=1, fp64=True)
+    x = r.forward(sc.pos, sc.vel)
+    cfg = O.cfg_for(sc)
+    assert cfg.extforce == 1
+    _, _, _, _, st = O.step(cfg, sc.pos, sc.vel, stages=True)
+    perm = st["sorted"][:, 3].view(np.int32).copy()
+    wx, wv = O.step_fixed_f64(cfg, sc.pos, sc.vel, perm, st["nbr"])
+    assert np.abs(x[:, :3] - wx[:, :3]).max() < 1e-10
+    assert np.abs(r.velocities()[:, :3] - wv[:, :3]).max() < 1e-8
+    r.close()
+    r = A.Rollout.for_scene(sc, steps=4, checkpoint_every=2, fp64=True)
+    t = r.forward(sc.pos, adjoint_scene(seed=40).vel)
+    x = r.forward(sc.pos, sc.vel)
+    g = mse_grad(x, t)
+    gv, gc, ge = r.backward(g)
+    v64 = sc.vel.astype(np.float64)
+    comps = np.random.default_rng(1).choice(sc.n * 3, 8, replace=False)
+    jv = np.empty(8)
+    for k, cidx in enumerate(comps):
+        i, a = divmod(int(cidx), 3)
+        dv = np.zeros_like(v64)
+        dv[i, a] = 1.0
+        jv[k] = float(np.sum(g[:, :3] * r.jvp(sc.pos, v64, dv)[1][:, :3]))
+    zero = np.zeros_like(v64)
+    jc = float(np.sum(g[:, :3] * r.jvp(sc.pos, v64, zero, 1.0, 0.0)[1][:, :3]))
+    je = float(np.sum(g[:, :3] * r.jvp(sc.pos, v64, zero, 0.0, 1.0)[1][:, :3]))
+    r.close()
+    ad = np.array([gv[int(c) // 3, int(c) % 3] for c in comps])
+    err = np.abs(ad - jv) / np.abs(jv)
+    say(f"[adjoint external force] components max rel {err.max():.2e}; c {gc:.12e} vs {jc:.12e}; "
+        f"eps {ge:.12e} vs {je:.12e}")
+    assert err.max() <= 1e-8
+    assert abs(gc - jc) <= 1e-8 * abs(jc)
+    assert abs(ge - je) <= 1e-8 * abs(je)
+
+
 def test_adjoint_matches_forward_mode_32_steps(gpu_lib):
     """The workload (32 steps x 3 iterations, checkpoints every 8 steps):
     the hand-written reverse pass against forward-mode differentiation of the

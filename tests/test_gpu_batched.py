@@ -101,19 +101,21 @@ def test_full_256_scene_mix_against_oracle(gpu_lib):
     assert moved == len(scenes)
 
 
-@pytest.mark.parametrize("iterations,vorticity", [(0, True), (2, False)])
-def test_iteration_counts_against_oracle(gpu_lib, iterations, vorticity):
+@pytest.mark.parametrize("iterations,vorticity,extforce", [(0, True, False), (2, False, False), (3, True, True)])
+def test_iteration_counts_against_oracle(gpu_lib, iterations, vorticity, extforce):
     """A batch whose steps run no solver iteration (src/SPH.cpp:303-311 with the count 0: the
     closing update reads the sorted predicted positions and the first vorticity pass alone writes
     the neighbour records the second reads), and one with vorticity off (the reference's default,
     src/SPH.cpp:25: the last position update does the closing update and keeps no sorted
-    velocities). Four scenes for 4 steps, each bit-exact against the oracle's step with that
-    scene's gravity, walls and XSPH coefficient."""
+    velocities), and one with the external force on (predictpos.glsl:26-27, against each scene's
+    own grid height: about half of every scene's particles). Four scenes for 4 steps, each
+    bit-exact against the oracle's step with that scene's gravity, walls and XSPH coefficient."""
     scenes = small_batch(nscenes=4, seed=9)
     for sc in scenes:
-        sc.params = dataclasses.replace(sc.params, iterations=iterations, vorticity=vorticity)
+        sc.params = dataclasses.replace(sc.params, iterations=iterations, vorticity=vorticity, extforce=extforce)
     steps = 4
-    say(f"[batched {iterations} iterations, vorticity {vorticity}] scenes={[s.n for s in scenes]} steps={steps}")
+    say(f"[batched {iterations} iterations, vorticity {vorticity}, extforce {extforce}] "
+        f"scenes={[s.n for s in scenes]} steps={steps}")
     b = B.Batch(scenes)
     b.upload()
     b.step(steps)
@@ -123,7 +125,7 @@ def test_iteration_counts_against_oracle(gpu_lib, iterations, vorticity):
     cell_off = np.concatenate([[0], np.cumsum([int(np.prod(s.grid)) for s in scenes])])
     for k, sc in enumerate(scenes):
         cfg = O.make_cfg(sc.n, sc.grid, sc.params)
-        assert cfg.iterations == iterations and cfg.vorticity == int(vorticity)
+        assert cfg.iterations == iterations and cfg.vorticity == int(vorticity) and cfg.extforce == int(extforce)
         p, v, e, vo = sc.pos, sc.vel, None, None
         for _ in range(steps):
             p, v, e, vo = O.step(cfg, p, v, e, vo, O.ORDER_JACOBI)
