@@ -185,23 +185,33 @@ def test_block_1m_one_step(gpu_lib):
 
 
 def test_block_1m_graph_rollout(gpu_lib):
-    """The timed configuration itself: the 1M block's first 20 steps (the window that
-    bench.py --steps 20 times) as hipGraph replays (the lattice phase, whose positions are tame and whose neighbour
-    passes read the packed neighbour records), bit-exact against the oracle, with the
-    never-cleared end table."""
+    """The timed configuration itself as hipGraph replays, bit-exact against the oracle with the
+    never-cleared end table: after the first 20 steps (the window that bench.py --steps 20
+    times: the lattice phase, whose positions are tame) and after 100 (through the violent
+    collapse, where particles travel up to about 220 h and the dense-particle and list-overflow
+    paths are busiest). The oracle takes about a minute for the 100 steps on 16 threads."""
     sc = S.cfg_block_1m()
-    steps = 20
-    say(f"[block_1m graph] n={sc.n} steps={steps}")
+    cfg = O.cfg_for(sc)
+    say(f"[block_1m graph] n={sc.n} steps=100, checked at 20 and 100")
     sph = P.SPH(sc.n, sc.grid)
     sph.set_params(P.scene_params(sc))
     sph.upload(sc.pos, sc.vel)
     sph.set_graph(True)
-    sph.step(steps)
-    pos, vel = sph.download()
-    wpos, wvel, wend = oracle_rollout(sc, steps)
-    assert_bitexact(pos, wpos, f"block_1m positions after {steps} graph steps")
-    assert_bitexact(vel, wvel, f"block_1m velocities after {steps} graph steps")
-    assert_bitexact(sph.end_table(), wend, f"block_1m end table after {steps} graph steps")
+    got, done = {}, 0
+    for steps in (20, 100):
+        sph.step(steps - done)
+        done = steps
+        got[steps] = (*sph.download(), sph.end_table().copy())
+    sph.close()
+    pos, vel, end, vort = sc.pos, sc.vel, None, None
+    for s in range(1, 101):
+        pos, vel, end, vort = O.step(cfg, pos, vel, end, vort, O.ORDER_JACOBI)
+        if s in got:
+            say(f"[block_1m graph] oracle at step {s}")
+            gp, gv, ge = got[s]
+            assert_bitexact(gp, pos, f"block_1m positions after {s} graph steps")
+            assert_bitexact(gv, vel, f"block_1m velocities after {s} graph steps")
+            assert_bitexact(ge, end, f"block_1m end table after {s} graph steps")
 
 
 def test_stress_4m_one_step(gpu_lib):
