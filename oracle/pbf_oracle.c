@@ -1,7 +1,7 @@
 /* CPU restatement of the reference's simulation step: TEST INFRASTRUCTURE ONLY.
  * See pbf_oracle.h for who may use it. Built with -ffp-contract=off.
  *
- * Parity pins (tests/test_oracle_golden.py and tests/test_oracle_closed_forms.py):
+ * Parity pins (tests/test_oracle_golden.py and tests/test_oracle_invariants.py):
  *   predict, keys, sort, findcells, neighbourcells, lambda, update: bit-exact
  *   against llvmpipe runs of the reference's shaders (fixtures in tests/golden);
  *   updatepos and vorticity: bit-exact against the same runs in the
