@@ -182,7 +182,9 @@ struct DenseList {
     int32_t *threshold = nullptr;    // 2 words: the threshold, the number of particles above it
 };
 void launch_dense_select(const DenseList &dl, hipStream_t st);
-// padded: rec, lambda and vort hold kPad zero entries after the n particles.
+// padded: the arrays the neighbour loops gather from (rec, q and nrec) hold
+// kPad zero entries after the n particles, so a batch of loads that runs past
+// the end of a row stays in bounds.
 // dl: the step's dense list, whose particles the launch's first workgroups
 // handle, kDenseGroup lanes per particle. Null: every particle takes the
 // one-thread-per-particle path.
