@@ -66,12 +66,12 @@ def test_batched_equals_plain_per_scene(gpu_lib):
 
 def test_full_256_scene_mix_against_oracle(gpu_lib):
     """The whole 256-scene workload (4,277,760 particles, n from 2,048 to 65,536; composite keys of
-    8 scene bits above up to 18 cell-key bits) for 3 graph-replayed steps at the tuned launch shape:
+    8 scene bits above up to 18 cell-key bits) for 10 graph-replayed steps at the tuned launch shape:
     every scene's positions, velocities and end table bit-exact against the oracle's race-free step
     with that scene's gravity direction, walls and XSPH coefficient (the per-scene forms of
     predictpos.glsl:32, updatepos.glsl:98-100 and vorticity.glsl:57; the step src/SPH.cpp:246-334)."""
     scenes = S.batched_scenes()
-    steps = 3
+    steps = 10
     total = sum(s.n for s in scenes)
     say(f"[full mix] scenes={len(scenes)} particles={total} max n={max(s.n for s in scenes)} steps={steps}")
     assert len(scenes) == 256 and total == 4277760
